@@ -1,0 +1,121 @@
+/* kfvm.h -- C ABI of the MI355X KFVM-WENO stage library (libkfvm_amd.so).
+ *
+ * The reference exposes this path through the C++ class KFVM::Solver
+ * (src/Solver.H:54-160), constructed from a ProblemSetup parsed from an init
+ * file (src/ProblemSetup.H:142-205) and driven by src/main.cpp:23-55.  It has
+ * no FFI of its own; each entry point below replaces one member of that
+ * surface (cited per function).  Conventions:
+ *   - every function returns 0 on success and a negative code on failure;
+ *     kfvm_last_error() then describes the failure (thread-local string);
+ *   - integers are int64_t, reals are double; arrays are host memory in the
+ *     reference's plot/checkpoint order [NV][nz][ny][nx] over interior cells;
+ *   - one solver owns one HIP stream on the current device; calls on one
+ *     solver must be serialised, different solvers may run concurrently.
+ */
+#ifndef KFVM_H
+#define KFVM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kfvm_solver kfvm_solver;
+
+/* Step outcome codes (TSStatus, src/Types.H:43, plus the two terminal
+ * conditions Solver::TakeStep reports, src/Solver.cpp:332-340). */
+enum {
+  KFVM_ACCEPTED = 0,
+  KFVM_TOLERANCE = 1,
+  KFVM_UNPHYSICAL = 2,
+  KFVM_REJECT_LIMIT = 3,
+  KFVM_STAGNATED = 4
+};
+
+typedef struct kfvm_info {
+  int64_t dim, nq, nv, nx, ny, nz, ncell;
+  int64_t nsub, nfaces, nqface, ncells_stencil, nindic; /* weight-table shape */
+  int64_t rk_stages;
+  double dx, dy, dz, gamma;
+} kfvm_info;
+
+/* Solver::Solver(ProblemSetup&) after ProblemSetup::setFromFile
+ * (src/Solver.cpp:36-81, src/ProblemSetup.H:142-205): parse the init file
+ * (reals with std::stof semantics), build the GP weight tables, fill the
+ * initial condition of the registry problem named by problem:name, prime the
+ * FSAL register and choose dt0 = 0.01 cfl dmin / vmax. */
+int kfvm_create(const char *init_path, kfvm_solver **out);
+/* Same, from the text of an init file. */
+int kfvm_create_from_text(const char *init_text, kfvm_solver **out);
+int kfvm_destroy(kfvm_solver *s);
+
+/* Solver::Solve (src/Solver.cpp:84-140): step until final time / maxsteps. */
+int kfvm_solve(kfvm_solver *s, int64_t *steps_taken);
+/* Solver::TakeStep (src/Solver.cpp:188-343); *status gets a KFVM_* code. */
+int kfvm_take_step(kfvm_solver *s, int *status);
+/* Stage-granular fixed-dt advance: exactly nstages RK stages (RHS evaluation
+ * plus register update, Solver.cpp:225-258) of steps of size dt, queued on
+ * the solver's stream without host synchronisation; every completed step is
+ * accepted.  No reference counterpart: it is the benchmark driver of the
+ * per-stage update and follows TakeStep's stage sequence. */
+int kfvm_advance_stages(kfvm_solver *s, int64_t nstages, double dt);
+/* Solver::evalRHS (src/Solver.cpp:410-530) on the current solution
+ * (which = 0) or the step-start solution (which = 1); the result becomes the
+ * FSAL register.  *max_vel gets the largest wave speed over all faces. */
+int kfvm_eval_rhs(kfvm_solver *s, int which, double *max_vel);
+
+/* Solution / RHS / WENO-flag access (U_halo interior, RHS, wenoFlagView). */
+int kfvm_get_state(kfvm_solver *s, double *host, int64_t n);
+int kfvm_set_state(kfvm_solver *s, const double *host, int64_t n);
+int kfvm_get_rhs(kfvm_solver *s, double *host, int64_t n);
+/* flags: [2][nz][ny][nx], slot 0 WENO flag, slot 1 positivity theta */
+int kfvm_get_flags(kfvm_solver *s, double *host, int64_t n);
+int kfvm_set_flags(kfvm_solver *s, const double *host, int64_t n);
+
+int kfvm_get_time(kfvm_solver *s, double *t, double *dt, int64_t *step);
+int kfvm_set_time(kfvm_solver *s, double t, double dt);
+int kfvm_get_info(kfvm_solver *s, kfvm_info *info);
+/* Counters printed at the end of Solver::Solve (src/Solver.cpp:129-137). */
+int kfvm_get_counters(kfvm_solver *s, int64_t *rhs_evals, int64_t *rejects_tol,
+                      int64_t *rejects_unphys, int64_t *last_nweno);
+/* Force the next RHS evaluation to run KFVM-WENO on every cell, as the
+ * reference does after an unphysical rejection (src/Solver.cpp:298, 437-443). */
+int kfvm_force_full_weno(kfvm_solver *s);
+
+/* Stencil::faceWeights [nsub][2*dim][Q][C] and Stencil::derivWeights
+ * [nsub][I][C] (src/stencil/Stencil.H:95-103).  set_weights replaces the
+ * tables the stage kernels read (e.g. with a committed table). */
+int kfvm_get_weights(kfvm_solver *s, double *face, int64_t nface, double *deriv, int64_t nderiv);
+int kfvm_set_weights(kfvm_solver *s, const double *face, int64_t nface, const double *deriv,
+                     int64_t nderiv);
+
+/* Per-kernel device timing with HIP events on the solver's stream.  While
+ * enabled, every launch is bracketed by events; kfvm_profile_get returns the
+ * accumulated milliseconds and launch count of one kernel family
+ * ("linear", "select", "weno", "pp", "flux_rk", "stage_pre"). */
+int kfvm_profile_enable(kfvm_solver *s, int on);
+int kfvm_profile_get(kfvm_solver *s, const char *name, double *ms, int64_t *count);
+int kfvm_profile_reset(kfvm_solver *s);
+int kfvm_sync(kfvm_solver *s);
+
+/* ---- host-only entry points (no device work; usable without a GPU) ---- */
+
+/* Shapes of the weight tables for (dim, nq): nface = nsub*2*dim*Q*C,
+ * nderiv = nsub*I*C. */
+int kfvm_weight_sizes(int dim, int nq, int64_t *nface, int64_t *nderiv);
+/* Stencil::findWeights (src/stencil/Stencil.cpp:281-846): the GP
+ * reconstruction weights for length scale lfac and cell sizes dx, dy, dz. */
+int kfvm_build_weights(int dim, int nq, double lfac, double dx, double dy, double dz,
+                       double *face, int64_t nface, double *deriv, int64_t nderiv);
+/* ProblemSetup::setFromFile on text (src/ProblemSetup.H:142-205): writes the
+ * parsed configuration as a JSON object into json[len]. */
+int kfvm_parse_init(const char *init_text, char *json, int64_t len);
+
+const char *kfvm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

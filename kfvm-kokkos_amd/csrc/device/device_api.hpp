@@ -1,0 +1,63 @@
+// Host-callable launchers of the stage kernels and the plain-data structs they
+// take by value.  No torch or framework types: raw device pointers, sizes and a
+// HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace kfvm {
+
+// Grid and physics constants passed by value to every stage kernel.
+struct Grid {
+  int nx, ny, nz;    // nz = 1 in 2D
+  int ncell;
+  int bc[6];         // per face W,E,S,N,B,T: 0 periodic, 1 outflow, 2 reflecting, 3 noslip
+  unsigned negmask[6];  // variables negated by the mirror at that face (reflecting / noslip)
+  double dx, dy, dz;
+  double gamma;
+  double wt[3];      // 1D Gauss weights on (-1/2, 1/2)
+  double vel[3];     // linear advection velocity
+  int haveVel;       // 0: velocity (cos gamma, sin gamma, 0) with no z flux
+  int rs;            // 0 LLF, 1 HLL, 2 HLLC, 3 exact upwind
+};
+
+enum RKMode : int { RK_PRIME = 0, RK_STAGE = 1, RK_FSAL = 2 };
+
+// Register-update epilogue of the flux kernel.
+struct RKArgs {
+  int mode;
+  double g1, g2, g3, delta, betaDt, bhatDt;
+  double atol, rtol;
+  double *K;            // RHS register (written in PRIME and FSAL modes)
+  const double *Uprev;  // step start state
+  double *Uhat;         // embedded solution
+  double *Utmp;         // 3S* register or nullptr
+  double *Unext;        // next stage state (STAGE mode)
+  double *errPartial;   // per-block error sums (FSAL mode)
+  double *posPartial;   // per-block positivity minima (FSAL mode)
+  unsigned long long *vmaxBits;  // running max wave speed (bit pattern of a non-negative double)
+};
+
+hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
+                         const Grid &g, const double *W0, hipStream_t s);
+hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
+                         double *wflag, int *list, int *counter, double wThresh, const Grid &g,
+                         hipStream_t s);
+hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, const Grid &g,
+                       const double *FW, const double *DW, const int *list, const int *counter,
+                       hipStream_t s);
+hipError_t launch_pp(int dim, int nq, const double *U, double *FS, double *theta, const Grid &g,
+                     hipStream_t s);
+hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
+                          const Grid &g, const RKArgs &a, hipStream_t s);
+int flux_rk_partials(int64_t ncell);
+hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double *U, double *Uhat,
+                            double *Utmp, double betaDt, double bhatDt, int ncell, hipStream_t s);
+hipError_t launch_finalize(const double *errPartial, const double *posPartial, int nPart,
+                           double *out, hipStream_t s);
+hipError_t launch_fill(double *p, double v, size_t n, hipStream_t s);
+
+}  // namespace kfvm

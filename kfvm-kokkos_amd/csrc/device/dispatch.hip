@@ -1,0 +1,74 @@
+// Run-time selection of the kernel instantiation and the shape-independent
+// small kernels (RK stage start, reductions, fills).
+#define KFVM_SMALL_KERNELS
+#include "launch_impl.hpp"
+
+namespace kfvm {
+
+#define KFVM_SELECT(dim, nq, CALL)                          \
+  do {                                                      \
+    if (dim == 3 && nq == 2) { using L = Launch<3, 2>; return CALL; } \
+    if (dim == 3 && nq == 3) { using L = Launch<3, 3>; return CALL; } \
+    if (dim == 2 && nq == 2) { using L = Launch<2, 2>; return CALL; } \
+    if (dim == 2 && nq == 3) { using L = Launch<2, 3>; return CALL; } \
+    return hipErrorInvalidValue;                            \
+  } while (0)
+
+hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
+                         const Grid &g, const double *W0, hipStream_t s) {
+  if (nv != 5 && nv != 1) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, g, W0, s));
+}
+
+hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
+                         double *wflag, int *list, int *counter, double wThresh, const Grid &g,
+                         hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::select(U, WI, theta, wflag, list, counter, wThresh, g, s));
+}
+
+hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, const Grid &g,
+                       const double *FW, const double *DW, const int *list, const int *counter,
+                       hipStream_t s) {
+  if (nv != 5 && nv != 1) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, g, FW, DW, list, counter, s));
+}
+
+hipError_t launch_pp(int dim, int nq, const double *U, double *FS, double *theta, const Grid &g,
+                     hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::pp(U, FS, theta, g, s));
+}
+
+hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
+                          const Grid &g, const RKArgs &a, hipStream_t s) {
+  if (nv != 5 && nv != 1) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::flux_rk(nv, U, FS, g, a, s));
+}
+
+int flux_rk_partials(int64_t ncell) { return int(stage_blocks(ncell)); }
+
+hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double *U, double *Uhat,
+                            double *Utmp, double betaDt, double bhatDt, int ncell, hipStream_t s) {
+  if (nv == 5)
+    hipLaunchKernelGGL((dev::k_stage_pre<5>), dim3(stage_blocks(ncell)), dim3(dev::BLK), 0, s, Uprev,
+                       K, U, Uhat, Utmp, betaDt, bhatDt, ncell);
+  else if (nv == 1)
+    hipLaunchKernelGGL((dev::k_stage_pre<1>), dim3(stage_blocks(ncell)), dim3(dev::BLK), 0, s, Uprev,
+                       K, U, Uhat, Utmp, betaDt, bhatDt, ncell);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double *errPartial, const double *posPartial, int nPart,
+                           double *out, hipStream_t s) {
+  hipLaunchKernelGGL(dev::k_finalize, dim3(1), dim3(1024), 0, s, errPartial, posPartial, nPart, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double *p, double v, size_t n, hipStream_t s) {
+  const unsigned nb = unsigned(std::min<size_t>((n + dev::BLK - 1) / dev::BLK, 4096));
+  hipLaunchKernelGGL(dev::k_fill, dim3(nb ? nb : 1), dim3(dev::BLK), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace kfvm

@@ -1,0 +1,671 @@
+// HIP (gfx950) kernels of one KFVM-WENO RK stage.
+//
+// Data layout in HBM (FP64, structure of arrays, x fastest):
+//   U[v][cell]                       cell averages, no ghost cells: boundary
+//                                    conditions are applied as index maps
+//                                    (mirror / periodic / signed mirror) when
+//                                    stencils are gathered
+//   FS[(f*Q + q)*NV + v][cell]       reconstructed state of each cell on its own
+//                                    face f (W,E,S,N,B,T) at face point q
+//   WI[f*Q + q][cell]                KXRCF indicator p/rho^gamma at those points
+//   theta[cell], wflag[cell]         positivity theta and WENO flag
+//
+// One stage = linear recon (+indicator)  -> KXRCF select (compacted list)
+//           -> KFVM-WENO on the list (or all cells) -> positivity limiter
+//           -> Riemann solves + flux divergence + RK register update (+ error
+//              norm on the FSAL stage), with the max wave speed reduced on
+//              the device.
+//
+// Reference correspondence (per kernel below):
+//   k_linear     Stencil_K.H:32-90     k_select   Numeric_K.H:292-398
+//   k_weno       Stencil_K.H:164-283, 416-454, 506-589
+//   k_pp         Common_K.H:299-643    k_flux_rk  Common_K.H:180-297,
+//                                      Numeric_K.H:109-290
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "../host/stencil_geom.hpp"
+#include "device_api.hpp"
+#include "physics.hpp"
+
+namespace kfvm {
+namespace dev {
+
+constexpr int BLK = 256;
+
+// ------------------------------------------------------------ index helpers
+__device__ inline int map_axis(int i, int n, int bcLo, int bcHi, int &side) {
+  side = -1;
+  if (i < 0) {
+    if (bcLo == 0) return i + n;  // periodic (|i| <= n)
+    side = 0;
+    return -i - 1;  // mirror
+  }
+  if (i >= n) {
+    if (bcHi == 0) return i - n;
+    side = 1;
+    return 2 * n - 1 - i;
+  }
+  return i;
+}
+
+// Cell average of variable v at (possibly ghost) cell (i,j,k).
+template <int D, int NV>
+__device__ inline double load_u(const double *__restrict__ U, const Grid &g, int i, int j, int k,
+                                int v) {
+  int sx, sy, sz = -1;
+  const int ii = map_axis(i, g.nx, g.bc[0], g.bc[1], sx);
+  const int jj = map_axis(j, g.ny, g.bc[2], g.bc[3], sy);
+  int kk = 0;
+  if (D == 3) kk = map_axis(k, g.nz, g.bc[4], g.bc[5], sz);
+  double x = U[size_t(v) * g.ncell + (size_t(kk) * g.ny + jj) * g.nx + ii];
+  if (NV == 5) {
+    unsigned m = 0;
+    if (sx >= 0) m ^= g.negmask[sx];
+    if (sy >= 0) m ^= g.negmask[2 + sy];
+    if (D == 3 && sz >= 0) m ^= g.negmask[4 + sz];
+    if ((m >> v) & 1u) x = -x;
+  }
+  return x;
+}
+
+template <int D>
+__device__ inline void cell_ijk(int c, const Grid &g, int &i, int &j, int &k) {
+  i = c % g.nx;
+  const int r = c / g.nx;
+  j = r % g.ny;
+  k = D == 3 ? r / g.ny : 0;
+}
+
+template <int D, int NQ>
+struct Shape {
+  static constexpr int F = 2 * D;
+  static constexpr int Q = D == 3 ? NQ * NQ : NQ;
+  static constexpr int C = SInfo<D>::C;
+};
+
+// ------------------------------------------------------- linear reconstruction
+template <int D, int NQ, int NV>
+__global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
+                                                double *__restrict__ FS,
+                                                double *__restrict__ WI, Grid g,
+                                                const double *__restrict__ W) {
+  constexpr auto ST = make_stencil_tables<D>();
+  constexpr int C = Shape<D, NQ>::C, F = Shape<D, NQ>::F, Q = Shape<D, NQ>::Q;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+  double face[F * Q][NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    double vs[C];
+#pragma unroll
+    for (int n = 0; n < C; n++) vs[n] = load_u<D, NV>(U, g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2], v);
+#pragma unroll
+    for (int fq = 0; fq < F * Q; fq++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int n = 0; n < C; n++) acc = fma(W[fq * C + n], vs[n], acc);
+      face[fq][v] = acc;
+      FS[(size_t(fq) * NV + v) * nc + c] = acc;
+    }
+  }
+  if (NV == 5) {
+#pragma unroll
+    for (int fq = 0; fq < F * Q; fq++) WI[size_t(fq) * nc + c] = Hydro::weno(g.gamma, face[fq]);
+  }
+}
+
+// ------------------------------------------------------------ KXRCF selector
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
+                                                const double *__restrict__ WI,
+                                                const double *__restrict__ theta,
+                                                double *__restrict__ wflag, int *__restrict__ list,
+                                                int *__restrict__ counter, double wThresh,
+                                                Grid g) {
+  constexpr int F = Shape<D, NQ>::F, Q = Shape<D, NQ>::Q;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  bool flagged = false;
+  if (c < g.ncell) {
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    const size_t nc = g.ncell;
+    double maxJump = 0.0;
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      const int ax = f >> 1, hi = f & 1;
+      int ni = i, nj = j, nk = k;
+      if (ax == 0) ni += hi ? 1 : -1;
+      if (ax == 1) nj += hi ? 1 : -1;
+      if (ax == 2) nk += hi ? 1 : -1;
+      const int n_ax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+      const int pos = ax == 0 ? ni : (ax == 1 ? nj : nk);
+      bool inside = pos >= 0 && pos < n_ax;
+      if (!inside && g.bc[f] == 0) {
+        inside = true;
+        if (ax == 0) ni = (ni + g.nx) % g.nx;
+        if (ax == 1) nj = (nj + g.ny) % g.ny;
+        if (ax == 2) nk = (nk + g.nz) % g.nz;
+      }
+      const size_t cn = (size_t(nk) * g.ny + nj) * g.nx + ni;
+      const int fo = f ^ 1;  // opposite face of the neighbour
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+        const double win = WI[size_t(f * Q + q) * nc + c];
+        const double wout = inside ? WI[size_t(fo * Q + q) * nc + cn] : win;
+        maxJump = fmax(maxJump, fabs(wout - win));
+      }
+    }
+    double uc[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) uc[v] = U[size_t(v) * nc + c];
+    const double ind = maxJump / Hydro::weno(g.gamma, uc);
+    flagged = ind >= wThresh || theta[c] < 1.0;
+    wflag[c] = flagged ? 1.0 : 0.0;
+  }
+  // wave-aggregated append to the compacted list
+  const unsigned long long mask = __ballot(flagged);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int cnt = __popcll(mask);
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, cnt);
+  base = __shfl(base, leader);
+  if (flagged) {
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    list[base + rank] = c;
+  }
+}
+
+// ---------------------------------------------------------------- KFVM-WENO
+template <int D, int NQ, int NV>
+__device__ inline void weno_cell(const double *__restrict__ U, double *__restrict__ FS, const Grid &g,
+                                 const double *__restrict__ FW, const double *__restrict__ DW,
+                                 int c) {
+  constexpr auto ST = make_stencil_tables<D>();
+  constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb, B = SInfo<D>::B;
+  constexpr int nSub = SInfo<D>::nSub, I = SInfo<D>::I;
+  constexpr int F = Shape<D, NQ>::F, Q = Shape<D, NQ>::Q, FQ = F * Q;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+
+  double ref[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) ref[v] = U[size_t(v) * nc + c];
+  double rho = 1.0, uu[3] = {0.0, 0.0, 0.0}, vsq = 0.0, gamm = 1.0;
+  if (NV == 5) {
+    LinPrim lp(ref, g.gamma);
+    rho = lp.rho;
+    uu[0] = lp.u;
+    uu[1] = lp.v;
+    uu[2] = lp.w;
+    vsq = lp.vsq;
+    gamm = lp.gamm;
+  }
+  // linear-weight gammas of the WENO-AO combination
+  const double gHi = 0.7, gLo = 0.7;
+  double gm[nSub];
+  gm[0] = gHi;
+  gm[1] = (1.0 - gHi) * gLo;
+#pragma unroll
+  for (int s = 2; s < nSub; s++) gm[s] = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
+
+  double densOut[FQ], eAcc[FQ];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    // stencil values of variable v in linearised primitive form
+    double vs[C];
+#pragma unroll
+    for (int n = 0; n < C; n++) {
+      const int a = i + ST.off[n][0], b = j + ST.off[n][1], d = k + ST.off[n][2];
+      if (NV == 5) {
+        const double s0 = load_u<D, NV>(U, g, a, b, d, 0);
+        if (v == 0) {
+          vs[n] = s0;
+        } else if (v < 4) {
+          const double sv = load_u<D, NV>(U, g, a, b, d, v);
+          vs[n] = (sv - uu[v - 1] * s0) / rho;
+        } else {
+          const double s1 = load_u<D, NV>(U, g, a, b, d, 1);
+          const double s2 = load_u<D, NV>(U, g, a, b, d, 2);
+          const double s3 = load_u<D, NV>(U, g, a, b, d, 3);
+          const double s4 = load_u<D, NV>(U, g, a, b, d, 4);
+          vs[n] = gamm * (vsq * s0 - uu[0] * s1 - uu[1] * s2 - uu[2] * s3 + s4);
+        }
+      } else {
+        vs[n] = load_u<D, NV>(U, g, a, b, d, v);
+      }
+    }
+    // smoothness indicators
+    double om[nSub];
+#pragma unroll
+    for (int s = 0; s < nSub; s++) {
+      const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
+      double beta = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < I; ii++) {
+        const double *d = DW + (s * I + ii) * C;
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < n; m++) {
+          const int cc = s < 2 ? m : ST.sub[s - 2][m];
+          t = fma(d[m], vs[cc], t);
+        }
+        beta = fma(t, t, beta);
+      }
+      om[s] = beta;
+    }
+    // WENO-JS weights, normalised, then the WENO-AO transform
+    double tot = 0.0;
+#pragma unroll
+    for (int s = 0; s < nSub; s++) {
+      om[s] = gm[s] / (om[s] * om[s] + 1.e-40);
+      tot += om[s];
+    }
+#pragma unroll
+    for (int s = 0; s < nSub; s++) om[s] /= tot;
+    om[0] /= gm[0];
+#pragma unroll
+    for (int s = 1; s < nSub; s++) om[s] -= om[0] * gm[s];
+
+    // face values: sum_s omega_s (W_s . V_s), then back to conserved variables
+#pragma unroll
+    for (int fq = 0; fq < FQ; fq++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < nSub; s++) {
+        const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
+        const double *w = FW + (s * FQ + fq) * C;
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < n; m++) {
+          const int cc = s < 2 ? m : ST.sub[s - 2][m];
+          t = fma(w[m], vs[cc], t);
+        }
+        acc = fma(om[s], t, acc);
+      }
+      double out = acc;
+      if (NV == 5) {
+        if (v == 0) {
+          densOut[fq] = acc;
+          eAcc[fq] = vsq * acc;
+        } else if (v < 4) {
+          out = rho * acc + uu[v - 1] * densOut[fq];
+          eAcc[fq] = eAcc[fq] + rho * uu[v - 1] * acc;
+        } else {
+          out = eAcc[fq] + acc / gamm;
+        }
+      }
+      FS[(size_t(fq) * NV + v) * nc + c] = out;
+    }
+  }
+  (void)B;
+}
+
+template <int D, int NQ, int NV>
+__global__ __launch_bounds__(BLK) void k_weno(const double *__restrict__ U,
+                                              double *__restrict__ FS, Grid g,
+                                              const double *__restrict__ FW,
+                                              const double *__restrict__ DW,
+                                              const int *__restrict__ list,
+                                              const int *__restrict__ counter) {
+  const int n = list ? *counter : g.ncell;
+  for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
+    const int c = list ? list[t] : t;
+    weno_cell<D, NQ, NV>(U, FS, g, FW, DW, c);
+  }
+}
+
+// ------------------------------------------------------ positivity limiter
+template <int D, int NQ>
+__device__ inline void pp_scale(double *__restrict__ FS, size_t nc, int c, const double *ref,
+                                double th, bool replace) {
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+#pragma unroll
+  for (int fq = 0; fq < FQ; fq++)
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      double *p = FS + (size_t(fq) * 5 + v) * nc + c;
+      *p = replace ? ref[v] : ref[v] + th * (*p - ref[v]);
+    }
+}
+
+__device__ inline double eint_pos(double emin, const double *ref, const double *s) {
+  double a = 0.0, b = 1.0, c;
+  double st[5];
+  for (int it = 0; it < 10; it++) {
+    c = 0.5 * (a + b);
+#pragma unroll
+    for (int v = 0; v < 5; v++) st[v] = ref[v] + c * (s[v] - ref[v]);
+    const double dec = Hydro::eint(st) - emin;
+    if (dec > 0.0) a = c;
+    else b = c;
+  }
+  return a;
+}
+
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_pp(const double *__restrict__ U, double *__restrict__ FS,
+                                            double *__restrict__ theta, Grid g) {
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+  double ref[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) ref[v] = U[size_t(v) * nc + c];
+  const double rDens = ref[0];
+  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
+#pragma unroll
+  for (int xo = -1; xo <= 1; xo++)
+#pragma unroll
+    for (int yo = -1; yo <= 1; yo++)
+#pragma unroll
+      for (int zo = zl; zo <= zh; zo++) {
+        double s[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) s[v] = load_u<D, 5>(U, g, i + xo, j + yo, k + zo, v);
+        maxDens = fmax(maxDens, s[0]);
+        minDens = fmin(minDens, s[0]);
+        const double ei = Hydro::eint(s);
+        const double sd = Hydro::sound(g.gamma, s);
+        minEint = fmin(minEint, ei);
+        minSdSp = fmin(minSdSp, sd);
+      }
+  const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
+                     load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
+  const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
+                     load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
+  double dzW;
+  if (D == 3)
+    dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
+          load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
+  else
+    dzW = ref[3] / ref[0] - ref[3] / ref[0];
+  const double divV = (dxU + dyV + dzW) / 2.0;
+  double kappa = 0.2;
+  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+  kappa *= (1.0 - eta);
+  maxDens *= (1.0 + kappa);
+  minDens *= (1.0 - kappa);
+  minEint *= (1.0 - kappa);
+
+  const double dP = maxDens - rDens, dM = rDens - minDens;
+  double thD = 1.0;
+#pragma unroll
+  for (int fq = 0; fq < FQ; fq++) {
+    const double rho = FS[(size_t(fq) * 5) * nc + c];
+    thD = rho > minDens ? thD : fmin(thD, dM / (rDens - rho));
+    thD = rho < maxDens ? thD : fmin(thD, dP / (rho - rDens));
+  }
+  if (thD > 0.0 && thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, false);
+  else if (thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, true);
+
+  double thP = 1.0;
+#pragma unroll
+  for (int fq = 0; fq < FQ; fq++) {
+    double s[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) s[v] = FS[(size_t(fq) * 5 + v) * nc + c];
+    const double ei = Hydro::eint(s);
+    thP = ei > minEint ? thP : fmin(thP, eint_pos(minEint, ref, s));
+  }
+  if (thP > 0.0 && thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, false);
+  else if (thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, true);
+  theta[c] = fmin(thD, thP);
+}
+
+// -------------------------------------------- Riemann + divergence + RK update
+template <int NV, int DIR>
+__device__ inline double riemann(const Grid &g, const double *uL, const double *uR, double *Fl) {
+  if (NV == 5) {
+    if (g.rs == 2) return Hydro::solve_hllc<DIR>(g.gamma, uL, uR, Fl);
+    return Hydro::solve_hll<DIR>(g.gamma, g.rs == 0, uL, uR, Fl);
+  } else {
+    if (!g.haveVel && DIR == 2) {
+      Fl[0] = 0.0;
+      return 0.0;
+    }
+    const double a = g.vel[DIR];
+    Fl[0] = a > 0.0 ? a * uL[0] : a * uR[0];
+    return fabs(a);
+  }
+}
+
+// Block-wide reductions with a fixed combination order (deterministic).
+__device__ inline double block_sum(double x, double *sh) {
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+__device__ inline double block_min(double x, double *sh) {
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = sh[threadIdx.x + s];
+      sh[threadIdx.x] = sh[threadIdx.x] > o ? o : sh[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+__device__ inline double block_max(double x, double *sh) {
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = sh[threadIdx.x + s];
+      sh[threadIdx.x] = o > sh[threadIdx.x] ? o : sh[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <int D, int NQ, int NV>
+__global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
+                                                 const double *__restrict__ FS, Grid g,
+                                                 RKArgs a) {
+  constexpr int Q = Shape<D, NQ>::Q, F = Shape<D, NQ>::F;
+  __shared__ double sh[BLK];
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  const bool live = c < g.ncell;
+  double vmax = 0.0, errLoc = 0.0, posLoc = 1.0;
+  if (live) {
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    const size_t nc = g.ncell;
+    // neighbours across each face (or -1 for a non-periodic domain boundary)
+    size_t nb[F];
+#pragma unroll
+    for (int f = 0; f < F; f++) {
+      const int ax = f >> 1, hi = f & 1;
+      int ni = i, nj = j, nk = k;
+      if (ax == 0) ni += hi ? 1 : -1;
+      if (ax == 1) nj += hi ? 1 : -1;
+      if (ax == 2) nk += hi ? 1 : -1;
+      const int n_ax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+      const int pos = ax == 0 ? ni : (ax == 1 ? nj : nk);
+      bool inside = pos >= 0 && pos < n_ax;
+      if (!inside && g.bc[f] == 0) {
+        inside = true;
+        if (ax == 0) ni = (ni + g.nx) % g.nx;
+        if (ax == 1) nj = (nj + g.ny) % g.ny;
+        if (ax == 2) nk = (nk + g.nz) % g.nz;
+      }
+      nb[f] = inside ? (size_t(nk) * g.ny + nj) * g.nx + ni : ~size_t(0);
+    }
+    double rhs[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) rhs[v] = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      double fl[F][NV];
+#pragma unroll
+      for (int f = 0; f < F; f++) {
+        double in[NV], out[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) in[v] = FS[(size_t(f * Q + q) * NV + v) * nc + c];
+        if (nb[f] != ~size_t(0)) {
+#pragma unroll
+          for (int v = 0; v < NV; v++) out[v] = FS[(size_t((f ^ 1) * Q + q) * NV + v) * nc + nb[f]];
+        } else {
+          const unsigned m = NV == 5 ? g.negmask[f] : 0u;
+#pragma unroll
+          for (int v = 0; v < NV; v++) out[v] = ((m >> v) & 1u) ? -in[v] : in[v];
+        }
+        const double *uL = (f & 1) ? in : out;
+        const double *uR = (f & 1) ? out : in;
+        double s;
+        if ((f >> 1) == 0) s = riemann<NV, 0>(g, uL, uR, fl[f]);
+        else if ((f >> 1) == 1) s = riemann<NV, 1>(g, uL, uR, fl[f]);
+        else s = riemann<NV, 2>(g, uL, uR, fl[f]);
+        vmax = s > vmax ? s : vmax;
+      }
+      // flux divergence in the reference's accumulation order
+      if (D == 2) {
+        const double w = g.wt[q];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          rhs[v] += w * (fl[0][v] - fl[1][v]) / g.dx;
+          rhs[v] += w * (fl[2][v] - fl[3][v]) / g.dy;
+        }
+      } else {
+        const double w = g.wt[q / NQ] * g.wt[q % NQ];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          rhs[v] += w * (fl[0][v] - fl[1][v]) / g.dx;
+          rhs[v] += w * (fl[2][v] - fl[3][v]) / g.dy;
+          rhs[v] += w * (fl[4][v] - fl[5][v]) / g.dz;
+        }
+      }
+    }
+    // RK register update
+    if (a.mode == RK_PRIME) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) a.K[size_t(v) * nc + c] = rhs[v];
+    } else if (a.mode == RK_STAGE) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const size_t o = size_t(v) * nc + c;
+        const double u = U[o];
+        double tmp = 0.0;
+        if (a.Utmp) {
+          tmp = a.Utmp[o] + a.delta * u;
+          a.Utmp[o] = tmp;
+        }
+        a.Unext[o] = a.g1 * u + a.g2 * tmp + a.g3 * a.Uprev[o] + a.betaDt * rhs[v];
+        a.Uhat[o] = a.Uhat[o] + a.bhatDt * rhs[v];
+      }
+    } else {  // RK_FSAL: finish the embedded solution, keep K, error norm
+      bool hasNan = false;
+      double st[NV];
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        const size_t o = size_t(v) * nc + c;
+        const double u = U[o];
+        const double uh = a.Uhat[o] + a.bhatDt * rhs[v];
+        a.Uhat[o] = uh;
+        a.K[o] = rhs[v];
+        st[v] = u;
+        hasNan = hasNan || isnan(u);
+        const double e = (u - uh) / (a.atol + a.rtol * fmax(fabs(u), fabs(uh)));
+        errLoc += pow(e, 2.0);
+      }
+      posLoc = hasNan ? -1.0 : (NV == 5 ? Hydro::positivity(st) : 1.0);
+    }
+  }
+  const double bmax = block_max(vmax, sh);
+  if (threadIdx.x == 0 && bmax > 0.0)
+    atomicMax(a.vmaxBits, (unsigned long long)__double_as_longlong(bmax));
+  if (a.mode == RK_FSAL) {
+    const double es = block_sum(errLoc, sh);
+    const double pm = block_min(posLoc, sh);
+    if (threadIdx.x == 0) {
+      a.errPartial[blockIdx.x] = es;
+      a.posPartial[blockIdx.x] = pm;
+    }
+  }
+}
+
+#ifdef KFVM_SMALL_KERNELS
+// ------------------------------------------------------ small register kernels
+template <int NV>
+__global__ __launch_bounds__(BLK) void k_stage_pre(const double *__restrict__ Uprev,
+                                                   const double *__restrict__ K,
+                                                   double *__restrict__ U, double *__restrict__ Uhat,
+                                                   double *__restrict__ Utmp, double betaDt,
+                                                   double bhatDt, int ncell) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= ncell) return;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const size_t o = size_t(v) * ncell + c;
+    const double up = Uprev[o], kk = K[o];
+    U[o] = up + betaDt * kk;
+    Uhat[o] = up + bhatDt * kk;
+    if (Utmp) Utmp[o] = up;
+  }
+}
+
+// Deterministic final reduction of the per-block partials.
+__global__ __launch_bounds__(1024) void k_finalize(const double *__restrict__ errPartial,
+                                                   const double *__restrict__ posPartial,
+                                                   int nPart, double *__restrict__ out) {
+  __shared__ double se[1024], sp[1024];
+  double e = 0.0, p = 1.0;
+  for (int t = threadIdx.x; t < nPart; t += 1024) {
+    e += errPartial[t];
+    const double o = posPartial[t];
+    p = p > o ? o : p;
+  }
+  se[threadIdx.x] = e;
+  sp[threadIdx.x] = p;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      se[threadIdx.x] += se[threadIdx.x + s];
+      const double o = sp[threadIdx.x + s];
+      sp[threadIdx.x] = sp[threadIdx.x] > o ? o : sp[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = se[0];
+    out[1] = sp[0];
+  }
+}
+
+__global__ void k_fill(double *__restrict__ p, double v, size_t n) {
+  for (size_t t = size_t(blockIdx.x) * BLK + threadIdx.x; t < n; t += size_t(gridDim.x) * BLK) p[t] = v;
+}
+
+#endif  // KFVM_SMALL_KERNELS
+
+}  // namespace dev
+
+}  // namespace kfvm

@@ -1,0 +1,339 @@
+// extern "C" boundary of the stage library (declared in include/kfvm.h).
+#include "../../../include/kfvm.h"
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "solver.hpp"
+
+struct kfvm_solver {
+  kfvm::Solver *s;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(const std::string &m) {
+  g_err = m;
+  return -1;
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    return fail(e.what());
+  } catch (...) {
+    return fail("unknown error");
+  }
+}
+
+int create(const kfvm::Config &cfg, const std::string &warnings, kfvm_solver **out) {
+  if (!out) return fail("null output pointer");
+  *out = nullptr;
+  return guarded([&] {
+    kfvm::Solver *s = new kfvm::Solver(cfg);
+    *out = new kfvm_solver{s};
+    if (!warnings.empty() && cfg.verbose > 0) std::fputs(warnings.c_str(), stderr);
+    return 0;
+  });
+}
+
+#define CHECK_SOLVER(s) \
+  if (!(s) || !(s)->s) return fail("null solver")
+
+int64_t state_size(const kfvm::Solver &s) { return s.ncell() * s.nv(); }
+}  // namespace
+
+extern "C" {
+
+const char *kfvm_last_error(void) { return g_err.c_str(); }
+
+int kfvm_create(const char *init_path, kfvm_solver **out) {
+  if (!init_path) return fail("null init path");
+  kfvm::Config cfg;
+  std::string warn;
+  if (!kfvm::parse_init_file(init_path, cfg, &warn))
+    return fail(std::string("could not open input file: ") + init_path);
+  return create(cfg, warn, out);
+}
+
+int kfvm_create_from_text(const char *init_text, kfvm_solver **out) {
+  if (!init_text) return fail("null init text");
+  kfvm::Config cfg;
+  std::string warn;
+  kfvm::parse_init_text(init_text, cfg, &warn);
+  return create(cfg, warn, out);
+}
+
+int kfvm_destroy(kfvm_solver *s) {
+  if (!s) return 0;
+  return guarded([&] {
+    delete s->s;
+    delete s;
+    return 0;
+  });
+}
+
+int kfvm_solve(kfvm_solver *s, int64_t *steps_taken) {
+  CHECK_SOLVER(s);
+  return guarded([&] {
+    const int64_t n = s->s->solve();
+    if (steps_taken) *steps_taken = n;
+    return 0;
+  });
+}
+
+int kfvm_take_step(kfvm_solver *s, int *status) {
+  CHECK_SOLVER(s);
+  return guarded([&] {
+    const int st = s->s->takeStep();
+    if (status) *status = st;
+    return 0;
+  });
+}
+
+int kfvm_advance_stages(kfvm_solver *s, int64_t nstages, double dt) {
+  CHECK_SOLVER(s);
+  if (nstages < 0 || !(dt > 0.0)) return fail("need nstages >= 0 and dt > 0");
+  return guarded([&] {
+    s->s->advanceStages(nstages, dt);
+    return 0;
+  });
+}
+
+int kfvm_eval_rhs(kfvm_solver *s, int which, double *max_vel) {
+  CHECK_SOLVER(s);
+  if (which != 0 && which != 1) return fail("which must be 0 (current) or 1 (step start)");
+  return guarded([&] {
+    const double v = s->s->evalRHS(which);
+    if (max_vel) *max_vel = v;
+    return 0;
+  });
+}
+
+int kfvm_get_state(kfvm_solver *s, double *host, int64_t n) {
+  CHECK_SOLVER(s);
+  if (!host || n != state_size(*s->s)) return fail("state buffer must hold NV*ncell doubles");
+  return guarded([&] {
+    s->s->getState(host);
+    return 0;
+  });
+}
+
+int kfvm_set_state(kfvm_solver *s, const double *host, int64_t n) {
+  CHECK_SOLVER(s);
+  if (!host || n != state_size(*s->s)) return fail("state buffer must hold NV*ncell doubles");
+  return guarded([&] {
+    s->s->setState(host);
+    return 0;
+  });
+}
+
+int kfvm_get_rhs(kfvm_solver *s, double *host, int64_t n) {
+  CHECK_SOLVER(s);
+  if (!host || n != state_size(*s->s)) return fail("rhs buffer must hold NV*ncell doubles");
+  return guarded([&] {
+    s->s->getRHS(host);
+    return 0;
+  });
+}
+
+int kfvm_get_flags(kfvm_solver *s, double *host, int64_t n) {
+  CHECK_SOLVER(s);
+  if (!host || n != 2 * s->s->ncell()) return fail("flag buffer must hold 2*ncell doubles");
+  return guarded([&] {
+    s->s->getFlags(host);
+    return 0;
+  });
+}
+
+int kfvm_set_flags(kfvm_solver *s, const double *host, int64_t n) {
+  CHECK_SOLVER(s);
+  if (!host || n != 2 * s->s->ncell()) return fail("flag buffer must hold 2*ncell doubles");
+  return guarded([&] {
+    s->s->setFlags(host);
+    return 0;
+  });
+}
+
+int kfvm_get_time(kfvm_solver *s, double *t, double *dt, int64_t *step) {
+  CHECK_SOLVER(s);
+  if (t) *t = s->s->time;
+  if (dt) *dt = s->s->dt;
+  if (step) *step = s->s->nTS;
+  return 0;
+}
+
+int kfvm_set_time(kfvm_solver *s, double t, double dt) {
+  CHECK_SOLVER(s);
+  s->s->time = t;
+  s->s->dt = dt;
+  return 0;
+}
+
+int kfvm_get_info(kfvm_solver *s, kfvm_info *info) {
+  CHECK_SOLVER(s);
+  if (!info) return fail("null info");
+  const kfvm::Config &c = s->s->config();
+  const kfvm::WeightTable &w = s->s->weights();
+  info->dim = c.dim;
+  info->nq = c.nQuad;
+  info->nv = s->s->nv();
+  info->nx = c.nX;
+  info->ny = c.nY;
+  info->nz = c.dim == 3 ? c.nZ : 1;
+  info->ncell = s->s->ncell();
+  info->nsub = w.nSub;
+  info->nfaces = w.F;
+  info->nqface = w.Q;
+  info->ncells_stencil = w.C;
+  info->nindic = w.I;
+  info->rk_stages = s->s->rkStages();
+  info->dx = c.dx();
+  info->dy = c.dy();
+  info->dz = c.dz();
+  info->gamma = c.gamma;
+  return 0;
+}
+
+int kfvm_get_counters(kfvm_solver *s, int64_t *rhs_evals, int64_t *rejects_tol,
+                      int64_t *rejects_unphys, int64_t *last_nweno) {
+  CHECK_SOLVER(s);
+  if (rhs_evals) *rhs_evals = s->s->nRhsEval;
+  if (rejects_tol) *rejects_tol = s->s->nRejectThresh;
+  if (rejects_unphys) *rejects_unphys = s->s->nRejectUnphys;
+  if (last_nweno) *last_nweno = s->s->lastNWeno;
+  return 0;
+}
+
+int kfvm_force_full_weno(kfvm_solver *s) {
+  CHECK_SOLVER(s);
+  s->s->useSparseWeno = false;
+  return 0;
+}
+
+int kfvm_get_weights(kfvm_solver *s, double *face, int64_t nface, double *deriv, int64_t nderiv) {
+  CHECK_SOLVER(s);
+  const kfvm::WeightTable &w = s->s->weights();
+  if (nface != int64_t(w.face.size()) || nderiv != int64_t(w.deriv.size()))
+    return fail("weight buffers have the wrong size");
+  if (face) std::memcpy(face, w.face.data(), w.face.size() * sizeof(double));
+  if (deriv) std::memcpy(deriv, w.deriv.data(), w.deriv.size() * sizeof(double));
+  return 0;
+}
+
+int kfvm_set_weights(kfvm_solver *s, const double *face, int64_t nface, const double *deriv,
+                     int64_t nderiv) {
+  CHECK_SOLVER(s);
+  const kfvm::WeightTable &w = s->s->weights();
+  if (!face || !deriv || nface != int64_t(w.face.size()) || nderiv != int64_t(w.deriv.size()))
+    return fail("weight buffers have the wrong size");
+  return guarded([&] {
+    s->s->setWeights(face, deriv);
+    return 0;
+  });
+}
+
+int kfvm_profile_enable(kfvm_solver *s, int on) {
+  CHECK_SOLVER(s);
+  return guarded([&] {
+    s->s->profileEnable(on != 0);
+    return 0;
+  });
+}
+
+int kfvm_profile_get(kfvm_solver *s, const char *name, double *ms, int64_t *count) {
+  CHECK_SOLVER(s);
+  if (!name) return fail("null kernel name");
+  return guarded([&] {
+    double m = 0.0;
+    int64_t c = 0;
+    s->s->profileGet(name, m, c);
+    if (ms) *ms = m;
+    if (count) *count = c;
+    return 0;
+  });
+}
+
+int kfvm_profile_reset(kfvm_solver *s) {
+  CHECK_SOLVER(s);
+  return guarded([&] {
+    s->s->profileReset();
+    return 0;
+  });
+}
+
+int kfvm_sync(kfvm_solver *s) {
+  CHECK_SOLVER(s);
+  return guarded([&] {
+    s->s->sync();
+    return 0;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- host only
+#include <sstream>
+
+#include "weights.hpp"
+
+extern "C" {
+
+int kfvm_weight_sizes(int dim, int nq, int64_t *nface, int64_t *nderiv) {
+  if ((dim != 2 && dim != 3) || (nq != 2 && nq != 3)) return fail("dim must be 2|3 and nq 2|3");
+  const int nSub = dim == 3 ? 8 : 6, C = dim == 3 ? 33 : 13, I = dim == 3 ? 19 : 5;
+  const int Q = dim == 3 ? nq * nq : nq;
+  if (nface) *nface = int64_t(nSub) * 2 * dim * Q * C;
+  if (nderiv) *nderiv = int64_t(nSub) * I * C;
+  return 0;
+}
+
+int kfvm_build_weights(int dim, int nq, double lfac, double dx, double dy, double dz, double *face,
+                       int64_t nface, double *deriv, int64_t nderiv) {
+  int64_t nf = 0, nd = 0;
+  if (kfvm_weight_sizes(dim, nq, &nf, &nd) != 0) return -1;
+  if (!face || !deriv || nface != nf || nderiv != nd) return fail("weight buffers have the wrong size");
+  return guarded([&] {
+    kfvm::WeightTable w = kfvm::build_weights(dim, nq, lfac, dx, dy, dz);
+    std::memcpy(face, w.face.data(), w.face.size() * sizeof(double));
+    std::memcpy(deriv, w.deriv.data(), w.deriv.size() * sizeof(double));
+    return 0;
+  });
+}
+
+int kfvm_parse_init(const char *init_text, char *json, int64_t len) {
+  if (!init_text || !json || len <= 0) return fail("null argument");
+  return guarded([&] {
+    kfvm::Config c;
+    std::string warn;
+    kfvm::parse_init_text(init_text, c, &warn);
+    std::ostringstream o;
+    o.precision(17);
+    const char *bcn[] = {"periodic", "outflow", "reflecting", "noslip"};
+    o << "{\"dim\":" << c.dim << ",\"n\":[" << c.nX << "," << c.nY << "," << c.nZ << "]"
+      << ",\"lo\":[" << c.xLo << "," << c.yLo << "," << c.zLo << "],\"hi\":[" << c.xUp << "," << c.yUp
+      << "," << c.zUp << "],\"nq\":" << c.nQuad << ",\"lfac\":" << c.lfac << ",\"eq\":" << int(c.eq)
+      << ",\"rs\":" << int(c.rs) << ",\"rk\":" << int(c.rk) << ",\"bc\":[";
+    for (int f = 0; f < 6; f++) o << (f ? "," : "") << "\"" << bcn[int(c.bc[f])] << "\"";
+    o << "],\"problem\":\"" << c.problem << "\",\"final\":" << c.finalTime << ",\"mindt\":" << c.minDt
+      << ",\"cfl\":" << c.cfl << ",\"atol\":" << c.atol << ",\"rtol\":" << c.rtol
+      << ",\"rejectionthresh\":" << c.rejectionThresh << ",\"rejectionlimit\":" << c.rejectionLimit
+      << ",\"maxsteps\":" << c.maxSteps << ",\"fixeddt\":" << c.fixedDt << ",\"gamma\":" << c.gamma
+      << ",\"wenothresh\":" << c.wenoThresh << ",\"advvel\":" << (c.haveAdvVel ? "[" : "null");
+    if (c.haveAdvVel) o << c.advVel[0] << "," << c.advVel[1] << "," << c.advVel[2] << "]";
+    o << ",\"user\":[";
+    for (int i = 0; i < 8; i++) o << (i ? "," : "") << c.user[i];
+    o << "],\"seed\":" << c.seed << ",\"warnings\":" << std::count(warn.begin(), warn.end(), '\n') << "}";
+    const std::string s = o.str();
+    if (int64_t(s.size()) + 1 > len) return fail("json buffer too small");
+    std::memcpy(json, s.c_str(), s.size() + 1);
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+// Compiled-in problem registry: point initial conditions and their Gauss-rule
+// cell averages.  Replaces the reference's per-problem InitialCondition.H
+// (textually included at src/numeric/Numeric_K.H:25-27) since device callbacks
+// cannot cross a C ABI; the averaging follows IntegrateIC_K
+// (src/numeric/Numeric_K.H:31-107).  The IC is computed on the host with the
+// C library's math so that it does not depend on device math routines.
+#include "problems.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+namespace kfvm {
+namespace {
+
+uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ULL;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+  return x ^ (x >> 31);
+}
+
+// counter-based uniform [0,1)
+double uniform01(uint64_t seed, uint64_t ctr) {
+  const uint64_t z = mix64(seed ^ mix64(ctr + 0x632BE59BD9B4E019ULL));
+  return double(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void prim2cons(double gamma, double rho, double u, double v, double w, double p, double *s) {
+  const double gamm = gamma - 1.0;
+  s[0] = rho;
+  s[1] = rho * u;
+  s[2] = rho * v;
+  s[3] = rho * w;
+  s[4] = p / gamm + rho * (u * u + v * v + w * w) / 2.0;
+}
+
+using PointIC = void (*)(const Config &, uint64_t cell, double x, double y, double z, double *s);
+
+void ic_linadv_sine(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  s[0] = std::sin(2.0 * M_PI * (c.user[0] * x + c.user[1] * y + c.user[2] * z));
+}
+void ic_riemann_planar(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  const double r = c.dim == 3 ? x + y + z : x + y;
+  if (r < c.user[0]) prim2cons(c.gamma, 1.0, 0.0, 0.0, 0.0, 1.0, s);
+  else prim2cons(c.gamma, 0.125, 0.0, 0.0, 0.0, 0.1, s);
+}
+void ic_sod_x(const Config &c, uint64_t, double x, double, double, double *s) {
+  if (x < c.user[0]) prim2cons(c.gamma, 1.0, 0.0, 0.0, 0.0, 1.0, s);
+  else prim2cons(c.gamma, 0.125, 0.0, 0.0, 0.0, 0.1, s);
+}
+void ic_shear2d(const Config &c, uint64_t cell, double, double y, double, double *s) {
+  const bool band = std::fabs(y - 0.5) < 0.25;
+  const double v = 1.0e-2 * (2.0 * uniform01(c.seed, cell) - 1.0);
+  prim2cons(c.gamma, band ? 2.0 : 1.0, band ? 0.5 : -0.5, v, 0.0, 2.5, s);
+}
+void ic_quadrant2d(const Config &c, uint64_t, double x, double y, double, double *s) {
+  const int q = (x < 0.5 ? 0 : 1) + (y < 0.5 ? 0 : 2);
+  const uint64_t b = 4 * uint64_t(q);
+  const double rho = 0.1 + 1.9 * uniform01(c.seed, b + 0);
+  const double p = 0.1 + 1.9 * uniform01(c.seed, b + 1);
+  const double u = -1.0 + 2.0 * uniform01(c.seed, b + 2);
+  const double v = -1.0 + 2.0 * uniform01(c.seed, b + 3);
+  prim2cons(c.gamma, rho, u, v, 0.0, p, s);
+}
+void ic_uniform(const Config &c, uint64_t, double, double, double, double *s) {
+  prim2cons(c.gamma, c.user[0], c.user[1], c.user[2], c.user[3], c.user[4], s);
+}
+void ic_smooth_wave(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  const double ph = 2.0 * M_PI * (x + (c.dim >= 2 ? y : 0.0) + (c.dim == 3 ? z : 0.0));
+  prim2cons(c.gamma, 1.0 + 0.2 * std::sin(ph), c.user[0], c.user[1], c.user[2],
+            1.0 + 0.1 * std::cos(ph), s);
+}
+
+struct Entry {
+  const char *name;
+  Eq eq;
+  PointIC fn;
+};
+const Entry kRegistry[] = {
+    {"linadv_sine", Eq::LinAdv, ic_linadv_sine},
+    {"riemann3d_planar", Eq::Hydro, ic_riemann_planar},
+    {"sod_x", Eq::Hydro, ic_sod_x},
+    {"shear2d", Eq::Hydro, ic_shear2d},
+    {"quadrant2d", Eq::Hydro, ic_quadrant2d},
+    {"uniform", Eq::Hydro, ic_uniform},
+    {"smooth_wave", Eq::Hydro, ic_smooth_wave},
+};
+
+}  // namespace
+
+bool problem_known(const std::string &name, Eq eq) {
+  for (const Entry &e : kRegistry)
+    if (name == e.name) return e.eq == eq;
+  return false;
+}
+
+void fill_initial_condition(const Config &c, std::vector<double> &U) {
+  const Entry *entry = nullptr;
+  for (const Entry &e : kRegistry)
+    if (c.problem == e.name) entry = &e;
+  if (!entry) throw std::runtime_error("unknown problem: " + c.problem);
+  if (entry->eq != c.eq) throw std::runtime_error("problem " + c.problem + " does not match problem:equation");
+  const int NV = num_vars(c.eq);
+  const int64_t nx = c.nX, ny = c.nY, nz = c.dim == 3 ? c.nZ : 1, ncell = nx * ny * nz;
+  U.assign(size_t(ncell) * NV, 0.0);
+  const QuadRule qr = quad_rule(c.nQuad);
+  const double dx = c.dx(), dy = c.dy(), dz = c.dz();
+  const int nQZ = c.dim == 3 ? qr.n : 1;
+  auto work = [&](int64_t k0, int64_t k1) {
+    double s[9], acc[9];
+    for (int64_t k = k0; k < k1; k++)
+      for (int64_t j = 0; j < ny; j++)
+        for (int64_t i = 0; i < nx; i++) {
+          const double xc = dx * (double(i) + 0.5) + c.xLo;
+          const double yc = dy * (double(j) + 0.5) + c.yLo;
+          const double zc = c.dim == 3 ? dz * (double(k) + 0.5) + c.zLo : 0.0;
+          const uint64_t cell = (uint64_t(k) * ny + j) * nx + i;
+          for (int v = 0; v < NV; v++) acc[v] = 0.0;
+          for (int a = 0; a < qr.n; a++)
+            for (int b = 0; b < qr.n; b++)
+              for (int e = 0; e < nQZ; e++) {
+                const double px = qr.ab[a] * dx + xc, py = qr.ab[b] * dy + yc;
+                const double pz = c.dim == 3 ? qr.ab[e] * dz + zc : c.zLo;
+                for (int v = 0; v < 9; v++) s[v] = 0.0;
+                entry->fn(c, cell, px, py, pz, s);
+                const double w = c.dim == 3 ? qr.wt[a] * qr.wt[b] * qr.wt[e] : qr.wt[a] * qr.wt[b];
+                for (int v = 0; v < NV; v++) acc[v] += w * s[v];
+              }
+          for (int v = 0; v < NV; v++) U[size_t(v) * ncell + cell] = acc[v];
+        }
+  };
+  const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (nz >= int64_t(nt) && ncell > 100000) {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++) {
+      const int64_t k0 = nz * t / nt, k1 = nz * (t + 1) / nt;
+      th.emplace_back(work, k0, k1);
+    }
+    for (auto &t : th) t.join();
+  } else {
+    work(0, nz);
+  }
+}
+
+}  // namespace kfvm

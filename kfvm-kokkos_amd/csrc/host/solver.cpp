@@ -1,0 +1,432 @@
+// Host driver of one KFVM problem on one GPU.  See solver.hpp.
+//
+// Per RK stage the device runs (Solver::evalRHS, src/Solver.cpp:410-530):
+//   sparse mode:  linear recon (+KXRCF indicator) -> select -> WENO on the
+//                 compacted list -> positivity limiter -> flux/RK kernel
+//   full mode:    WENO on every cell -> positivity limiter -> flux/RK kernel
+// The RK register update (src/numeric/Numeric_K.H:198-260) and the error norm
+// (Numeric_K.H:262-290) are the flux kernel's epilogue, U ping-pongs between
+// two buffers, and the host reads three scalars once per step attempt.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "problems.hpp"
+
+namespace kfvm {
+
+namespace {
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+template <class T>
+T *dalloc(size_t n, const char *what) {
+  void *p = nullptr;
+  check(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), what);
+  return static_cast<T *>(p);
+}
+}  // namespace
+
+Solver::Solver(const Config &cfg) : cfg_(cfg) {
+  if (cfg_.dim != 2 && cfg_.dim != 3) throw std::runtime_error("problem:dim must be 2 or 3");
+  if (cfg_.rad != 2) throw std::runtime_error("only stencil radius 2 is built");
+  if (cfg_.nQuad != 2 && cfg_.nQuad != 3) throw std::runtime_error("problem:nquad must be 2 or 3");
+  if (cfg_.eq == Eq::MhdGlm) throw std::runtime_error("mhd_glm is not built yet");
+  if (cfg_.dim == 2) cfg_.nZ = 1;
+  const int64_t nmin = cfg_.dim == 3 ? std::min({cfg_.nX, cfg_.nY, cfg_.nZ}) : std::min(cfg_.nX, cfg_.nY);
+  if (nmin < 2) throw std::runtime_error("every direction needs at least 2 cells");
+  for (int a = 0; a < cfg_.dim; a++)
+    if ((cfg_.bc[2 * a] == BC::Periodic) != (cfg_.bc[2 * a + 1] == BC::Periodic))
+      throw std::runtime_error("periodic boundaries must be set on both sides of an axis");
+  if (cfg_.eq == Eq::LinAdv && cfg_.rs != Riemann::Exact)
+    throw std::runtime_error("linear advection uses problem:riemann exact");
+  if (cfg_.eq == Eq::Hydro && cfg_.rs == Riemann::Exact)
+    throw std::runtime_error("hydro needs problem:riemann hllc, hll or llf");
+  if (!problem_known(cfg_.problem, cfg_.eq))
+    throw std::runtime_error("unknown problem '" + cfg_.problem + "' for this equation");
+  if (cfg_.cfl <= 0.0) cfg_.cfl = cfg_.cflValue();
+
+  nv_ = num_vars(cfg_.eq);
+  ncell_ = cfg_.nCells();
+  if (ncell_ > (int64_t(1) << 31) - 1) throw std::runtime_error("grid too large for one block");
+  rk_ = rk_table(cfg_.rk);
+  wts_ = build_weights(cfg_.dim, cfg_.nQuad, cfg_.lfac, cfg_.dx(), cfg_.dy(), cfg_.dz());
+
+  grid_.nx = int(cfg_.nX);
+  grid_.ny = int(cfg_.nY);
+  grid_.nz = int(cfg_.dim == 3 ? cfg_.nZ : 1);
+  grid_.ncell = int(ncell_);
+  for (int f = 0; f < 6; f++) {
+    grid_.bc[f] = int(cfg_.bc[f]);
+    const int ax = f / 2;
+    grid_.negmask[f] = cfg_.bc[f] == BC::Reflecting ? (1u << (1 + ax))
+                       : cfg_.bc[f] == BC::Noslip   ? 0xEu
+                                                    : 0u;
+  }
+  grid_.dx = cfg_.dx();
+  grid_.dy = cfg_.dy();
+  grid_.dz = cfg_.dz();
+  grid_.gamma = cfg_.gamma;
+  const QuadRule qr = quad_rule(cfg_.nQuad);
+  for (int q = 0; q < 3; q++) grid_.wt[q] = q < qr.n ? qr.wt[q] : 0.0;
+  grid_.haveVel = cfg_.haveAdvVel ? 1 : 0;
+  if (cfg_.haveAdvVel) {
+    for (int d = 0; d < 3; d++) grid_.vel[d] = cfg_.advVel[d];
+  } else {
+    grid_.vel[0] = std::cos(cfg_.gamma);
+    grid_.vel[1] = std::sin(cfg_.gamma);
+    grid_.vel[2] = 0.0;
+  }
+  grid_.rs = int(cfg_.rs);
+
+  check(hipStreamCreate(&stream_), "hipStreamCreate");
+  const size_t nc = size_t(ncell_);
+  const size_t FQ = size_t(wts_.F) * wts_.Q;
+  U_[0] = dalloc<double>(nc * nv_, "U0");
+  U_[1] = dalloc<double>(nc * nv_, "U1");
+  Uprev_ = dalloc<double>(nc * nv_, "Uprev");
+  Uhat_ = dalloc<double>(nc * nv_, "Uhat");
+  K_ = dalloc<double>(nc * nv_, "K");
+  if (rk_.needsTmp()) Utmp_ = dalloc<double>(nc * nv_, "Utmp");
+  FS_ = dalloc<double>(nc * FQ * nv_, "face states");
+  WI_ = dalloc<double>(nc * FQ, "indicator");
+  theta_ = dalloc<double>(nc, "theta");
+  wflag_ = dalloc<double>(nc, "wflag");
+  list_ = dalloc<int>(nc, "weno list");
+  counter_ = dalloc<int>(1, "weno count");
+  faceW_ = dalloc<double>(wts_.face.size(), "face weights");
+  derivW_ = dalloc<double>(wts_.deriv.size(), "deriv weights");
+  nPart_ = flux_rk_partials(ncell_);
+  errPart_ = dalloc<double>(nPart_, "err partials");
+  posPart_ = dalloc<double>(nPart_, "pos partials");
+  redOut_ = dalloc<double>(2, "reduction");
+  vmax_ = dalloc<unsigned long long>(1, "vmax");
+  check(hipHostMalloc((void **)&hostRed_, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+
+  check(hipMemcpy(faceW_, wts_.face.data(), wts_.face.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  check(hipMemcpy(derivW_, wts_.deriv.data(), wts_.deriv.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  check(launch_fill(theta_, 1.0, nc, stream_), "fill theta");
+  check(launch_fill(wflag_, 0.0, nc, stream_), "fill wflag");
+  check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
+
+  std::vector<double> U0;
+  fill_initial_condition(cfg_, U0);
+  setState(U0.data());
+  useSparseWeno = cfg_.wenoThresh > 0.0;
+
+  // FSAL prime and the small first step (Solver.cpp:73-80)
+  const double maxVel = evalRHS(0);
+  dt = 0.01 * cfg_.cfl * cfg_.dmin() / maxVel;
+}
+
+Solver::~Solver() {
+  profileEnable(false);
+  void *bufs[] = {U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, theta_, wflag_, list_, counter_,
+                  faceW_, derivW_, errPart_, posPart_, redOut_, vmax_};
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (hostRed_) (void)hipHostFree(hostRed_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <class F>
+void Solver::timed(const char *name, F &&launch) {
+  if (!profOn_) {
+    check(launch(), name);
+    return;
+  }
+  hipEvent_t a, b;
+  check(hipEventCreate(&a), "hipEventCreate");
+  check(hipEventCreate(&b), "hipEventCreate");
+  check(hipEventRecord(a, stream_), "hipEventRecord");
+  check(launch(), name);
+  check(hipEventRecord(b, stream_), "hipEventRecord");
+  prof_[name].pending.emplace_back(a, b);
+}
+
+void Solver::harvest() {
+  check(hipStreamSynchronize(stream_), "sync");
+  for (auto &kv : prof_) {
+    for (auto &ev : kv.second.pending) {
+      float ms = 0.f;
+      check(hipEventElapsedTime(&ms, ev.first, ev.second), "hipEventElapsedTime");
+      kv.second.ms += ms;
+      kv.second.count++;
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+void Solver::profileEnable(bool on) {
+  if (!on && profOn_) harvest();
+  profOn_ = on;
+}
+void Solver::profileReset() {
+  harvest();
+  prof_.clear();
+}
+bool Solver::profileGet(const std::string &name, double &ms, int64_t &count) {
+  harvest();
+  auto it = prof_.find(name);
+  if (it == prof_.end()) {
+    ms = 0.0;
+    count = 0;
+    return false;
+  }
+  ms = it->second.ms;
+  count = it->second.count;
+  return true;
+}
+
+void Solver::pipeline(const double *Uin, const RKArgs &a) {
+  const int dim = cfg_.dim, nq = cfg_.nQuad;
+  const bool hydro = nv_ == 5;
+  if (useSparseWeno) {
+    timed("linear", [&] { return launch_linear(dim, nq, nv_, Uin, FS_, WI_, grid_, faceW_, stream_); });
+    if (hydro) {
+      const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
+      check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
+      timed("select", [&] {
+        return launch_select(dim, nq, Uin, WI_, theta_, wflag_, list_, counter_, wThresh, grid_, stream_);
+      });
+      timed("weno", [&] {
+        return launch_weno(dim, nq, nv_, Uin, FS_, grid_, faceW_, derivW_, list_, counter_, stream_);
+      });
+    }
+  } else {
+    timed("weno", [&] {
+      return launch_weno(dim, nq, nv_, Uin, FS_, grid_, faceW_, derivW_, nullptr, nullptr, stream_);
+    });
+    useSparseWeno = cfg_.wenoThresh > 0.0;
+  }
+  if (hydro) timed("pp", [&] { return launch_pp(dim, nq, Uin, FS_, theta_, grid_, stream_); });
+  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, a, stream_); });
+  nRhsEval++;
+}
+
+double Solver::readSpeed() {
+  check(hipMemcpyAsync(&hostRed_[2], vmax_, sizeof(double), hipMemcpyDeviceToHost, stream_), "read vmax");
+  check(hipStreamSynchronize(stream_), "sync");
+  double v;
+  std::memcpy(&v, &hostRed_[2], sizeof(double));
+  return v;
+}
+
+double Solver::evalRHS(int which) {
+  check(hipMemsetAsync(vmax_, 0, sizeof(unsigned long long), stream_), "clear vmax");
+  RKArgs a{};
+  a.mode = RK_PRIME;
+  a.K = K_;
+  a.vmaxBits = vmax_;
+  const double *Uin = which == 1 ? Uprev_ : (stateInCur_ ? U_[cur_] : Uprev_);
+  pipeline(Uin, a);
+  return readSpeed();
+}
+
+int Solver::takeStep() {
+  const bool fixed = cfg_.fixedDt > 0.0;
+  if (fixed) dt = cfg_.fixedDt;
+  if (time + dt > cfg_.finalTime) {
+    dt = cfg_.finalTime - time;
+    lastTimeStep = true;
+  }
+  const double dmin = cfg_.dmin();
+  const double nDofs = double(ncell_) * nv_;
+  bool accepted = false, firstUnphys = true;
+  int status = ACCEPTED;
+  stateInCur_ = false;
+  stagePos_ = 0;
+  for (int nT = 0; nT < cfg_.rejectionLimit; nT++) {
+    check(hipMemsetAsync(vmax_, 0, sizeof(unsigned long long), stream_), "clear vmax");
+    if (nT > 0) {
+      RKArgs a{};
+      a.mode = RK_PRIME;
+      a.K = K_;
+      a.vmaxBits = vmax_;
+      pipeline(Uprev_, a);
+    }
+    timed("stage_pre", [&] {
+      return launch_stage_pre(nv_, Uprev_, K_, U_[cur_], Uhat_, Utmp_, rk_.beta[0] * dt,
+                              rk_.bhat[0] * dt, int(ncell_), stream_);
+    });
+    for (int s = 1; s < rk_.nStages; s++) {
+      RKArgs a{};
+      a.mode = RK_STAGE;
+      a.g1 = rk_.gam1[s];
+      a.g2 = rk_.gam2[s];
+      a.g3 = rk_.gam3[s];
+      a.delta = rk_.delta[s];
+      a.betaDt = rk_.beta[s] * dt;
+      a.bhatDt = rk_.bhat[s] * dt;
+      a.Uprev = Uprev_;
+      a.Uhat = Uhat_;
+      a.Utmp = Utmp_;
+      a.Unext = U_[cur_ ^ 1];
+      a.vmaxBits = vmax_;
+      pipeline(U_[cur_], a);
+      cur_ ^= 1;
+    }
+    {
+      RKArgs a{};
+      a.mode = RK_FSAL;
+      a.bhatDt = rk_.bhatfsal * dt;
+      a.atol = cfg_.atol;
+      a.rtol = cfg_.rtol;
+      a.K = K_;
+      a.Uhat = Uhat_;
+      a.errPartial = errPart_;
+      a.posPartial = posPart_;
+      a.vmaxBits = vmax_;
+      pipeline(U_[cur_], a);
+    }
+    check(launch_finalize(errPart_, posPart_, nPart_, redOut_, stream_), "finalize");
+    check(hipMemcpyAsync(hostRed_, redOut_, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_), "read err");
+    check(hipMemcpyAsync(&hostRed_[2], vmax_, sizeof(double), hipMemcpyDeviceToHost, stream_), "read vmax");
+    check(hipMemcpyAsync(&hostRed_[3], counter_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read count");
+    check(hipStreamSynchronize(stream_), "sync");
+    const double errSum = hostRed_[0], posFlag = hostRed_[1];
+    double maxVel;
+    std::memcpy(&maxVel, &hostRed_[2], sizeof(double));
+    int cnt;
+    std::memcpy(&cnt, &hostRed_[3], sizeof(int));
+    lastNWeno = cnt;
+
+    const double errNew = 1.0 / std::sqrt(errSum / nDofs);
+    double dtfac = std::pow(errNew, rk_.ep1) * std::pow(errEst, rk_.ep2);
+    dtfac = std::isnan(dtfac) ? 1.2 : 1.0 + std::atan(dtfac - 1.0);
+    int lStat;
+    if (posFlag < 0.0) lStat = UNPHYSICAL;
+    else if (dtfac >= cfg_.rejectionThresh || fixed) lStat = ACCEPTED;
+    else lStat = TOLERANCE;
+    status = lStat;
+    if (lStat == UNPHYSICAL) {
+      dt = firstUnphys ? std::fmin(0.25 * cfg_.cfl * dmin / maxVel, dt / 4.0) : dt / 2.0;
+      firstUnphys = false;
+      useSparseWeno = false;
+      nRejectUnphys++;
+      lastTimeStep = false;
+    } else if (lStat == ACCEPTED) {
+      accepted = true;
+      time += dt;
+      const double dterr = dt * dtfac, dtcfl = cfg_.cfl * dmin / maxVel;
+      dt = fixed ? cfg_.fixedDt : std::fmin(dterr, dtcfl);
+      errEst = errNew;
+      std::swap(Uprev_, U_[cur_]);  // Uprev <- U
+      break;
+    } else {
+      dt = dt * dtfac;
+      nRejectThresh++;
+      lastTimeStep = false;
+    }
+  }
+  if (!accepted) {
+    lastTimeStep = true;
+    status = REJECT_LIMIT;
+    stateInCur_ = true;
+  }
+  if (dt < cfg_.minDt) {
+    lastTimeStep = true;
+    if (accepted) status = STAGNATED;
+  }
+  nTS++;
+  return status;
+}
+
+void Solver::advanceStages(int64_t n, double dtFixed) {
+  stateInCur_ = false;
+  for (int64_t it = 0; it < n; it++) {
+    if (stagePos_ == 0) {
+      check(hipMemsetAsync(vmax_, 0, sizeof(unsigned long long), stream_), "clear vmax");
+      timed("stage_pre", [&] {
+        return launch_stage_pre(nv_, Uprev_, K_, U_[cur_], Uhat_, Utmp_, rk_.beta[0] * dtFixed,
+                                rk_.bhat[0] * dtFixed, int(ncell_), stream_);
+      });
+      stagePos_ = 1;
+    }
+    RKArgs a{};
+    a.vmaxBits = vmax_;
+    if (stagePos_ < rk_.nStages) {
+      const int s = stagePos_;
+      a.mode = RK_STAGE;
+      a.g1 = rk_.gam1[s];
+      a.g2 = rk_.gam2[s];
+      a.g3 = rk_.gam3[s];
+      a.delta = rk_.delta[s];
+      a.betaDt = rk_.beta[s] * dtFixed;
+      a.bhatDt = rk_.bhat[s] * dtFixed;
+      a.Uprev = Uprev_;
+      a.Uhat = Uhat_;
+      a.Utmp = Utmp_;
+      a.Unext = U_[cur_ ^ 1];
+      pipeline(U_[cur_], a);
+      cur_ ^= 1;
+      stagePos_++;
+    } else {
+      a.mode = RK_FSAL;
+      a.bhatDt = rk_.bhatfsal * dtFixed;
+      a.atol = cfg_.atol;
+      a.rtol = cfg_.rtol;
+      a.K = K_;
+      a.Uhat = Uhat_;
+      a.errPartial = errPart_;
+      a.posPartial = posPart_;
+      pipeline(U_[cur_], a);
+      check(launch_finalize(errPart_, posPart_, nPart_, redOut_, stream_), "finalize");
+      std::swap(Uprev_, U_[cur_]);
+      time += dtFixed;
+      nTS++;
+      stagePos_ = 0;
+    }
+  }
+}
+
+int64_t Solver::solve() {
+  int64_t n = 0;
+  for (; nTS < cfg_.maxSteps && !lastTimeStep; n++) takeStep();
+  return n;
+}
+
+void Solver::getState(double *host) const {
+  const double *src = stateInCur_ ? U_[cur_] : Uprev_;
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipMemcpy(host, src, size_t(ncell_) * nv_ * sizeof(double), hipMemcpyDeviceToHost), "get state");
+}
+void Solver::setState(const double *host) {
+  const size_t bytes = size_t(ncell_) * nv_ * sizeof(double);
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipMemcpy(Uprev_, host, bytes, hipMemcpyHostToDevice), "set state");
+  check(hipMemcpy(U_[cur_], host, bytes, hipMemcpyHostToDevice), "set state");
+  stateInCur_ = false;
+}
+void Solver::getRHS(double *host) const {
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipMemcpy(host, K_, size_t(ncell_) * nv_ * sizeof(double), hipMemcpyDeviceToHost), "get rhs");
+}
+void Solver::getFlags(double *host) const {
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipMemcpy(host, wflag_, size_t(ncell_) * sizeof(double), hipMemcpyDeviceToHost), "get flags");
+  check(hipMemcpy(host + ncell_, theta_, size_t(ncell_) * sizeof(double), hipMemcpyDeviceToHost), "get flags");
+}
+void Solver::setFlags(const double *host) {
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipMemcpy(wflag_, host, size_t(ncell_) * sizeof(double), hipMemcpyHostToDevice), "set flags");
+  check(hipMemcpy(theta_, host + ncell_, size_t(ncell_) * sizeof(double), hipMemcpyHostToDevice), "set flags");
+}
+void Solver::setWeights(const double *face, const double *deriv) {
+  std::copy(face, face + wts_.face.size(), wts_.face.begin());
+  std::copy(deriv, deriv + wts_.deriv.size(), wts_.deriv.begin());
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipMemcpy(faceW_, wts_.face.data(), wts_.face.size() * sizeof(double), hipMemcpyHostToDevice), "set weights");
+  check(hipMemcpy(derivW_, wts_.deriv.data(), wts_.deriv.size() * sizeof(double), hipMemcpyHostToDevice), "set weights");
+}
+void Solver::sync() const { check(hipStreamSynchronize(stream_), "sync"); }
+
+}  // namespace kfvm

@@ -1,0 +1,98 @@
+// Host driver of the stage library: owns the device state of one problem and
+// runs the RK-FSAL step controller around the HIP stage kernels.
+//
+//   Solver::Solver   src/Solver.cpp:36-81      Solver::Solve    src/Solver.cpp:84-140
+//   Solver::TakeStep src/Solver.cpp:188-343    Solver::evalRHS  src/Solver.cpp:410-530
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../device/device_api.hpp"
+#include "common.hpp"
+#include "weights.hpp"
+
+namespace kfvm {
+
+class Solver {
+ public:
+  explicit Solver(const Config &cfg);
+  ~Solver();
+  Solver(const Solver &) = delete;
+  Solver &operator=(const Solver &) = delete;
+
+  // Solver::evalRHS on the current (which=0) or step-start (which=1) state;
+  // stores the FSAL register and returns the max wave speed.
+  double evalRHS(int which);
+  int takeStep();
+  int64_t solve();
+  // Advance exactly n RK stages of fixed-dt steps, queued on the device with
+  // no host synchronisation (a step completes every rkStages() stages).
+  void advanceStages(int64_t n, double dtFixed);
+
+  void getState(double *host) const;
+  void setState(const double *host);
+  void getRHS(double *host) const;
+  void getFlags(double *host) const;
+  void setFlags(const double *host);
+  void setWeights(const double *face, const double *deriv);
+  void sync() const;
+
+  const Config &config() const { return cfg_; }
+  const WeightTable &weights() const { return wts_; }
+  int nv() const { return nv_; }
+  int64_t ncell() const { return ncell_; }
+  int rkStages() const { return rk_.nStages; }
+
+  double time = 0.0, dt = 0.0, errEst = 1.0;
+  int64_t nTS = 1, nRhsEval = 0, nRejectUnphys = 0, nRejectThresh = 0, lastNWeno = 0;
+  bool lastTimeStep = false, useSparseWeno = true;
+
+  // per-kernel event timing
+  void profileEnable(bool on);
+  void profileReset();
+  bool profileGet(const std::string &name, double &ms, int64_t &count);
+
+ private:
+  struct Prof {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double ms = 0.0;
+    int64_t count = 0;
+  };
+  template <class F>
+  void timed(const char *name, F &&launch);
+  void harvest();
+
+  void pipeline(const double *Uin, const RKArgs &a);
+  double readSpeed();
+
+  Config cfg_;
+  RKTable rk_;
+  WeightTable wts_;
+  Grid grid_{};
+  int nv_ = 5;
+  int64_t ncell_ = 0;
+  hipStream_t stream_ = nullptr;
+
+  // device buffers
+  double *U_[2] = {nullptr, nullptr};  // ping-pong stage states
+  int cur_ = 0;
+  double *Uprev_ = nullptr, *Uhat_ = nullptr, *Utmp_ = nullptr, *K_ = nullptr;
+  double *FS_ = nullptr, *WI_ = nullptr, *theta_ = nullptr, *wflag_ = nullptr;
+  int *list_ = nullptr, *counter_ = nullptr;
+  double *faceW_ = nullptr, *derivW_ = nullptr;
+  double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;
+  unsigned long long *vmax_ = nullptr;
+  double *hostRed_ = nullptr;  // pinned: [err, pos, vmax bits, counter]
+  int nPart_ = 0;
+  bool stateInCur_ = false;  // last step was not accepted: the solution is in U_[cur_]
+  int stagePos_ = 0;         // position inside a step for advanceStages
+
+  bool profOn_ = false;
+  std::map<std::string, Prof> prof_;
+};
+
+}  // namespace kfvm

@@ -1,0 +1,271 @@
+"""Python mirror of the reference's Solver surface over the C ABI of
+libkfvm_amd.so (include/kfvm.h).
+
+    from kfvm_amd import Solver
+    s = Solver(init_text=open("riem.init").read())
+    s.solve()
+    U = s.state            # numpy [NV, nz, ny, nx]
+
+Method names follow KFVM::Solver (src/Solver.H:54-160): solve, take_step,
+eval_rhs.  The compute path is the HIP library only; importing this module on
+a machine without the built library or without a GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "build", "libkfvm_amd.so"))
+
+ACCEPTED, TOLERANCE, UNPHYSICAL, REJECT_LIMIT, STAGNATED = range(5)
+STATUS_NAMES = {ACCEPTED: "accepted", TOLERANCE: "tolerance", UNPHYSICAL: "unphysical",
+                REJECT_LIMIT: "reject_limit", STAGNATED: "stagnated"}
+
+# Every extern "C" symbol of include/kfvm.h.
+EXPORTS = [
+    "kfvm_create", "kfvm_create_from_text", "kfvm_destroy", "kfvm_solve", "kfvm_take_step",
+    "kfvm_eval_rhs", "kfvm_advance_stages", "kfvm_get_state", "kfvm_set_state", "kfvm_get_rhs", "kfvm_get_flags",
+    "kfvm_set_flags", "kfvm_get_time", "kfvm_set_time", "kfvm_get_info", "kfvm_get_counters",
+    "kfvm_force_full_weno", "kfvm_get_weights", "kfvm_set_weights", "kfvm_profile_enable",
+    "kfvm_profile_get", "kfvm_profile_reset", "kfvm_sync", "kfvm_weight_sizes",
+    "kfvm_build_weights", "kfvm_parse_init", "kfvm_last_error",
+]
+
+
+class KfvmInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "dim", "nq", "nv", "nx", "ny", "nz", "ncell", "nsub", "nfaces", "nqface",
+        "ncells_stencil", "nindic", "rk_stages")] + [
+        (n, ctypes.c_double) for n in ("dx", "dy", "dz", "gamma")]
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libkfvm_amd.so and declare the C signatures (raises if absent)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libkfvm_amd.so not built ({p}); run kfvm-kokkos_amd/build.py")
+    lib = ctypes.CDLL(p)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    I64 = ctypes.c_int64
+    sig = {
+        "kfvm_create": [ctypes.c_char_p, ctypes.POINTER(P)],
+        "kfvm_create_from_text": [ctypes.c_char_p, ctypes.POINTER(P)],
+        "kfvm_destroy": [P],
+        "kfvm_solve": [P, ctypes.POINTER(I64)],
+        "kfvm_take_step": [P, ctypes.POINTER(ctypes.c_int)],
+        "kfvm_eval_rhs": [P, ctypes.c_int, D],
+        "kfvm_advance_stages": [P, I64, ctypes.c_double],
+        "kfvm_get_state": [P, D, I64],
+        "kfvm_set_state": [P, D, I64],
+        "kfvm_get_rhs": [P, D, I64],
+        "kfvm_get_flags": [P, D, I64],
+        "kfvm_set_flags": [P, D, I64],
+        "kfvm_get_time": [P, D, D, ctypes.POINTER(I64)],
+        "kfvm_set_time": [P, ctypes.c_double, ctypes.c_double],
+        "kfvm_get_info": [P, ctypes.POINTER(KfvmInfo)],
+        "kfvm_get_counters": [P] + [ctypes.POINTER(I64)] * 4,
+        "kfvm_force_full_weno": [P],
+        "kfvm_get_weights": [P, D, I64, D, I64],
+        "kfvm_set_weights": [P, D, I64, D, I64],
+        "kfvm_profile_enable": [P, ctypes.c_int],
+        "kfvm_profile_get": [P, ctypes.c_char_p, D, ctypes.POINTER(I64)],
+        "kfvm_profile_reset": [P],
+        "kfvm_sync": [P],
+        "kfvm_weight_sizes": [ctypes.c_int, ctypes.c_int, ctypes.POINTER(I64), ctypes.POINTER(I64)],
+        "kfvm_build_weights": [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [D, I64, D, I64],
+        "kfvm_parse_init": [ctypes.c_char_p, ctypes.c_char_p, I64],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.kfvm_last_error.argtypes = []
+    lib.kfvm_last_error.restype = ctypes.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class KfvmError(RuntimeError):
+    pass
+
+
+def _check(lib, rc, name):
+    if rc != 0:
+        raise KfvmError(f"{name}: {lib.kfvm_last_error().decode()}")
+
+
+def build_weights(dim, nq, lfac, dx, dy, dz=1.0):
+    """Host-side GP weight tables (face [nsub,2*dim,Q,C], deriv [nsub,I,C])."""
+    lib = load_library()
+    nf, nd = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib, lib.kfvm_weight_sizes(dim, nq, ctypes.byref(nf), ctypes.byref(nd)), "kfvm_weight_sizes")
+    nsub, C, I = (8, 33, 19) if dim == 3 else (6, 13, 5)
+    Q = nq * nq if dim == 3 else nq
+    f = np.empty((nsub, 2 * dim, Q, C))
+    d = np.empty((nsub, I, C))
+    assert f.size == nf.value and d.size == nd.value
+    _check(lib, lib.kfvm_build_weights(dim, nq, lfac, dx, dy, dz, _dptr(f), f.size, _dptr(d), d.size),
+           "kfvm_build_weights")
+    return f, d
+
+
+def parse_init(text):
+    """The library's reading of an init file, as a dict."""
+    import json
+    lib = load_library()
+    buf = ctypes.create_string_buffer(8192)
+    _check(lib, lib.kfvm_parse_init(text.encode(), buf, len(buf)), "kfvm_parse_init")
+    return json.loads(buf.value.decode())
+
+
+class Solver:
+    """One problem on one GPU (KFVM::Solver)."""
+
+    def __init__(self, init_path=None, init_text=None):
+        self._lib = load_library()
+        self._h = ctypes.c_void_p()
+        if (init_path is None) == (init_text is None):
+            raise ValueError("give exactly one of init_path / init_text")
+        if init_path is not None:
+            self._call("kfvm_create", init_path.encode(), ctypes.byref(self._h))
+        else:
+            self._call("kfvm_create_from_text", init_text.encode(), ctypes.byref(self._h))
+        info = KfvmInfo()
+        self._call("kfvm_get_info", self._h, ctypes.byref(info))
+        self.info = info
+        self.shape = (info.nv, info.nz, info.ny, info.nx)
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, name)(*args)
+        if rc != 0:
+            raise KfvmError(f"{name}: {self._lib.kfvm_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.kfvm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- Solver members
+    def solve(self):
+        n = ctypes.c_int64()
+        self._call("kfvm_solve", self._h, ctypes.byref(n))
+        return n.value
+
+    def take_step(self):
+        st = ctypes.c_int()
+        self._call("kfvm_take_step", self._h, ctypes.byref(st))
+        return st.value
+
+    def eval_rhs(self, which=0):
+        v = ctypes.c_double()
+        self._call("kfvm_eval_rhs", self._h, int(which), ctypes.byref(v))
+        return v.value
+
+    def advance_stages(self, n, dt):
+        """Queue exactly n RK stages of fixed-dt steps (no host sync)."""
+        self._call("kfvm_advance_stages", self._h, int(n), float(dt))
+
+    def force_full_weno(self):
+        self._call("kfvm_force_full_weno", self._h)
+
+    # ---- data
+    @property
+    def state(self):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._call("kfvm_get_state", self._h, _dptr(a), a.size)
+        return a
+
+    @state.setter
+    def state(self, U):
+        a = np.ascontiguousarray(U, dtype=np.float64).reshape(self.shape)
+        self._call("kfvm_set_state", self._h, _dptr(a), a.size)
+
+    @property
+    def rhs(self):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._call("kfvm_get_rhs", self._h, _dptr(a), a.size)
+        return a
+
+    @property
+    def flags(self):
+        a = np.empty((2,) + self.shape[1:], dtype=np.float64)
+        self._call("kfvm_get_flags", self._h, _dptr(a), a.size)
+        return a
+
+    @flags.setter
+    def flags(self, F):
+        a = np.ascontiguousarray(F, dtype=np.float64).reshape((2,) + self.shape[1:])
+        self._call("kfvm_set_flags", self._h, _dptr(a), a.size)
+
+    @property
+    def time(self):
+        t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        self._call("kfvm_get_time", self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n))
+        return t.value, dt.value, n.value
+
+    def set_time(self, t, dt):
+        self._call("kfvm_set_time", self._h, float(t), float(dt))
+
+    @property
+    def counters(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        self._call("kfvm_get_counters", self._h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("rhs_evals", "rejects_tol", "rejects_unphys", "last_nweno"),
+                        [x.value for x in v]))
+
+    def weight_shapes(self):
+        i = self.info
+        return ((i.nsub, i.nfaces, i.nqface, i.ncells_stencil), (i.nsub, i.nindic, i.ncells_stencil))
+
+    @property
+    def weights(self):
+        fs, ds = self.weight_shapes()
+        f = np.empty(fs)
+        d = np.empty(ds)
+        self._call("kfvm_get_weights", self._h, _dptr(f), f.size, _dptr(d), d.size)
+        return f, d
+
+    def set_weights(self, face, deriv):
+        f = np.ascontiguousarray(face, dtype=np.float64)
+        d = np.ascontiguousarray(deriv, dtype=np.float64)
+        self._call("kfvm_set_weights", self._h, _dptr(f), f.size, _dptr(d), d.size)
+
+    # ---- device timing
+    def profile(self, on=True):
+        self._call("kfvm_profile_enable", self._h, 1 if on else 0)
+
+    def profile_reset(self):
+        self._call("kfvm_profile_reset", self._h)
+
+    def profile_get(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self._call("kfvm_profile_get", self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
+    def sync(self):
+        self._call("kfvm_sync", self._h)

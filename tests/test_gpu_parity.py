@@ -1,0 +1,130 @@
+"""Parity of the HIP stage library with the CPU oracle on the same inputs.
+
+Gate (per the path's contract): per-cell relative difference <= 1e-12 after one
+fixed-dt step, same weight table and initial state on both sides.  The
+relative difference of a cell is max_v |a_v - b_v| / max_v |b_v|.  Cells whose
+discrete decisions differ (WENO flag) are counted separately and must be zero
+here.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import cell_rel_diff, config, init_text, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-12
+
+
+def _pair(c):
+    import kfvm_amd
+    s = kfvm_amd.Solver(init_text=init_text(c))
+    parsed = kfvm_amd.parse_init(init_text(c))
+    f, d = s.weights
+    o = oracle_for(c, parsed, f, d)
+    return s, o
+
+
+def _start_oracle(s, o):
+    """Give the oracle the library's initial state and flags, then FSAL-prime it."""
+    U0 = s.state
+    o.state = U0
+    fl = np.zeros((2,) + s.shape[1:])
+    fl[1] = 1.0
+    o.flags = fl
+    o.prime()
+    return U0
+
+
+def _report(name, r, extra=""):
+    print(f"[parity] {name}: max cell rel diff {r.max():.3e} {extra}", flush=True)
+
+
+CASES = {
+    "riemann3d_sparse": config(n=(24, 24, 24), wenothresh=1.0, fixeddt=0.004),
+    "riemann3d_full": config(n=(24, 24, 24), wenothresh=-1.0, fixeddt=0.004),
+    "riemann3d_nq3_full": config(n=(16, 16, 16), nq=3, wenothresh=-1.0, fixeddt=0.004),
+    "riemann3d_hll_reflect": config(n=(16, 20, 12), riemann="hll", bc="reflecting", fixeddt=0.004),
+    "riemann3d_llf_noslip_threestar": config(n=(16, 16, 16), riemann="llf", bc="noslip",
+                                              rk="threestarp", fixeddt=0.004),
+    "quadrant2d_sparse": config(dim=2, n=(48, 40, 1), problem="quadrant2d", seed=7, user=(),
+                                fixeddt=0.002),
+    "quadrant2d_full_nq3": config(dim=2, n=(32, 32, 1), nq=3, problem="quadrant2d", seed=3,
+                                  user=(), wenothresh=-1.0, fixeddt=0.002),
+    "shear2d_periodic": config(dim=2, n=(48, 48, 1), problem="shear2d", seed=1, user=(),
+                               bc="periodic", fixeddt=0.002),
+    "linadv2d_full_nq3": config(dim=2, n=(32, 32, 1), nq=3, equation="linadv", riemann="exact",
+                                problem="linadv_sine", user=(1.0, 1.0, 0.0), bc="periodic",
+                                wenothresh=-1.0, advvel=(1.0, 1.0, 0.0), fixeddt=0.01),
+    "linadv3d_sparse": config(n=(16, 16, 16), equation="linadv", riemann="exact",
+                              problem="linadv_sine", user=(1.0, 1.0, 1.0), bc="periodic",
+                              advvel=(1.0, 0.5, -0.25), fixeddt=0.01),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_first_rhs_and_one_step(name):
+    c = CASES[name]
+    s, o = _pair(c)
+    _start_oracle(s, o)
+    # first RHS (the FSAL prime) and the flags it leaves
+    K_gpu, K_cpu = s.rhs, o.rhs
+    scale = max(np.abs(K_cpu).max(), 1e-300)
+    rk = np.abs(K_gpu - K_cpu).max() / scale
+    print(f"[parity] {name}: first RHS max diff / max|RHS| = {rk:.3e}", flush=True)
+    assert rk <= 1e-12
+    fg, fc = s.flags, o.flags
+    assert np.array_equal(fg[0], fc[0]), f"WENO flag flips: {int((fg[0] != fc[0]).sum())}"
+    assert np.abs(fg[1] - fc[1]).max() <= 1e-12
+    # one fixed-dt step
+    st_g = s.take_step()
+    st_c = o.take_step()
+    assert st_g == st_c == 0
+    r = cell_rel_diff(s.state, o.state)
+    flips = int((s.flags[0] != o.flags[0]).sum())
+    _report(name, r, f"flag flips {flips}")
+    assert flips == 0
+    assert r.max() <= TOL_STEP
+    tg, dtg, ng = s.time
+    tc, dtc, nc = o.time
+    assert tg == tc and ng == nc
+
+
+def test_three_adaptive_steps_match():
+    """Adaptive dt: the controller sees the same error norm up to summation order."""
+    c = config(n=(16, 16, 16), wenothresh=1.0)
+    s, o = _pair(c)
+    _start_oracle(s, o)
+    t0g, dt0g, _ = s.time
+    o.set_time(0.0, dt0g)
+    for _ in range(3):
+        assert s.take_step() == o.take_step()
+    tg, dtg, _ = s.time
+    tc, dtc, _ = o.time
+    assert abs(tg - tc) <= 1e-12 * tc
+    assert abs(dtg - dtc) <= 1e-10 * dtc
+    r = cell_rel_diff(s.state, o.state)
+    _report("adaptive 3 steps", r)
+    assert r.max() <= 1e-10
+
+
+def test_unphysical_rejection_forces_full_weno():
+    """cfl far too large: the step is rejected as unphysical, retried with
+    full WENO, exactly like the reference's state machine."""
+    c = config(n=(16, 16, 16), fixeddt=None, cfl=50.0)
+    s, o = _pair(c)
+    _start_oracle(s, o)
+    _, dt, _ = s.time
+    s.set_time(0.0, 0.2)
+    o.set_time(0.0, 0.2)
+    sg = s.take_step()
+    sc = o.take_step()
+    assert sg == sc
+    cnt = s.counters
+    assert cnt["rejects_unphys"] >= 1
+    r = cell_rel_diff(s.state, o.state)
+    _report("unphysical retry", r)
+    assert r.max() <= 1e-10
