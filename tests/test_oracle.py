@@ -1,0 +1,240 @@
+"""Pinning of the CPU oracle (no GPU): closed forms, invariants, convergence.
+
+The oracle restates the reference path; before it is trusted as the checker of
+the HIP library, each part is tied to something the mathematics fixes:
+  * free-stream preservation: a uniform state gives RHS == 0 exactly (periodic
+    and outflow, linear and full WENO) -- catches any consistency error of the
+    Riemann solvers, the reconstruction (sum of weights) or the divergence;
+  * discrete conservation with periodic boundaries: sum of RHS == 0;
+  * directional symmetry of the 3D planar Riemann problem under x<->y<->z;
+  * mirror symmetry with reflecting walls;
+  * convergence: linear advection of sin(2 pi (x+y)) converges at high order to
+    the exact cell averages (closed form) -- configuration "2D LinAdv" of the
+    path; Sod's shock tube converges to the exact Riemann solution in L1;
+  * RK tables: observed temporal order of each low-storage FSAL table on the
+    linear (no-WENO) advection semi-discretisation.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import kfvm_amd
+from helpers import config, init_text, oracle_for
+
+
+def make(c, threads=4):
+    text = init_text(c)
+    p = kfvm_amd.parse_init(text)
+    n = p["n"]
+    dims = [(p["hi"][a] - p["lo"][a]) / n[a] for a in range(3)]
+    fw, dw = kfvm_amd.build_weights(p["dim"], p["nq"], p["lfac"], dims[0], dims[1],
+                                    dims[2] if p["dim"] == 3 else 1.0)
+    return oracle_for(c, p, fw, dw, threads=threads), p
+
+
+@pytest.mark.parametrize("dim,wt,bc", [(3, 1.0, "periodic"), (3, -1.0, "outflow"),
+                                       (2, 1.0, "outflow"), (2, -1.0, "periodic"),
+                                       (3, -1.0, "periodic"), (2, -1.0, "outflow")])
+def test_free_stream_preservation(dim, wt, bc):
+    """Uniform state: every face of one direction sees the same (uL, uR) pair,
+    so interior cells have RHS == 0 exactly.  At an outflow boundary the outer
+    state is a copy of the inner one, which differs from the interior faces'
+    left state by the rounding of a permuted weight sum; those cells are at
+    round-off level only."""
+    c = config(dim=dim, n=(8, 8, 8) if dim == 3 else (12, 12, 1), bc=bc, problem="uniform",
+               user=(1.3, 0.4, -0.2, 0.1 if dim == 3 else 0.0, 2.0), wenothresh=wt)
+    o, p = make(c)
+    o.set_ic("uniform", c["user"])
+    v = o.eval_rhs(0)
+    assert v > 0
+    r = o.rhs
+    if bc == "periodic":
+        assert np.all(r == 0.0)
+    else:
+        inner = r[:, 1:-1, 1:-1, 1:-1] if dim == 3 else r[:, :, 1:-1, 1:-1]
+        assert np.all(inner == 0.0)
+        flux_scale = 5.0 * v / (1.0 / 8 if dim == 3 else 1.0 / 12)
+        assert np.abs(r).max() <= 1e-13 * flux_scale
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_periodic_conservation(dim):
+    c = config(dim=dim, n=(12, 12, 12) if dim == 3 else (24, 24, 1), bc="periodic",
+               problem="smooth_wave", user=(0.5, -0.3, 0.2), wenothresh=-1.0)
+    o, _ = make(c)
+    o.set_ic("smooth_wave", c["user"])
+    o.eval_rhs(0)
+    r = o.rhs
+    tot = r.reshape(r.shape[0], -1).sum(axis=1)
+    scale = np.abs(r).sum()
+    assert np.all(np.abs(tot) <= 1e-12 * scale)
+
+
+def test_planar_riemann_is_symmetric_under_axis_permutation():
+    c = config(n=(12, 12, 12), fixeddt=0.004, wenothresh=1.0)
+    o, _ = make(c)
+    o.set_ic("riemann3d_planar", (1.5,))
+    o.prime()
+    o.take_step()
+    U = o.state  # [v, z, y, x]
+    rho = U[0]
+    # Not bitwise: the stencil and face-point enumerations are not invariant
+    # under an axis permutation, so sums run in another order and the WENO
+    # weights amplify that rounding.  A wrong index or sign is O(1).
+    tol = 1e-10
+    assert np.allclose(rho, rho.transpose(0, 2, 1), rtol=0, atol=tol)
+    assert np.allclose(rho, rho.transpose(2, 1, 0), rtol=0, atol=tol)
+    assert np.allclose(U[1], U[2].transpose(0, 2, 1), rtol=0, atol=tol)
+    assert np.allclose(U[1], U[3].transpose(2, 1, 0), rtol=0, atol=tol)
+    assert np.array_equal(o.flags[0], o.flags[0].transpose(2, 1, 0))
+
+
+def test_reflecting_walls_mirror_symmetry():
+    # blast centred in a reflecting box stays mirror symmetric
+    c = config(dim=2, n=(16, 16, 1), bc="reflecting", problem="riemann3d_planar", user=(1.0,),
+               fixeddt=0.002)
+    o, _ = make(c)
+    o.set_ic("riemann3d_planar", (1.0,))
+    o.prime()
+    for _ in range(2):
+        o.take_step()
+    U = o.state[:, 0]  # [v, y, x]
+    # the IC x+y<1 is symmetric under x<->y
+    assert np.allclose(U[0], U[0].T, rtol=0, atol=1e-10)
+    assert np.allclose(U[1], U[2].T, rtol=0, atol=1e-10)
+
+
+def _linadv_l1(n, wenothresh, final=0.25):
+    c = config(dim=2, n=(n, n, 1), nq=3, equation="linadv", riemann="exact", problem="linadv_sine",
+               user=(1.0, 1.0, 0.0), bc="periodic", wenothresh=wenothresh, advvel=(1.0, 1.0, 0.0),
+               cfl=0.8, final=final, atol=1e-8, rtol=1e-8)
+    o, p = make(c)
+    o.set_ic("linadv_sine", (1.0, 1.0, 0.0))
+    o.prime()
+    o.solve()
+    t = o.time[0]
+    assert abs(t - p["final"]) < 1e-12
+    h = 1.0 / n
+    xc = (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(xc, xc, indexing="xy")
+    # exact cell average of sin(2 pi (x + y - 2 t)) over an h x h cell
+    s = math.sin(math.pi * h) / (math.pi * h)
+    exact = np.sin(2 * math.pi * (X + Y - 2 * t)) * s * s
+    return np.abs(o.state[0, 0] - exact).mean()
+
+
+def test_linadv_linear_scheme_converges_at_high_order():
+    # LinAdv never trips the KXRCF selector, so a positive threshold runs the
+    # linear GP reconstruction (degree-3 polynomial part + SE kernel)
+    e1 = _linadv_l1(16, 1.0)
+    e2 = _linadv_l1(32, 1.0)
+    assert e2 < 2e-5
+    assert math.log2(e1 / e2) > 4.5, (e1, e2)
+
+
+def test_linadv_full_weno_converges():
+    """Full KFVM-WENO on smooth data: the reference's indicator (beta squared
+    again, epsilon 1e-40) drops order near extrema, so only a modest L1 order
+    is expected; this pins the nonlinear path against the closed form."""
+    e1 = _linadv_l1(16, -1.0)
+    e2 = _linadv_l1(32, -1.0)
+    assert e2 < 1e-2
+    assert math.log2(e1 / e2) > 1.4, (e1, e2)
+
+
+def _sod_exact(x, t, gamma=1.4):
+    """Exact solution of Sod's problem (rho, u, p) at positions x - 0.5."""
+    rl, ul, pl, rr, ur, pr = 1.0, 0.0, 1.0, 0.125, 0.0, 0.1
+    cl, cr = math.sqrt(gamma * pl / rl), math.sqrt(gamma * pr / rr)
+    def f(p, rk, pk, ck):
+        if p > pk:
+            A, B = 2 / ((gamma + 1) * rk), (gamma - 1) / (gamma + 1) * pk
+            return (p - pk) * math.sqrt(A / (p + B))
+        return 2 * ck / (gamma - 1) * ((p / pk) ** ((gamma - 1) / (2 * gamma)) - 1)
+    lo, hi = 1e-6, 1.0
+    for _ in range(200):
+        m = 0.5 * (lo + hi)
+        if f(m, rl, pl, cl) + f(m, rr, pr, cr) + ur - ul > 0:
+            hi = m
+        else:
+            lo = m
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (ul + ur) + 0.5 * (f(ps, rr, pr, cr) - f(ps, rl, pl, cl))
+    rsl = rl * (ps / pl) ** (1 / gamma)
+    rsr = rr * ((ps / pr + (gamma - 1) / (gamma + 1)) / ((gamma - 1) / (gamma + 1) * ps / pr + 1))
+    S = ur + cr * math.sqrt((gamma + 1) / (2 * gamma) * ps / pr + (gamma - 1) / (2 * gamma))
+    csl = cl * (ps / pl) ** ((gamma - 1) / (2 * gamma))
+    out = np.zeros((3, len(x)))
+    for n, xx in enumerate(x):
+        s = xx / t
+        if s < ul - cl:
+            r, u, p = rl, ul, pl
+        elif s < us - csl:
+            u = 2 / (gamma + 1) * (cl + (gamma - 1) / 2 * ul + s)
+            c = 2 / (gamma + 1) * (cl + (gamma - 1) / 2 * (ul - s))
+            r = rl * (c / cl) ** (2 / (gamma - 1))
+            p = pl * (c / cl) ** (2 * gamma / (gamma - 1))
+        elif s < us:
+            r, u, p = rsl, us, ps
+        elif s < S:
+            r, u, p = rsr, us, ps
+        else:
+            r, u, p = rr, ur, pr
+        out[:, n] = (r, u, p)
+    return out
+
+
+@pytest.mark.parametrize("rs", ["hllc", "hll"])
+def test_sod_converges(rs):
+    errs = []
+    for n in (50, 100):
+        c = config(n=(n, 4, 4), riemann=rs, problem="sod_x", user=(0.5,), final=0.1,
+                   bc=("outflow", "outflow", "periodic", "periodic", "periodic", "periodic"),
+                   hi=(1.0, 4.0 / n, 4.0 / n), wenothresh=1.0, atol=1e-4, rtol=1e-4)
+        o, p = make(c)
+        o.set_ic("sod_x", (0.5,))
+        o.prime()
+        o.solve()
+        t = o.time[0]
+        x = (np.arange(n) + 0.5) / n - 0.5
+        ex = _sod_exact(x, t)
+        rho = o.state[0, 0, 0]
+        errs.append(np.abs(rho - ex[0]).mean())
+    assert errs[1] < 0.7 * errs[0]
+    assert errs[1] < 6e-3
+
+
+def _linadv_fixed_dt(rk, dt, n=16, final=0.2):
+    c = config(dim=2, n=(n, n, 1), equation="linadv", riemann="exact", problem="linadv_sine",
+               user=(1.0, 1.0, 0.0), bc="periodic", wenothresh=1.0, advvel=(1.0, 0.5, 0.0),
+               rk=rk, fixeddt=dt, final=final)
+    o, _ = make(c)
+    o.set_ic("linadv_sine", (1.0, 1.0, 0.0))
+    o.prime()
+    o.solve()
+    return o.state[0, 0]
+
+
+@pytest.mark.parametrize("rk,order", [("ssp4_3_2", 3), ("threestarp", 3), ("fourstarp", 4),
+                                      ("ssp10_4_3", 4)])
+def test_rk_tables_temporal_order(rk, order):
+    """Linear scheme (no WENO) on a fixed grid: a linear ODE system, so the
+    error against a tiny-dt run shows the order of the RK-FSAL table."""
+    ref = _linadv_fixed_dt(rk, 0.2 / 128)
+    e1 = np.abs(_linadv_fixed_dt(rk, 0.2 / 8) - ref).max()
+    e2 = np.abs(_linadv_fixed_dt(rk, 0.2 / 16) - ref).max()
+    assert math.log2(e1 / e2) > order - 0.4, (e1, e2)
+
+
+def test_uniform_state_steps_exactly():
+    c = config(n=(8, 8, 8), bc="periodic", problem="uniform", user=(1.0, 0.2, 0.1, -0.3, 1.5),
+               fixeddt=0.01)
+    o, _ = make(c)
+    o.set_ic("uniform", c["user"])
+    U0 = o.state
+    o.prime()
+    for _ in range(2):
+        assert o.take_step() == 0
+    # RHS is exactly zero; the stage combinations (1/3 U + 2/3 U) round
+    assert np.abs(o.state - U0).max() <= 4e-16 * np.abs(U0).max()
