@@ -41,19 +41,25 @@ struct RKArgs {
   unsigned long long *vmaxBits;  // running max wave speed (bit pattern of a non-negative double)
 };
 
+// Linear reconstruction, KXRCF indicator and positivity limiter (all cells).
+// props [2][ncell] receives each cell's internal energy and sound speed
+// (Euler), which the limiter bounds of this pass and of a following
+// launch_weno over a list read.
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
-                         const Grid &g, const double *W0, hipStream_t s);
+                         double *thetaNew, double *props, const Grid &g, const double *W0,
+                         hipStream_t s);
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                          double *wflag, int *list, int *counter, double wThresh, const Grid &g,
                          hipStream_t s);
-hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, const Grid &g,
-                       const double *FW, const double *DW, const int *list, const int *counter,
-                       hipStream_t s);
-hipError_t launch_pp(int dim, int nq, const double *U, double *FS, double *theta, const Grid &g,
-                     hipStream_t s);
+// KFVM-WENO and positivity limiter on the listed cells (list == nullptr: all
+// cells, and props is recomputed from U first; with a list, props must hold
+// the values launch_linear wrote for the same U).
+hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
+                       double *props, const Grid &g, const double *FW, const double *DW,
+                       const int *list, const int *counter, hipStream_t s);
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
                           const Grid &g, const RKArgs &a, hipStream_t s);
-int flux_rk_partials(int64_t ncell);
+int flux_rk_partials(int dim, const Grid &g);
 hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double *U, double *Uhat,
                             double *Utmp, double betaDt, double bhatDt, int ncell, hipStream_t s);
 hipError_t launch_finalize(const double *errPartial, const double *posPartial, int nPart,

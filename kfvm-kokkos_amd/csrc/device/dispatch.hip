@@ -15,9 +15,10 @@ namespace kfvm {
   } while (0)
 
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
-                         const Grid &g, const double *W0, hipStream_t s) {
+                         double *thetaNew, double *props, const Grid &g, const double *W0,
+                         hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, g, W0, s));
+  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, s));
 }
 
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
@@ -26,16 +27,11 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
   KFVM_SELECT(dim, nq, L::select(U, WI, theta, wflag, list, counter, wThresh, g, s));
 }
 
-hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, const Grid &g,
-                       const double *FW, const double *DW, const int *list, const int *counter,
-                       hipStream_t s) {
+hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
+                       double *props, const Grid &g, const double *FW, const double *DW,
+                       const int *list, const int *counter, hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, g, FW, DW, list, counter, s));
-}
-
-hipError_t launch_pp(int dim, int nq, const double *U, double *FS, double *theta, const Grid &g,
-                     hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::pp(U, FS, theta, g, s));
+  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, props, g, FW, DW, list, counter, s));
 }
 
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
@@ -44,7 +40,10 @@ hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double
   KFVM_SELECT(dim, nq, L::flux_rk(nv, U, FS, g, a, s));
 }
 
-int flux_rk_partials(int64_t ncell) { return int(stage_blocks(ncell)); }
+int flux_rk_partials(int dim, const Grid &g) {
+  const dim3 gr = dim == 3 ? Launch<3, 2>::flux_grid(g) : Launch<2, 2>::flux_grid(g);
+  return int(gr.x * gr.y * gr.z);
+}
 
 hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double *U, double *Uhat,
                             double *Utmp, double betaDt, double bhatDt, int ncell, hipStream_t s) {

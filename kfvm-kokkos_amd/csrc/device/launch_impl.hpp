@@ -11,12 +11,21 @@ inline unsigned stage_blocks(int64_t n) { return unsigned((n + dev::BLK - 1) / d
 
 template <int D, int NQ>
 struct Launch {
-  static hipError_t linear(int nv, const double *U, double *FS, double *WI, const Grid &g,
-                           const double *W0, hipStream_t s) {
-    if (nv == 5)
-      hipLaunchKernelGGL((dev::k_linear<D, NQ, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, WI, g, W0);
-    else
-      hipLaunchKernelGGL((dev::k_linear<D, NQ, 1>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, WI, g, W0);
+  static unsigned bricks(const Grid &g) {
+    using Bx = dev::RBox<D>;
+    return unsigned(((g.nx + Bx::BX - 1) / Bx::BX) * ((g.ny + Bx::BY - 1) / Bx::BY) *
+                    ((g.nz + Bx::BZ - 1) / Bx::BZ));
+  }
+  static hipError_t linear(int nv, const double *U, double *FS, double *WI, double *thetaNew,
+                           double *props, const Grid &g, const double *W0, hipStream_t s) {
+    if (nv == 5) {
+      hipLaunchKernelGGL((dev::k_linrec_brick<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, FS,
+                         props, g, W0);
+      hipLaunchKernelGGL((dev::k_linear_post<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U,
+                         FS, WI, thetaNew, props, g);
+    } else
+      hipLaunchKernelGGL((dev::k_linear<D, NQ, 1>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, WI,
+                         thetaNew, g, W0);
     return hipGetLastError();
   }
   static hipError_t select(const double *U, const double *WI, const double *theta, double *wflag,
@@ -25,25 +34,43 @@ struct Launch {
                        list, counter, wThresh, g);
     return hipGetLastError();
   }
-  static hipError_t weno(int nv, const double *U, double *FS, const Grid &g, const double *FW,
-                         const double *DW, const int *list, const int *counter, hipStream_t s) {
+  static hipError_t weno(int nv, const double *U, double *FS, double *thetaNew, double *props,
+                         const Grid &g, const double *FW, const double *DW, const int *list,
+                         const int *counter, hipStream_t s) {
+    if (nv == 5) {
+      // listed cells: props come from the linear pass over the same state
+      if (!list)
+        hipLaunchKernelGGL((dev::k_cell_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props,
+                           g);
+      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 1u << 16);
+      hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, FW, DW, list,
+                         counter);
+      const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
+      hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
+                         counter);
+      return hipGetLastError();
+    }
     const unsigned nb = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
-    if (nv == 5)
-      hipLaunchKernelGGL((dev::k_weno<D, NQ, 5>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, FW, DW, list, counter);
+    if (false)
+      ;
     else
-      hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, FW, DW, list, counter);
+      hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW,
+                         list, counter);
     return hipGetLastError();
   }
-  static hipError_t pp(const double *U, double *FS, double *theta, const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_pp<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, theta, g);
-    return hipGetLastError();
+  static dim3 flux_grid(const Grid &g) {
+    using Bk = dev::Brick<D>;
+    return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X), unsigned((g.ny + Bk::Y - 1) / Bk::Y),
+                unsigned((g.nz + Bk::Z - 1) / Bk::Z));
   }
   static hipError_t flux_rk(int nv, const double *U, const double *FS, const Grid &g,
                             const RKArgs &a, hipStream_t s) {
+    using Bk = dev::Brick<D>;
+    const dim3 blk(Bk::X, Bk::Y, Bk::Z);
     if (nv == 5)
-      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, g, a);
+      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 5>), flux_grid(g), blk, 0, s, U, FS, g, a);
     else
-      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 1>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, g, a);
+      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 1>), flux_grid(g), blk, 0, s, U, FS, g, a);
     return hipGetLastError();
   }
 };

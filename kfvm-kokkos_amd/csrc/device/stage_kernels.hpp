@@ -38,19 +38,17 @@ namespace dev {
 constexpr int BLK = 256;
 
 // ------------------------------------------------------------ index helpers
+// Source index along one axis for a (possibly ghost) cell index i, |i| <= n:
+// periodic wrap or mirror; side = 0 / 1 when mirrored at the low / high end.
+// Written with selects so a gather compiles without branches.
 __device__ inline int map_axis(int i, int n, int bcLo, int bcHi, int &side) {
-  side = -1;
-  if (i < 0) {
-    if (bcLo == 0) return i + n;  // periodic (|i| <= n)
-    side = 0;
-    return -i - 1;  // mirror
-  }
-  if (i >= n) {
-    if (bcHi == 0) return i - n;
-    side = 1;
-    return 2 * n - 1 - i;
-  }
-  return i;
+  const bool lo = i < 0, hi = i >= n;
+  const int per = lo ? i + n : (hi ? i - n : i);
+  const int mir = lo ? -i - 1 : (hi ? 2 * n - 1 - i : i);
+  const bool perLo = bcLo == 0, perHi = bcHi == 0;
+  const bool usePer = (lo && perLo) || (hi && perHi);
+  side = (lo && !perLo) ? 0 : ((hi && !perHi) ? 1 : -1);
+  return usePer ? per : mir;
 }
 
 // Cell average of variable v at (possibly ghost) cell (i,j,k).
@@ -64,11 +62,9 @@ __device__ inline double load_u(const double *__restrict__ U, const Grid &g, int
   if (D == 3) kk = map_axis(k, g.nz, g.bc[4], g.bc[5], sz);
   double x = U[size_t(v) * g.ncell + (size_t(kk) * g.ny + jj) * g.nx + ii];
   if (NV == 5) {
-    unsigned m = 0;
-    if (sx >= 0) m ^= g.negmask[sx];
-    if (sy >= 0) m ^= g.negmask[2 + sy];
-    if (D == 3 && sz >= 0) m ^= g.negmask[4 + sz];
-    if ((m >> v) & 1u) x = -x;
+    unsigned m = (sx >= 0 ? g.negmask[sx] : 0u) ^ (sy >= 0 ? g.negmask[2 + sy] : 0u);
+    if (D == 3) m ^= sz >= 0 ? g.negmask[4 + sz] : 0u;
+    x = ((m >> v) & 1u) ? -x : x;
   }
   return x;
 }
@@ -88,37 +84,173 @@ struct Shape {
   static constexpr int C = SInfo<D>::C;
 };
 
-// ------------------------------------------------------- linear reconstruction
+// ------------------------------------------------------ positivity limiter
+// Bounds of the Zhang-Shu-type limiter from the 3^D neighbourhood, widened by
+// the flattener (Common_K.H:329-387).
+struct PPBounds {
+  double rDens, maxDens, minDens, minEint;
+};
+
+template <int D>
+__device__ inline PPBounds pp_bounds(const double *__restrict__ U, const Grid &g, int i, int j,
+                                     int k, const double *ref) {
+  PPBounds b;
+  b.rDens = ref[0];
+  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
+#pragma unroll 1
+  for (int xo = -1; xo <= 1; xo++)
+#pragma unroll 1
+    for (int yo = -1; yo <= 1; yo++)
+#pragma unroll
+      for (int zo = zl; zo <= zh; zo++) {
+        double s[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) s[v] = load_u<D, 5>(U, g, i + xo, j + yo, k + zo, v);
+        maxDens = fmax(maxDens, s[0]);
+        minDens = fmin(minDens, s[0]);
+        const double ei = Hydro::eint(s);
+        const double sd = Hydro::sound(g.gamma, s);
+        minEint = fmin(minEint, ei);
+        minSdSp = fmin(minSdSp, sd);
+      }
+  const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
+                     load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
+  const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
+                     load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
+  double dzW;
+  if (D == 3)
+    dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
+          load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
+  else
+    dzW = ref[3] / ref[0] - ref[3] / ref[0];
+  const double divV = (dxU + dyV + dzW) / 2.0;
+  double kappa = 0.2;
+  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+  kappa *= (1.0 - eta);
+  b.maxDens = maxDens * (1.0 + kappa);
+  b.minDens = minDens * (1.0 - kappa);
+  b.minEint = minEint * (1.0 - kappa);
+  return b;
+}
+
+__device__ inline double dens_theta(double th, double rho, const PPBounds &b) {
+  th = rho > b.minDens ? th : fmin(th, (b.rDens - b.minDens) / (b.rDens - rho));
+  th = rho < b.maxDens ? th : fmin(th, (b.maxDens - b.rDens) / (rho - b.rDens));
+  return th;
+}
+
+__device__ inline double eint_pos(double emin, const double *ref, const double *s) {
+  double a = 0.0, b = 1.0, c;
+  double st[5];
+  for (int it = 0; it < 10; it++) {
+    c = 0.5 * (a + b);
+#pragma unroll
+    for (int v = 0; v < 5; v++) st[v] = ref[v] + c * (s[v] - ref[v]);
+    const double dec = Hydro::eint(st) - emin;
+    if (dec > 0.0) a = c;
+    else b = c;
+  }
+  return a;
+}
+
+template <int D, int NQ>
+__device__ inline void pp_scale(double *__restrict__ FS, size_t nc, int c, const double *ref,
+                                double th, bool replace) {
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+#pragma unroll 1
+  for (int fq = 0; fq < FQ; fq++)
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      double *p = FS + (size_t(fq) * 5 + v) * nc + c;
+      *p = replace ? ref[v] : ref[v] + th * (*p - ref[v]);
+    }
+}
+
+// The limiter on the face states of cell c held in FS, in the reference's
+// order: density theta, mix, internal-energy theta (10-step bisection), mix.
+template <int D, int NQ>
+__device__ inline double pp_apply(double *__restrict__ FS, size_t nc, int c, const double *ref,
+                                  const PPBounds &b) {
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  double thD = 1.0;
+#pragma unroll 1
+  for (int fq = 0; fq < FQ; fq++) thD = dens_theta(thD, FS[(size_t(fq) * 5) * nc + c], b);
+  if (thD > 0.0 && thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, false);
+  else if (thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, true);
+  double thP = 1.0;
+#pragma unroll 1
+  for (int fq = 0; fq < FQ; fq++) {
+    double s[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) s[v] = FS[(size_t(fq) * 5 + v) * nc + c];
+    const double ei = Hydro::eint(s);
+    thP = ei > b.minEint ? thP : fmin(thP, eint_pos(b.minEint, ref, s));
+  }
+  if (thP > 0.0 && thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, false);
+  else if (thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, true);
+  return fmin(thD, thP);
+}
+
+// ----------------------------------------- linear reconstruction + limiter
+// Face states from the full-stencil weights in conserved variables
+// (Stencil_K.H:32-90), the KXRCF indicator p/rho^gamma of each (unlimited)
+// state, and the positivity limiter: a fast path keeps the states when no
+// bound is violated, otherwise the exact reference sequence runs on FS.
 template <int D, int NQ, int NV>
 __global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
                                                 double *__restrict__ FS,
-                                                double *__restrict__ WI, Grid g,
+                                                double *__restrict__ WI,
+                                                double *__restrict__ thetaNew, Grid g,
                                                 const double *__restrict__ W) {
   constexpr auto ST = make_stencil_tables<D>();
-  constexpr int C = Shape<D, NQ>::C, F = Shape<D, NQ>::F, Q = Shape<D, NQ>::Q;
+  constexpr int C = Shape<D, NQ>::C, F = Shape<D, NQ>::F, Q = Shape<D, NQ>::Q, FQ = F * Q;
   const int c = blockIdx.x * BLK + threadIdx.x;
   if (c >= g.ncell) return;
   int i, j, k;
   cell_ijk<D>(c, g, i, j, k);
   const size_t nc = g.ncell;
-  double face[F * Q][NV];
+  double ref[NV];
 #pragma unroll
+  for (int v = 0; v < NV; v++) ref[v] = U[size_t(v) * nc + c];
+  PPBounds pb;
+  if (NV == 5) pb = pp_bounds<D>(U, g, i, j, k, ref);
+  double rhoF[FQ], msq[FQ];
+  double thD = 1.0;
+  bool okE = true;
+#pragma unroll 1
   for (int v = 0; v < NV; v++) {
     double vs[C];
 #pragma unroll
-    for (int n = 0; n < C; n++) vs[n] = load_u<D, NV>(U, g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2], v);
+    for (int n = 0; n < C; n++)
+      vs[n] = load_u<D, NV>(U, g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2], v);
 #pragma unroll
-    for (int fq = 0; fq < F * Q; fq++) {
+    for (int fq = 0; fq < FQ; fq++) {
       double acc = 0.0;
 #pragma unroll
       for (int n = 0; n < C; n++) acc = fma(W[fq * C + n], vs[n], acc);
-      face[fq][v] = acc;
       FS[(size_t(fq) * NV + v) * nc + c] = acc;
+      if (NV == 5) {
+        if (v == 0) {
+          rhoF[fq] = acc;
+          thD = dens_theta(thD, acc, pb);
+        } else if (v == 1) {
+          msq[fq] = acc * acc;
+        } else if (v < 4) {
+          msq[fq] = msq[fq] + acc * acc;
+        } else {
+          const double ei = acc - 0.5 * msq[fq] / rhoF[fq];
+          okE = okE && ei > pb.minEint;
+          WI[size_t(fq) * nc + c] = (g.gamma - 1.0) * ei / pow(rhoF[fq], g.gamma);
+        }
+      }
     }
   }
   if (NV == 5) {
-#pragma unroll
-    for (int fq = 0; fq < F * Q; fq++) WI[size_t(fq) * nc + c] = Hydro::weno(g.gamma, face[fq]);
+    double th = 1.0;
+    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, c, ref, pb);
+    thetaNew[c] = th;
   }
 }
 
@@ -186,12 +318,16 @@ __global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
 }
 
 // ---------------------------------------------------------------- KFVM-WENO
+// One cell: linearised-primitive stencil values, smoothness indicators of the
+// full / centre / biased substencils, WENO-AO weights, face states, back to
+// conserved variables.  The conserved energy is accumulated in FS in the
+// reference's operand order, so no per-point arrays stay live in registers.
 template <int D, int NQ, int NV>
 __device__ inline void weno_cell(const double *__restrict__ U, double *__restrict__ FS, const Grid &g,
                                  const double *__restrict__ FW, const double *__restrict__ DW,
                                  int c) {
   constexpr auto ST = make_stencil_tables<D>();
-  constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb, B = SInfo<D>::B;
+  constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb;
   constexpr int nSub = SInfo<D>::nSub, I = SInfo<D>::I;
   constexpr int F = Shape<D, NQ>::F, Q = Shape<D, NQ>::Q, FQ = F * Q;
   int i, j, k;
@@ -211,19 +347,19 @@ __device__ inline void weno_cell(const double *__restrict__ U, double *__restric
     vsq = lp.vsq;
     gamm = lp.gamm;
   }
-  // linear-weight gammas of the WENO-AO combination
-  const double gHi = 0.7, gLo = 0.7;
-  double gm[nSub];
-  gm[0] = gHi;
-  gm[1] = (1.0 - gHi) * gLo;
-#pragma unroll
-  for (int s = 2; s < nSub; s++) gm[s] = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
+  // linear weights of the WENO-AO combination
+  constexpr double gHi = 0.7, gLo = 0.7;
+  constexpr double g0 = gHi, g1 = (1.0 - gHi) * gLo, gb = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
 
-  double densOut[FQ], eAcc[FQ];
-#pragma unroll
+#pragma unroll 1
   for (int v = 0; v < NV; v++) {
     // stencil values of variable v in linearised primitive form
     double vs[C];
+    double uv = 0.0, ruv = 0.0;
+    if (NV == 5 && v >= 1 && v <= 3) {
+      uv = v == 1 ? uu[0] : (v == 2 ? uu[1] : uu[2]);
+      ruv = rho * uv;
+    }
 #pragma unroll
     for (int n = 0; n < C; n++) {
       const int a = i + ST.off[n][0], b = j + ST.off[n][1], d = k + ST.off[n][2];
@@ -233,7 +369,7 @@ __device__ inline void weno_cell(const double *__restrict__ U, double *__restric
           vs[n] = s0;
         } else if (v < 4) {
           const double sv = load_u<D, NV>(U, g, a, b, d, v);
-          vs[n] = (sv - uu[v - 1] * s0) / rho;
+          vs[n] = (sv - uv * s0) / rho;
         } else {
           const double s1 = load_u<D, NV>(U, g, a, b, d, 1);
           const double s2 = load_u<D, NV>(U, g, a, b, d, 2);
@@ -253,12 +389,12 @@ __device__ inline void weno_cell(const double *__restrict__ U, double *__restric
       double beta = 0.0;
 #pragma unroll
       for (int ii = 0; ii < I; ii++) {
-        const double *d = DW + (s * I + ii) * C;
+        const double *dd = DW + (s * I + ii) * C;
         double t = 0.0;
 #pragma unroll
         for (int m = 0; m < n; m++) {
           const int cc = s < 2 ? m : ST.sub[s - 2][m];
-          t = fma(d[m], vs[cc], t);
+          t = fma(dd[m], vs[cc], t);
         }
         beta = fma(t, t, beta);
       }
@@ -268,52 +404,60 @@ __device__ inline void weno_cell(const double *__restrict__ U, double *__restric
     double tot = 0.0;
 #pragma unroll
     for (int s = 0; s < nSub; s++) {
-      om[s] = gm[s] / (om[s] * om[s] + 1.e-40);
+      const double gs = s == 0 ? g0 : (s == 1 ? g1 : gb);
+      om[s] = gs / (om[s] * om[s] + 1.e-40);
       tot += om[s];
     }
 #pragma unroll
     for (int s = 0; s < nSub; s++) om[s] /= tot;
-    om[0] /= gm[0];
+    om[0] /= g0;
 #pragma unroll
-    for (int s = 1; s < nSub; s++) om[s] -= om[0] * gm[s];
+    for (int s = 1; s < nSub; s++) om[s] -= om[0] * (s == 1 ? g1 : gb);
 
-    // face values: sum_s omega_s (W_s . V_s), then back to conserved variables
+    // face values sum_s omega_s (W_s . V_s), converted back to conserved
+#pragma unroll 1
+    for (int f = 0; f < F; f++) {
 #pragma unroll
-    for (int fq = 0; fq < FQ; fq++) {
-      double acc = 0.0;
+      for (int q = 0; q < Q; q++) {
+        const int fq = f * Q + q;
+        double acc = 0.0;
 #pragma unroll
-      for (int s = 0; s < nSub; s++) {
-        const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
-        const double *w = FW + (s * FQ + fq) * C;
-        double t = 0.0;
+        for (int s = 0; s < nSub; s++) {
+          const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
+          const double *w = FW + (s * FQ + fq) * C;
+          double t = 0.0;
 #pragma unroll
-        for (int m = 0; m < n; m++) {
-          const int cc = s < 2 ? m : ST.sub[s - 2][m];
-          t = fma(w[m], vs[cc], t);
+          for (int m = 0; m < n; m++) {
+            const int cc = s < 2 ? m : ST.sub[s - 2][m];
+            t = fma(w[m], vs[cc], t);
+          }
+          acc = fma(om[s], t, acc);
         }
-        acc = fma(om[s], t, acc);
-      }
-      double out = acc;
-      if (NV == 5) {
-        if (v == 0) {
-          densOut[fq] = acc;
-          eAcc[fq] = vsq * acc;
-        } else if (v < 4) {
-          out = rho * acc + uu[v - 1] * densOut[fq];
-          eAcc[fq] = eAcc[fq] + rho * uu[v - 1] * acc;
+        double *out = FS + (size_t(fq) * NV + v) * nc + c;
+        if (NV == 5) {
+          double *dens = FS + (size_t(fq) * NV + 0) * nc + c;
+          double *etot = FS + (size_t(fq) * NV + 4) * nc + c;
+          if (v == 0) {
+            *out = acc;
+            *etot = vsq * acc;
+          } else if (v < 4) {
+            *out = rho * acc + uv * (*dens);
+            *etot = *etot + ruv * acc;
+          } else {
+            *out = *etot + acc / gamm;
+          }
         } else {
-          out = eAcc[fq] + acc / gamm;
+          *out = acc;
         }
       }
-      FS[(size_t(fq) * NV + v) * nc + c] = out;
     }
   }
-  (void)B;
 }
 
 template <int D, int NQ, int NV>
 __global__ __launch_bounds__(BLK) void k_weno(const double *__restrict__ U,
-                                              double *__restrict__ FS, Grid g,
+                                              double *__restrict__ FS,
+                                              double *__restrict__ thetaNew, Grid g,
                                               const double *__restrict__ FW,
                                               const double *__restrict__ DW,
                                               const int *__restrict__ list,
@@ -322,110 +466,16 @@ __global__ __launch_bounds__(BLK) void k_weno(const double *__restrict__ U,
   for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
     const int c = list ? list[t] : t;
     weno_cell<D, NQ, NV>(U, FS, g, FW, DW, c);
-  }
-}
-
-// ------------------------------------------------------ positivity limiter
-template <int D, int NQ>
-__device__ inline void pp_scale(double *__restrict__ FS, size_t nc, int c, const double *ref,
-                                double th, bool replace) {
-  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+    if (NV == 5) {
+      int i, j, k;
+      cell_ijk<D>(c, g, i, j, k);
+      double ref[5];
 #pragma unroll
-  for (int fq = 0; fq < FQ; fq++)
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-      double *p = FS + (size_t(fq) * 5 + v) * nc + c;
-      *p = replace ? ref[v] : ref[v] + th * (*p - ref[v]);
+      for (int v = 0; v < 5; v++) ref[v] = U[size_t(v) * g.ncell + c];
+      const PPBounds pb = pp_bounds<D>(U, g, i, j, k, ref);
+      thetaNew[c] = pp_apply<D, NQ>(FS, size_t(g.ncell), c, ref, pb);
     }
-}
-
-__device__ inline double eint_pos(double emin, const double *ref, const double *s) {
-  double a = 0.0, b = 1.0, c;
-  double st[5];
-  for (int it = 0; it < 10; it++) {
-    c = 0.5 * (a + b);
-#pragma unroll
-    for (int v = 0; v < 5; v++) st[v] = ref[v] + c * (s[v] - ref[v]);
-    const double dec = Hydro::eint(st) - emin;
-    if (dec > 0.0) a = c;
-    else b = c;
   }
-  return a;
-}
-
-template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_pp(const double *__restrict__ U, double *__restrict__ FS,
-                                            double *__restrict__ theta, Grid g) {
-  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
-  int i, j, k;
-  cell_ijk<D>(c, g, i, j, k);
-  const size_t nc = g.ncell;
-  double ref[5];
-#pragma unroll
-  for (int v = 0; v < 5; v++) ref[v] = U[size_t(v) * nc + c];
-  const double rDens = ref[0];
-  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
-  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
-  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
-#pragma unroll
-  for (int xo = -1; xo <= 1; xo++)
-#pragma unroll
-    for (int yo = -1; yo <= 1; yo++)
-#pragma unroll
-      for (int zo = zl; zo <= zh; zo++) {
-        double s[5];
-#pragma unroll
-        for (int v = 0; v < 5; v++) s[v] = load_u<D, 5>(U, g, i + xo, j + yo, k + zo, v);
-        maxDens = fmax(maxDens, s[0]);
-        minDens = fmin(minDens, s[0]);
-        const double ei = Hydro::eint(s);
-        const double sd = Hydro::sound(g.gamma, s);
-        minEint = fmin(minEint, ei);
-        minSdSp = fmin(minSdSp, sd);
-      }
-  const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
-                     load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
-  const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
-                     load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
-  double dzW;
-  if (D == 3)
-    dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
-          load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
-  else
-    dzW = ref[3] / ref[0] - ref[3] / ref[0];
-  const double divV = (dxU + dyV + dzW) / 2.0;
-  double kappa = 0.2;
-  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
-  kappa *= (1.0 - eta);
-  maxDens *= (1.0 + kappa);
-  minDens *= (1.0 - kappa);
-  minEint *= (1.0 - kappa);
-
-  const double dP = maxDens - rDens, dM = rDens - minDens;
-  double thD = 1.0;
-#pragma unroll
-  for (int fq = 0; fq < FQ; fq++) {
-    const double rho = FS[(size_t(fq) * 5) * nc + c];
-    thD = rho > minDens ? thD : fmin(thD, dM / (rDens - rho));
-    thD = rho < maxDens ? thD : fmin(thD, dP / (rho - rDens));
-  }
-  if (thD > 0.0 && thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, false);
-  else if (thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, true);
-
-  double thP = 1.0;
-#pragma unroll
-  for (int fq = 0; fq < FQ; fq++) {
-    double s[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) s[v] = FS[(size_t(fq) * 5 + v) * nc + c];
-    const double ei = Hydro::eint(s);
-    thP = ei > minEint ? thP : fmin(thP, eint_pos(minEint, ref, s));
-  }
-  if (thP > 0.0 && thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, false);
-  else if (thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, true);
-  theta[c] = fmin(thD, thP);
 }
 
 // -------------------------------------------- Riemann + divergence + RK update
@@ -446,24 +496,24 @@ __device__ inline double riemann(const Grid &g, const double *uL, const double *
 }
 
 // Block-wide reductions with a fixed combination order (deterministic).
-__device__ inline double block_sum(double x, double *sh) {
-  sh[threadIdx.x] = x;
+__device__ inline double block_sum(double x, double *sh, int tid) {
+  sh[tid] = x;
   __syncthreads();
   for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    if (tid < s) sh[tid] += sh[tid + s];
     __syncthreads();
   }
   const double r = sh[0];
   __syncthreads();
   return r;
 }
-__device__ inline double block_min(double x, double *sh) {
-  sh[threadIdx.x] = x;
+__device__ inline double block_min(double x, double *sh, int tid) {
+  sh[tid] = x;
   __syncthreads();
   for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      const double o = sh[threadIdx.x + s];
-      sh[threadIdx.x] = sh[threadIdx.x] > o ? o : sh[threadIdx.x];
+    if (tid < s) {
+      const double o = sh[tid + s];
+      sh[tid] = sh[tid] > o ? o : sh[tid];
     }
     __syncthreads();
   }
@@ -471,13 +521,13 @@ __device__ inline double block_min(double x, double *sh) {
   __syncthreads();
   return r;
 }
-__device__ inline double block_max(double x, double *sh) {
-  sh[threadIdx.x] = x;
+__device__ inline double block_max(double x, double *sh, int tid) {
+  sh[tid] = x;
   __syncthreads();
   for (int s = BLK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      const double o = sh[threadIdx.x + s];
-      sh[threadIdx.x] = o > sh[threadIdx.x] ? o : sh[threadIdx.x];
+    if (tid < s) {
+      const double o = sh[tid + s];
+      sh[tid] = o > sh[tid] ? o : sh[tid];
     }
     __syncthreads();
   }
@@ -486,85 +536,143 @@ __device__ inline double block_max(double x, double *sh) {
   return r;
 }
 
+// Brick of cells handled by one workgroup of the flux kernel.
+template <int D>
+struct Brick {
+  static constexpr int X = D == 3 ? 8 : 16, Y = D == 3 ? 8 : 16, Z = D == 3 ? 4 : 1;
+  static constexpr int NXF = (X + 1) * Y * Z, NYF = X * (Y + 1) * Z, NZF = D == 3 ? X * Y * (Z + 1) : 0;
+  static constexpr int NF = NXF + NYF + NZF;
+};
+
+// Face state of cell (ci,cj,ck) on its face f at point q; cells outside the
+// domain resolve through periodic wrap or the face boundary condition (copy of
+// the inner state, with the mirror's sign for reflecting / noslip walls).
+template <int D, int NQ, int NV>
+__device__ inline void face_state(const double *__restrict__ FS, const Grid &g, int ci, int cj,
+                                  int ck, int f, int q, const double *inner, double *out) {
+  constexpr int Q = Shape<D, NQ>::Q;
+  const int ax = f >> 1;
+  const int n_ax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+  int pos = ax == 0 ? ci : (ax == 1 ? cj : ck);
+  if (pos < 0 || pos >= n_ax) {
+    const int bface = pos < 0 ? 2 * ax : 2 * ax + 1;
+    if (g.bc[bface] != 0) {
+      const unsigned m = NV == 5 ? g.negmask[bface] : 0u;
+#pragma unroll
+      for (int v = 0; v < NV; v++) out[v] = ((m >> v) & 1u) ? -inner[v] : inner[v];
+      return;
+    }
+    pos = pos < 0 ? pos + n_ax : pos - n_ax;
+    if (ax == 0) ci = pos;
+    else if (ax == 1) cj = pos;
+    else ck = pos;
+  }
+  const size_t cell = (size_t(ck) * g.ny + cj) * g.nx + ci;
+  const size_t nc = g.ncell;
+#pragma unroll
+  for (int v = 0; v < NV; v++) out[v] = FS[(size_t(f * Q + q) * NV + v) * nc + cell];
+}
+
 template <int D, int NQ, int NV>
 __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
                                                  const double *__restrict__ FS, Grid g,
                                                  RKArgs a) {
-  constexpr int Q = Shape<D, NQ>::Q, F = Shape<D, NQ>::F;
+  using Bk = Brick<D>;
+  constexpr int Q = Shape<D, NQ>::Q;
+  __shared__ double fl[NV][Bk::NF];
   __shared__ double sh[BLK];
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  const bool live = c < g.ncell;
+  const int tid = threadIdx.x + Bk::X * (threadIdx.y + Bk::Y * threadIdx.z);
+  const int i0 = blockIdx.x * Bk::X, j0 = blockIdx.y * Bk::Y, k0 = blockIdx.z * Bk::Z;
+  const int ci = i0 + threadIdx.x, cj = j0 + threadIdx.y, ck = k0 + threadIdx.z;
+  const bool live = ci < g.nx && cj < g.ny && (D == 2 || ck < g.nz);
+  const size_t nc = g.ncell;
+  const size_t c = live ? (size_t(ck) * g.ny + cj) * g.nx + ci : 0;
   double vmax = 0.0, errLoc = 0.0, posLoc = 1.0;
-  if (live) {
-    int i, j, k;
-    cell_ijk<D>(c, g, i, j, k);
-    const size_t nc = g.ncell;
-    // neighbours across each face (or -1 for a non-periodic domain boundary)
-    size_t nb[F];
+  double rhs[NV];
 #pragma unroll
-    for (int f = 0; f < F; f++) {
-      const int ax = f >> 1, hi = f & 1;
-      int ni = i, nj = j, nk = k;
-      if (ax == 0) ni += hi ? 1 : -1;
-      if (ax == 1) nj += hi ? 1 : -1;
-      if (ax == 2) nk += hi ? 1 : -1;
-      const int n_ax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
-      const int pos = ax == 0 ? ni : (ax == 1 ? nj : nk);
-      bool inside = pos >= 0 && pos < n_ax;
-      if (!inside && g.bc[f] == 0) {
-        inside = true;
-        if (ax == 0) ni = (ni + g.nx) % g.nx;
-        if (ax == 1) nj = (nj + g.ny) % g.ny;
-        if (ax == 2) nk = (nk + g.nz) % g.nz;
+  for (int v = 0; v < NV; v++) rhs[v] = 0.0;
+
+#pragma unroll 1
+  for (int q = 0; q < Q; q++) {
+    // every face of the brick at point q, solved once
+#pragma unroll 1
+    for (int t = tid; t < Bk::NF; t += BLK) {
+      int dir, fi, fj, fk;
+      if (t < Bk::NXF) {
+        dir = 0;
+        fi = t % (Bk::X + 1);
+        fj = (t / (Bk::X + 1)) % Bk::Y;
+        fk = t / ((Bk::X + 1) * Bk::Y);
+      } else if (t < Bk::NXF + Bk::NYF) {
+        const int u = t - Bk::NXF;
+        dir = 1;
+        fi = u % Bk::X;
+        fj = (u / Bk::X) % (Bk::Y + 1);
+        fk = u / (Bk::X * (Bk::Y + 1));
+      } else {
+        const int u = t - Bk::NXF - Bk::NYF;
+        dir = 2;
+        fi = u % Bk::X;
+        fj = (u / Bk::X) % Bk::Y;
+        fk = u / (Bk::X * Bk::Y);
       }
-      nb[f] = inside ? (size_t(nk) * g.ny + nj) * g.nx + ni : ~size_t(0);
-    }
-    double rhs[NV];
+      // right cell (the face is its lower face) and left cell
+      const int ri = i0 + fi, rj = j0 + fj, rk = k0 + fk;
+      const int li = ri - (dir == 0), lj = rj - (dir == 1), lk = rk - (dir == 2);
+      const bool rIn = ri < g.nx && rj < g.ny && (D == 2 || rk < g.nz);
+      const bool lIn = li >= 0 && lj >= 0 && lk >= 0 && li < g.nx && lj < g.ny && (D == 2 || lk < g.nz);
+      double Fl[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) rhs[v] = 0.0;
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      double fl[F][NV];
-#pragma unroll
-      for (int f = 0; f < F; f++) {
-        double in[NV], out[NV];
-#pragma unroll
-        for (int v = 0; v < NV; v++) in[v] = FS[(size_t(f * Q + q) * NV + v) * nc + c];
-        if (nb[f] != ~size_t(0)) {
-#pragma unroll
-          for (int v = 0; v < NV; v++) out[v] = FS[(size_t((f ^ 1) * Q + q) * NV + v) * nc + nb[f]];
+      for (int v = 0; v < NV; v++) Fl[v] = 0.0;
+      // a face is needed when one of its cells is a live cell of this brick
+      const bool rBrick = fi < Bk::X && fj < Bk::Y && fk < Bk::Z && rIn;
+      const bool lBrick = (dir == 0 ? fi > 0 : (dir == 1 ? fj > 0 : fk > 0)) && lIn;
+      if (rBrick || lBrick) {
+        double uL[NV], uR[NV];
+        const int lo = 2 * dir + 1, hi = 2 * dir;  // left cell's upper face, right cell's lower face
+        if (rIn) {
+          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, nullptr, uR);
+          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, uR, uL);
         } else {
-          const unsigned m = NV == 5 ? g.negmask[f] : 0u;
-#pragma unroll
-          for (int v = 0; v < NV; v++) out[v] = ((m >> v) & 1u) ? -in[v] : in[v];
+          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, nullptr, uL);
+          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, uL, uR);
         }
-        const double *uL = (f & 1) ? in : out;
-        const double *uR = (f & 1) ? out : in;
-        double s;
-        if ((f >> 1) == 0) s = riemann<NV, 0>(g, uL, uR, fl[f]);
-        else if ((f >> 1) == 1) s = riemann<NV, 1>(g, uL, uR, fl[f]);
-        else s = riemann<NV, 2>(g, uL, uR, fl[f]);
-        vmax = s > vmax ? s : vmax;
+        double sp;
+        if (dir == 0) sp = riemann<NV, 0>(g, uL, uR, Fl);
+        else if (dir == 1) sp = riemann<NV, 1>(g, uL, uR, Fl);
+        else sp = riemann<NV, 2>(g, uL, uR, Fl);
+        vmax = sp > vmax ? sp : vmax;
       }
-      // flux divergence in the reference's accumulation order
+#pragma unroll
+      for (int v = 0; v < NV; v++) fl[v][t] = Fl[v];
+    }
+    __syncthreads();
+    if (live) {
+      const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+      const int xW = tx + (Bk::X + 1) * (ty + Bk::Y * tz), xE = xW + 1;
+      const int yS = Bk::NXF + tx + Bk::X * (ty + (Bk::Y + 1) * tz), yN = yS + Bk::X;
+      const int zB = Bk::NXF + Bk::NYF + tx + Bk::X * (ty + Bk::Y * tz), zT = zB + Bk::X * Bk::Y;
       if (D == 2) {
         const double w = g.wt[q];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          rhs[v] += w * (fl[0][v] - fl[1][v]) / g.dx;
-          rhs[v] += w * (fl[2][v] - fl[3][v]) / g.dy;
+          rhs[v] += w * (fl[v][xW] - fl[v][xE]) / g.dx;
+          rhs[v] += w * (fl[v][yS] - fl[v][yN]) / g.dy;
         }
       } else {
         const double w = g.wt[q / NQ] * g.wt[q % NQ];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          rhs[v] += w * (fl[0][v] - fl[1][v]) / g.dx;
-          rhs[v] += w * (fl[2][v] - fl[3][v]) / g.dy;
-          rhs[v] += w * (fl[4][v] - fl[5][v]) / g.dz;
+          rhs[v] += w * (fl[v][xW] - fl[v][xE]) / g.dx;
+          rhs[v] += w * (fl[v][yS] - fl[v][yN]) / g.dy;
+          rhs[v] += w * (fl[v][zB] - fl[v][zT]) / g.dz;
         }
       }
     }
-    // RK register update
+    __syncthreads();
+  }
+
+  if (live) {
     if (a.mode == RK_PRIME) {
 #pragma unroll
       for (int v = 0; v < NV; v++) a.K[size_t(v) * nc + c] = rhs[v];
@@ -599,18 +707,26 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
       posLoc = hasNan ? -1.0 : (NV == 5 ? Hydro::positivity(st) : 1.0);
     }
   }
-  const double bmax = block_max(vmax, sh);
-  if (threadIdx.x == 0 && bmax > 0.0)
-    atomicMax(a.vmaxBits, (unsigned long long)__double_as_longlong(bmax));
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const double bmax = block_max(vmax, sh, tid);
+  if (tid == 0 && bmax > 0.0) atomicMax(a.vmaxBits, (unsigned long long)__double_as_longlong(bmax));
   if (a.mode == RK_FSAL) {
-    const double es = block_sum(errLoc, sh);
-    const double pm = block_min(posLoc, sh);
-    if (threadIdx.x == 0) {
-      a.errPartial[blockIdx.x] = es;
-      a.posPartial[blockIdx.x] = pm;
+    const double es = block_sum(errLoc, sh, tid);
+    const double pm = block_min(posLoc, sh, tid);
+    if (tid == 0) {
+      a.errPartial[bid] = es;
+      a.posPartial[bid] = pm;
     }
   }
 }
+
+}  // namespace dev
+}  // namespace kfvm
+
+#include "recon5.hpp"
+
+namespace kfvm {
+namespace dev {
 
 #ifdef KFVM_SMALL_KERNELS
 // ------------------------------------------------------ small register kernels

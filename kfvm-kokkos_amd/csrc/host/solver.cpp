@@ -95,13 +95,15 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   if (rk_.needsTmp()) Utmp_ = dalloc<double>(nc * nv_, "Utmp");
   FS_ = dalloc<double>(nc * FQ * nv_, "face states");
   WI_ = dalloc<double>(nc * FQ, "indicator");
-  theta_ = dalloc<double>(nc, "theta");
+  props_ = dalloc<double>(nc * 2, "cell properties");
+  theta_[0] = dalloc<double>(nc, "theta");
+  theta_[1] = dalloc<double>(nc, "theta");
   wflag_ = dalloc<double>(nc, "wflag");
   list_ = dalloc<int>(nc, "weno list");
   counter_ = dalloc<int>(1, "weno count");
   faceW_ = dalloc<double>(wts_.face.size(), "face weights");
   derivW_ = dalloc<double>(wts_.deriv.size(), "deriv weights");
-  nPart_ = flux_rk_partials(ncell_);
+  nPart_ = flux_rk_partials(cfg_.dim, grid_);
   errPart_ = dalloc<double>(nPart_, "err partials");
   posPart_ = dalloc<double>(nPart_, "pos partials");
   redOut_ = dalloc<double>(2, "reduction");
@@ -110,7 +112,8 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
 
   check(hipMemcpy(faceW_, wts_.face.data(), wts_.face.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(derivW_, wts_.deriv.data(), wts_.deriv.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
-  check(launch_fill(theta_, 1.0, nc, stream_), "fill theta");
+  check(launch_fill(theta_[0], 1.0, nc, stream_), "fill theta");
+  check(launch_fill(theta_[1], 1.0, nc, stream_), "fill theta");
   check(launch_fill(wflag_, 0.0, nc, stream_), "fill wflag");
   check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
 
@@ -126,7 +129,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
 
 Solver::~Solver() {
   profileEnable(false);
-  void *bufs[] = {U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, theta_, wflag_, list_, counter_,
+  void *bufs[] = {U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_,
                   faceW_, derivW_, errPart_, posPart_, redOut_, vmax_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
@@ -189,25 +192,28 @@ bool Solver::profileGet(const std::string &name, double &ms, int64_t &count) {
 void Solver::pipeline(const double *Uin, const RKArgs &a) {
   const int dim = cfg_.dim, nq = cfg_.nQuad;
   const bool hydro = nv_ == 5;
+  // theta of the previous RHS evaluation feeds the selector; this one writes the other buffer
+  const double *thPrev = theta_[th_];
+  double *thNew = theta_[th_ ^ 1];
   if (useSparseWeno) {
-    timed("linear", [&] { return launch_linear(dim, nq, nv_, Uin, FS_, WI_, grid_, faceW_, stream_); });
+    timed("linear", [&] { return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, stream_); });
     if (hydro) {
       const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
       check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
       timed("select", [&] {
-        return launch_select(dim, nq, Uin, WI_, theta_, wflag_, list_, counter_, wThresh, grid_, stream_);
+        return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
       });
       timed("weno", [&] {
-        return launch_weno(dim, nq, nv_, Uin, FS_, grid_, faceW_, derivW_, list_, counter_, stream_);
+        return launch_weno(dim, nq, nv_, Uin, FS_, thNew, props_, grid_, faceW_, derivW_, list_, counter_, stream_);
       });
     }
   } else {
     timed("weno", [&] {
-      return launch_weno(dim, nq, nv_, Uin, FS_, grid_, faceW_, derivW_, nullptr, nullptr, stream_);
+      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, props_, grid_, faceW_, derivW_, nullptr, nullptr, stream_);
     });
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
-  if (hydro) timed("pp", [&] { return launch_pp(dim, nq, Uin, FS_, theta_, grid_, stream_); });
+  if (hydro) th_ ^= 1;
   timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, a, stream_); });
   nRhsEval++;
 }
@@ -413,12 +419,12 @@ void Solver::getRHS(double *host) const {
 void Solver::getFlags(double *host) const {
   check(hipStreamSynchronize(stream_), "sync");
   check(hipMemcpy(host, wflag_, size_t(ncell_) * sizeof(double), hipMemcpyDeviceToHost), "get flags");
-  check(hipMemcpy(host + ncell_, theta_, size_t(ncell_) * sizeof(double), hipMemcpyDeviceToHost), "get flags");
+  check(hipMemcpy(host + ncell_, theta_[th_], size_t(ncell_) * sizeof(double), hipMemcpyDeviceToHost), "get flags");
 }
 void Solver::setFlags(const double *host) {
   check(hipStreamSynchronize(stream_), "sync");
   check(hipMemcpy(wflag_, host, size_t(ncell_) * sizeof(double), hipMemcpyHostToDevice), "set flags");
-  check(hipMemcpy(theta_, host + ncell_, size_t(ncell_) * sizeof(double), hipMemcpyHostToDevice), "set flags");
+  check(hipMemcpy(theta_[th_], host + ncell_, size_t(ncell_) * sizeof(double), hipMemcpyHostToDevice), "set flags");
 }
 void Solver::setWeights(const double *face, const double *deriv) {
   std::copy(face, face + wts_.face.size(), wts_.face.begin());

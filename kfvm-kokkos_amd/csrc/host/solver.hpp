@@ -81,7 +81,10 @@ class Solver {
   double *U_[2] = {nullptr, nullptr};  // ping-pong stage states
   int cur_ = 0;
   double *Uprev_ = nullptr, *Uhat_ = nullptr, *Utmp_ = nullptr, *K_ = nullptr;
-  double *FS_ = nullptr, *WI_ = nullptr, *theta_ = nullptr, *wflag_ = nullptr;
+  double *FS_ = nullptr, *WI_ = nullptr, *wflag_ = nullptr;
+  double *props_ = nullptr;  // per-cell internal energy and sound speed [2][ncell]
+  double *theta_[2] = {nullptr, nullptr};  // positivity theta: previous RHS / this RHS
+  int th_ = 0;                             // theta_[th_] holds the latest values
   int *list_ = nullptr, *counter_ = nullptr;
   double *faceW_ = nullptr, *derivW_ = nullptr;
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;
