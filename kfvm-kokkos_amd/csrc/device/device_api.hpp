@@ -41,22 +41,28 @@ struct RKArgs {
   unsigned long long *vmaxBits;  // running max wave speed (bit pattern of a non-negative double)
 };
 
+// Euler: per-cell internal energy and sound speed, props [2][ncell], read by
+// the limiter bounds of launch_linear and launch_weno_post.
+hipError_t launch_cell_props(int dim, const double *U, double *props, const Grid &g, hipStream_t s);
 // Linear reconstruction, KXRCF indicator and positivity limiter (all cells).
-// props [2][ncell] receives each cell's internal energy and sound speed
-// (Euler), which the limiter bounds of this pass and of a following
-// launch_weno over a list read.
+// W0: full-stencil face weights [F*Q][C]; W0T: the same regrouped per pass
+// of linear_pass_points(dim, nq) points as [F*Q/P][C][P].
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
-                         double *thetaNew, double *props, const Grid &g, const double *W0,
-                         hipStream_t s);
+                         double *thetaNew, const double *props, const Grid &g, const double *W0,
+                         const double *W0T, hipStream_t s);
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                          double *wflag, int *list, int *counter, double wThresh, const Grid &g,
                          hipStream_t s);
-// KFVM-WENO and positivity limiter on the listed cells (list == nullptr: all
-// cells, and props is recomputed from U first; with a list, props must hold
-// the values launch_linear wrote for the same U).
+// KFVM-WENO reconstruction of the listed cells (list == nullptr: all).  Euler
+// leaves the face states in linearised primitive variables for
+// launch_weno_post; linear advection completes here.
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
-                       double *props, const Grid &g, const double *FW, const double *DW,
-                       const int *list, const int *counter, hipStream_t s);
+                       const Grid &g, const double *FW, const double *DW, const int *list,
+                       const int *counter, hipStream_t s);
+// Euler: back to conserved variables and positivity limiter on the listed cells.
+hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,
+                            const double *props, const Grid &g, const int *list, const int *counter,
+                            hipStream_t s);
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
                           const Grid &g, const RKArgs &a, hipStream_t s);
 int flux_rk_partials(int dim, const Grid &g);

@@ -14,11 +14,15 @@ namespace kfvm {
     return hipErrorInvalidValue;                            \
   } while (0)
 
+hipError_t launch_cell_props(int dim, const double *U, double *props, const Grid &g, hipStream_t s) {
+  KFVM_SELECT(dim, 2, L::cell_props(U, props, g, s));
+}
+
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
-                         double *thetaNew, double *props, const Grid &g, const double *W0,
-                         hipStream_t s) {
+                         double *thetaNew, const double *props, const Grid &g, const double *W0,
+                         const double *W0T, hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, s));
+  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, W0T, s));
 }
 
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
@@ -28,10 +32,16 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
 }
 
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
-                       double *props, const Grid &g, const double *FW, const double *DW,
-                       const int *list, const int *counter, hipStream_t s) {
+                       const Grid &g, const double *FW, const double *DW, const int *list,
+                       const int *counter, hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, props, g, FW, DW, list, counter, s));
+  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, g, FW, DW, list, counter, s));
+}
+
+hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,
+                            const double *props, const Grid &g, const int *list, const int *counter,
+                            hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::weno_post(U, FS, thetaNew, props, g, list, counter, s));
 }
 
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,

@@ -16,14 +16,17 @@ struct Launch {
     return unsigned(((g.nx + Bx::BX - 1) / Bx::BX) * ((g.ny + Bx::BY - 1) / Bx::BY) *
                     ((g.nz + Bx::BZ - 1) / Bx::BZ));
   }
+  static hipError_t cell_props(const double *U, double *props, const Grid &g, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_cell_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
+    return hipGetLastError();
+  }
   static hipError_t linear(int nv, const double *U, double *FS, double *WI, double *thetaNew,
-                           double *props, const Grid &g, const double *W0, hipStream_t s) {
-    if (nv == 5) {
-      hipLaunchKernelGGL((dev::k_linrec_brick<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, FS,
-                         props, g, W0);
-      hipLaunchKernelGGL((dev::k_linear_post<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U,
-                         FS, WI, thetaNew, props, g);
-    } else
+                           const double *props, const Grid &g, const double *W0, const double *W0T,
+                           hipStream_t s) {
+    if (nv == 5)
+      hipLaunchKernelGGL((dev::k_linear_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, FS, WI,
+                         thetaNew, props, g, W0T);
+    else
       hipLaunchKernelGGL((dev::k_linear<D, NQ, 1>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, WI,
                          thetaNew, g, W0);
     return hipGetLastError();
@@ -34,28 +37,23 @@ struct Launch {
                        list, counter, wThresh, g);
     return hipGetLastError();
   }
-  static hipError_t weno(int nv, const double *U, double *FS, double *thetaNew, double *props,
-                         const Grid &g, const double *FW, const double *DW, const int *list,
-                         const int *counter, hipStream_t s) {
-    if (nv == 5) {
-      // listed cells: props come from the linear pass over the same state
-      if (!list)
-        hipLaunchKernelGGL((dev::k_cell_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props,
-                           g);
-      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 1u << 16);
+  static hipError_t weno(int nv, const double *U, double *FS, double *thetaNew, const Grid &g,
+                         const double *FW, const double *DW, const int *list, const int *counter,
+                         hipStream_t s) {
+    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 1u << 16);
+    if (nv == 5)
       hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, FW, DW, list,
                          counter);
-      const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
-      hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
-                         counter);
-      return hipGetLastError();
-    }
-    const unsigned nb = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
-    if (false)
-      ;
     else
-      hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW,
-                         list, counter);
+      hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
+                         dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
+    return hipGetLastError();
+  }
+  static hipError_t weno_post(const double *U, double *FS, double *thetaNew, const double *props,
+                              const Grid &g, const int *list, const int *counter, hipStream_t s) {
+    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
+    hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
+                       counter);
     return hipGetLastError();
   }
   static dim3 flux_grid(const Grid &g) {

@@ -1,18 +1,16 @@
 // Reconstruction kernels of the 5-variable (Euler) equations.
 //
-//   k_linrec_brick  Stencil_K.H:32-90: linear GP face states in conserved
-//                   variables.  A block owns a brick of cells; each conserved
-//                   variable of the brick plus its radius-2 halo is staged in
-//                   LDS with coalesced loads (boundary conditions applied by
-//                   the index map of the load), then every thread reads its 33
-//                   stencil values at compile-time LDS offsets and forms the
-//                   F*Q dot products with the uniform weights.
+//   k_linear_fused  Stencil_K.H:32-90 linear GP face states in conserved
+//                   variables, the Numeric_K.H:315-379 indicator p/rho^gamma
+//                   of every unlimited face state, and the Common_K.H:299-643
+//                   limiter.  A block owns a brick of cells; the five
+//                   conserved variables of the brick plus its radius-2 halo
+//                   are staged in LDS with coalesced loads (boundary
+//                   conditions applied by the index map of the load).
 //   k_cell_props    per-cell internal energy and sound speed, the two
 //                   neighbourhood quantities of the limiter bounds.
 //   k_wenorec5      Stencil_K.H:164-283, 416-454, 506-589: one lane per
 //                   (cell, variable) over the compacted WENO list.
-//   k_linear_post   Numeric_K.H:315-379 indicator p/rho^gamma of every
-//                   unlimited face state + Common_K.H:299-643 limiter.
 //   k_weno_post     Hydro_K.H:53-64 back-conversion + Common_K.H:299-643.
 #pragma once
 
@@ -33,11 +31,15 @@ struct Group {
   }
 };
 
+// Read-only data in the constant address space: uniform loads become scalar
+// (s_load) loads.
+typedef __attribute__((address_space(4))) double cdouble;
+
 // ----------------------------------------------------------- brick geometry
 template <int D>
 struct RBox {
   static constexpr int H = 2;  // stencil radius
-  static constexpr int BX = D == 3 ? 8 : 16, BY = D == 3 ? 8 : 16, BZ = D == 3 ? 8 : 1;
+  static constexpr int BX = D == 3 ? 8 : 16, BY = D == 3 ? 8 : 16, BZ = D == 3 ? 4 : 1;
   static constexpr int LX = BX + 2 * H, LY = BY + 2 * H, LZ = D == 3 ? BZ + 2 * H : 1;
   static constexpr int N = LX * LY * LZ;   // staged cells per variable
   static constexpr int NT = BX * BY * BZ;  // threads per block
@@ -57,61 +59,169 @@ __device__ inline int box_out(int t) {
   return t < H ? H - t : (t >= B + H ? t - (B + H) + 1 : 0);
 }
 
+// Limiter bounds (Common_K.H:329-387) inside a staged brick: densities and
+// velocities from LDS, the neighbours' internal energy and sound speed from
+// the per-cell property planes (both are even in the momenta, so the mirror
+// images of wall boundaries need no sign handling).
+template <int D>
+__device__ inline PPBounds pp_bounds_lds(const double *__restrict__ lds, int c0,
+                                         const double *__restrict__ props, const Grid &g, int i,
+                                         int j, int k, const double *ref) {
+  using Bx = RBox<D>;
+  constexpr int SX = 1, SY = Bx::LX, SZ = Bx::LX * Bx::LY, NP = Bx::N;
+  const size_t nc = g.ncell;
+  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
+#pragma unroll 1
+  for (int xo = -1; xo <= 1; xo++) {
+    int sx;
+    const int ii = map_axis(i + xo, g.nx, g.bc[0], g.bc[1], sx);
+#pragma unroll
+    for (int yo = -1; yo <= 1; yo++) {
+      int sy;
+      const int jj = map_axis(j + yo, g.ny, g.bc[2], g.bc[3], sy);
+#pragma unroll
+      for (int zo = zl; zo <= zh; zo++) {
+        int sz;
+        const int kk = D == 3 ? map_axis(k + zo, g.nz, g.bc[4], g.bc[5], sz) : 0;
+        const size_t c = (size_t(kk) * g.ny + jj) * g.nx + ii;
+        const double rho = lds[c0 + xo * SX + yo * SY + zo * SZ];
+        maxDens = fmax(maxDens, rho);
+        minDens = fmin(minDens, rho);
+        minEint = fmin(minEint, props[c]);
+        minSdSp = fmin(minSdSp, props[nc + c]);
+      }
+    }
+  }
+  const double *r0 = lds + c0, *m1 = lds + NP + c0, *m2 = lds + 2 * NP + c0, *m3 = lds + 3 * NP + c0;
+  const double dxU = m1[SX] / r0[SX] - m1[-SX] / r0[-SX];
+  const double dyV = m2[SY] / r0[SY] - m2[-SY] / r0[-SY];
+  double dzW;
+  if (D == 3)
+    dzW = m3[SZ] / r0[SZ] - m3[-SZ] / r0[-SZ];
+  else
+    dzW = ref[3] / ref[0] - ref[3] / ref[0];
+  const double divV = (dxU + dyV + dzW) / 2.0;
+  double kappa = 0.2;
+  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+  kappa *= (1.0 - eta);
+  PPBounds b;
+  b.rDens = ref[0];
+  b.maxDens = maxDens * (1.0 + kappa);
+  b.minDens = minDens * (1.0 - kappa);
+  b.minEint = minEint * (1.0 - kappa);
+  return b;
+}
+
+// Linear reconstruction + KXRCF indicator + limiter, one thread per cell of a
+// staged brick.  All five conserved variables of the brick and its halo sit
+// in LDS.  The face points are produced in passes of P points (both faces of
+// an axis, or one face for 3D 3x3 points): for each variable the thread reads
+// its stencil values at compile-time LDS offsets, forms the dot products with
+// the uniform weights and stores the states; density and |m|^2 of the pass'
+// points stay in registers until the energy completes them, which gives
+// p/rho^gamma and the limiter's fast test without reading the states back.
 template <int D, int NQ>
-__global__ __launch_bounds__(RBox<D>::NT) void k_linrec_brick(const double *__restrict__ U,
-                                                              double *__restrict__ FS,
-                                                              double *__restrict__ props, Grid g,
-                                                              const double *__restrict__ W) {
+__global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
+    const double *__restrict__ U, double *__restrict__ FS, double *__restrict__ WI,
+    double *__restrict__ thetaNew, const double *__restrict__ props, Grid g,
+    const double *__restrict__ WT) {
   using Bx = RBox<D>;
   constexpr auto ST = make_stencil_tables<D>();
-  constexpr int C = SInfo<D>::C, FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q, H = Bx::H;
-  __shared__ double lds[Bx::N];
+  constexpr int C = SInfo<D>::C, Q = Shape<D, NQ>::Q, H = Bx::H, NP = Bx::N;
+  constexpr int P = linear_pass_points(D, NQ), NPASS = 2 * D * Q / P;
+  __shared__ double lds[5 * NP];
   const int nbx = (g.nx + Bx::BX - 1) / Bx::BX, nby = (g.ny + Bx::BY - 1) / Bx::BY;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int i0 = (b % nbx) * Bx::BX, j0 = ((b / nbx) % nby) * Bx::BY, k0 = (b / (nbx * nby)) * Bx::BZ;
   const int tid = threadIdx.x;
+  const size_t nc = g.ncell;
+  for (int idx = tid; idx < NP; idx += Bx::NT) {
+    const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, c = r / Bx::LY;
+    const int ex = box_out<Bx::BX, H>(a), ey = box_out<Bx::BY, H>(bb);
+    const int ez = D == 3 ? box_out<Bx::BZ, H>(c) : 0;
+    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + c - H : 0;
+    // cells no stencil of a live cell reads (box corners, beyond the halo)
+    const bool need = ex * ex + ey * ey + ez * ez <= H * H && gi < g.nx + H && gj < g.ny + H &&
+                      (D == 2 || gk < g.nz + H);
+    if (need) {
+      const CellRef cr = map_cell<D, 5>(g, gi, gj, gk);
+#pragma unroll
+      for (int w = 0; w < 5; w++) lds[w * NP + idx] = ld(U, nc, cr, w);
+    }
+  }
+  __syncthreads();
   const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
-  const bool live = i < g.nx && j < g.ny && (D == 2 || k < g.nz);
-  const size_t nc = g.ncell;
-  const size_t cell = (size_t(k) * g.ny + j) * g.nx + i;
-  const double *p = lds + (tz * Bx::LY + ty) * Bx::LX + tx;  // stencil box corner (-H,-H,-H)
-  constexpr int ctr = H + Bx::LX * H + (D == 3 ? Bx::LX * Bx::LY * H : 0);
+  if (!(i < g.nx && j < g.ny && (D == 2 || k < g.nz))) return;  // no barrier follows
+  const int cell = (k * g.ny + j) * g.nx + i;
+  const int corner = (tz * Bx::LY + ty) * Bx::LX + tx;  // stencil box corner (-H,-H,-H)
+  const int c0 = corner + H + Bx::LX * H + (D == 3 ? Bx::LX * Bx::LY * H : 0);
   double ref[5];
+#pragma unroll
+  for (int w = 0; w < 5; w++) ref[w] = lds[w * NP + c0];
+  const PPBounds pb = pp_bounds_lds<D>(lds, c0, props, g, i, j, k, ref);
+  const double gm1 = g.gamma - 1.0;
+  double thD = 1.0;
+  bool okE = true;
+  // one variable of one pass: P dot products of length C, accumulated with
+  // the stencil walk outermost (each accumulator still sums n = 0..C-1 in
+  // order), so only the P accumulators and one stencil value are live and
+  // the weights of one stencil cell arrive as one scalar load
+  auto axis_var = [&](int ax, int v, double (&acc)[P]) {
+    const double *p = lds + v * NP + corner;
+    const double *wt0 = WT + ax * C * P;
+    // opaque to loop-invariant code motion: hoisting the weight loads out of
+    // the variable loop would park 2*C*P scalars in VGPR lanes; the constant
+    // address space keeps them scalar loads
+    asm volatile("" : "+s"(wt0));
+    const cdouble *wt = (const cdouble *)wt0;
+#pragma unroll
+    for (int q = 0; q < P; q++) acc[q] = 0.0;
+#pragma unroll
+    for (int n = 0; n < C; n++) {
+      const double x = p[(ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
+                         (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0)];
+#pragma unroll
+      for (int q = 0; q < P; q++) acc[q] = fma(wt[n * P + q], x, acc[q]);
+      // interleave: one LDS read, then this stencil cell's P FMAs
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, P, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < P; q++) FS[(size_t(ax * P + q) * 5 + v) * nc + cell] = acc[q];
+    // keep the scheduler from hoisting the next variable's stencil loads
+    __builtin_amdgcn_sched_barrier(0);
+  };
 #pragma unroll 1
-  for (int v = 0; v < 5; v++) {
-    for (int idx = tid; idx < Bx::N; idx += Bx::NT) {
-      const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, c = r / Bx::LY;
-      const int ex = box_out<Bx::BX, H>(a), ey = box_out<Bx::BY, H>(bb);
-      const int ez = D == 3 ? box_out<Bx::BZ, H>(c) : 0;
-      const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + c - H : 0;
-      // cells no stencil of a live cell reads (box corners, beyond the halo)
-      const bool need = ex * ex + ey * ey + ez * ez <= H * H && gi < g.nx + H && gj < g.ny + H &&
-                        (D == 2 || gk < g.nz + H);
-      if (need) lds[idx] = load_u<D, 5>(U, g, gi, gj, gk, v);
+  for (int ax = 0; ax < NPASS; ax++) {
+    double rho[P], msq[P], t[P];
+    axis_var(ax, 0, rho);
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      thD = dens_theta(thD, rho[q], pb);
+      msq[q] = 0.0;  // 0 + m^2 == m^2 exactly: same sums as Hydro_EOS.H
     }
-    __syncthreads();
-    if (live) {
-      double vs[C];
+#pragma unroll 1
+    for (int v = 1; v < 4; v++) {
+      axis_var(ax, v, t);
 #pragma unroll
-      for (int n = 0; n < C; n++)
-        vs[n] = p[(ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
-                  (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0)];
-      ref[v] = p[ctr];
-#pragma unroll
-      for (int fq = 0; fq < FQ; fq++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int n = 0; n < C; n++) acc = fma(W[fq * C + n], vs[n], acc);
-        FS[(size_t(fq) * 5 + v) * nc + cell] = acc;
-      }
+      for (int q = 0; q < P; q++) msq[q] = msq[q] + t[q] * t[q];
     }
-    __syncthreads();
+    axis_var(ax, 4, t);
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+      // Hydro::eint / pres / weno of the point (Hydro_EOS.H)
+      const double ei = t[q] - 0.5 * msq[q] / rho[q];
+      okE = okE && ei > pb.minEint;
+      WI[size_t(ax * P + q) * nc + cell] = (gm1 * ei) / pow(rho[q], g.gamma);
+      __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
+    }
   }
-  if (live) {
-    props[cell] = Hydro::eint(ref);
-    props[nc + cell] = Hydro::sound(g.gamma, ref);
-  }
+  double th = 1.0;
+  if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, cell, ref, pb);
+  thetaNew[cell] = th;
 }
 
 template <int D>
@@ -218,19 +328,14 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
     double vs[C];
 #pragma unroll
     for (int m = 0; m < C; m++) {
-      const int a = i + ST.off[m][0], b = j + ST.off[m][1], d = k + ST.off[m][2];
-      const double s0 = load_u<D, 5>(U, g, a, b, d, 0);
-      if (G.v == 0) {
-        vs[m] = s0;
-      } else if (G.v < 4) {
-        const double sv = load_u<D, 5>(U, g, a, b, d, G.v);
-        vs[m] = (sv - uv * s0) / lp.rho;
+      const CellRef r = map_cell<D, 5>(g, i + ST.off[m][0], j + ST.off[m][1], k + ST.off[m][2]);
+      const double s0 = ld(U, nc, r, 0);
+      const double sv = ld(U, nc, r, G.v);
+      if (G.v == 4) {
+        const double s1 = ld(U, nc, r, 1), s2 = ld(U, nc, r, 2), s3 = ld(U, nc, r, 3);
+        vs[m] = lp.gamm * (lp.vsq * s0 - lp.u * s1 - lp.v * s2 - lp.w * s3 + sv);
       } else {
-        const double s1 = load_u<D, 5>(U, g, a, b, d, 1);
-        const double s2 = load_u<D, 5>(U, g, a, b, d, 2);
-        const double s3 = load_u<D, 5>(U, g, a, b, d, 3);
-        const double s4 = load_u<D, 5>(U, g, a, b, d, 4);
-        vs[m] = lp.gamm * (lp.vsq * s0 - lp.u * s1 - lp.v * s2 - lp.w * s3 + s4);
+        vs[m] = G.v == 0 ? s0 : (sv - uv * s0) / lp.rho;
       }
     }
     double om[nSub];
@@ -238,7 +343,7 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
     for (int s = 0; s < nSub; s++) {
       const int ns = s == 0 ? C : (s == 1 ? Cc : Cb);
       double beta = 0.0;
-#pragma unroll
+#pragma unroll 1
       for (int ii = 0; ii < I; ii++) {
         const double *dd = DW + (s * I + ii) * C;
         double tt = 0.0;
@@ -260,7 +365,8 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
     om[0] /= g0;
 #pragma unroll
     for (int s = 1; s < nSub; s++) om[s] -= om[0] * (s == 1 ? g1 : gb);
-#pragma unroll
+    // rolled over face points: bounds the live weights to one point's 106
+#pragma unroll 1
     for (int fq = 0; fq < FQ; fq++) {
       double acc = 0.0;
 #pragma unroll
@@ -277,43 +383,7 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
   }
 }
 
-// ------------------------------------------- per-cell post-processing passes
-// After linear reconstruction: KXRCF indicator p/rho^gamma of every unlimited
-// face state, then the positivity limiter (fast test, exact reference
-// sequence when a bound is violated).
-template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_linear_post(const double *__restrict__ U,
-                                                     double *__restrict__ FS,
-                                                     double *__restrict__ WI,
-                                                     double *__restrict__ thetaNew,
-                                                     const double *__restrict__ props, Grid g) {
-  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
-  int i, j, k;
-  cell_ijk<D>(c, g, i, j, k);
-  const size_t nc = g.ncell;
-  double ref[5];
-#pragma unroll
-  for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
-  const PPBounds pb = pp_bounds_p<D>(U, props, g, i, j, k, ref);
-  double thD = 1.0;
-  bool okE = true;
-#pragma unroll 2
-  for (int fq = 0; fq < FQ; fq++) {
-    double s[5];
-#pragma unroll
-    for (int w = 0; w < 5; w++) s[w] = FS[(size_t(fq) * 5 + w) * nc + c];
-    thD = dens_theta(thD, s[0], pb);
-    const double ei = Hydro::eint(s);
-    okE = okE && ei > pb.minEint;
-    WI[size_t(fq) * nc + c] = Hydro::weno(g.gamma, s);
-  }
-  double th = 1.0;
-  if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, c, ref, pb);
-  thetaNew[c] = th;
-}
-
+// ------------------------------------------------- post-WENO pass
 // After WENO: back to conserved variables about the cell's own average
 // (Hydro_K.H:53-64), then the positivity limiter.
 template <int D, int NQ>

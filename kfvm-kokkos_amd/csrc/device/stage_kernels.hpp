@@ -51,22 +51,46 @@ __device__ inline int map_axis(int i, int n, int bcLo, int bcHi, int &side) {
   return usePer ? per : mir;
 }
 
-// Cell average of variable v at (possibly ghost) cell (i,j,k).
+// Source of a (possibly ghost) cell: linear index of the interior cell it
+// copies and the mask of variables the wall mirrors negate.  The per-face
+// masks are picked with selects: a lane-dependent index into the Grid
+// argument would turn every gather into extra memory loads.
+struct CellRef {
+  size_t idx;
+  unsigned neg;
+};
+
+__device__ inline unsigned side_mask(int side, unsigned lo, unsigned hi) {
+  return side == 0 ? lo : (side == 1 ? hi : 0u);
+}
+
 template <int D, int NV>
-__device__ inline double load_u(const double *__restrict__ U, const Grid &g, int i, int j, int k,
-                                int v) {
+__device__ inline CellRef map_cell(const Grid &g, int i, int j, int k) {
   int sx, sy, sz = -1;
   const int ii = map_axis(i, g.nx, g.bc[0], g.bc[1], sx);
   const int jj = map_axis(j, g.ny, g.bc[2], g.bc[3], sy);
   int kk = 0;
   if (D == 3) kk = map_axis(k, g.nz, g.bc[4], g.bc[5], sz);
-  double x = U[size_t(v) * g.ncell + (size_t(kk) * g.ny + jj) * g.nx + ii];
+  CellRef r;
+  r.idx = (size_t(kk) * g.ny + jj) * g.nx + ii;
+  r.neg = 0u;
   if (NV == 5) {
-    unsigned m = (sx >= 0 ? g.negmask[sx] : 0u) ^ (sy >= 0 ? g.negmask[2 + sy] : 0u);
-    if (D == 3) m ^= sz >= 0 ? g.negmask[4 + sz] : 0u;
-    x = ((m >> v) & 1u) ? -x : x;
+    r.neg = side_mask(sx, g.negmask[0], g.negmask[1]) ^ side_mask(sy, g.negmask[2], g.negmask[3]);
+    if (D == 3) r.neg ^= side_mask(sz, g.negmask[4], g.negmask[5]);
   }
-  return x;
+  return r;
+}
+
+__device__ inline double ld(const double *__restrict__ U, size_t nc, const CellRef &r, int v) {
+  const double x = U[size_t(v) * nc + r.idx];
+  return ((r.neg >> v) & 1u) ? -x : x;
+}
+
+// Cell average of variable v at (possibly ghost) cell (i,j,k).
+template <int D, int NV>
+__device__ inline double load_u(const double *__restrict__ U, const Grid &g, int i, int j, int k,
+                                int v) {
+  return ld(U, g.ncell, map_cell<D, NV>(g, i, j, k), v);
 }
 
 template <int D>
@@ -555,9 +579,13 @@ __device__ inline void face_state(const double *__restrict__ FS, const Grid &g, 
   const int n_ax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
   int pos = ax == 0 ? ci : (ax == 1 ? cj : ck);
   if (pos < 0 || pos >= n_ax) {
-    const int bface = pos < 0 ? 2 * ax : 2 * ax + 1;
-    if (g.bc[bface] != 0) {
-      const unsigned m = NV == 5 ? g.negmask[bface] : 0u;
+    const bool low = pos < 0;
+    const int bcLo = ax == 0 ? g.bc[0] : (ax == 1 ? g.bc[2] : g.bc[4]);
+    const int bcHi = ax == 0 ? g.bc[1] : (ax == 1 ? g.bc[3] : g.bc[5]);
+    if ((low ? bcLo : bcHi) != 0) {
+      const unsigned mLo = ax == 0 ? g.negmask[0] : (ax == 1 ? g.negmask[2] : g.negmask[4]);
+      const unsigned mHi = ax == 0 ? g.negmask[1] : (ax == 1 ? g.negmask[3] : g.negmask[5]);
+      const unsigned m = NV == 5 ? (low ? mLo : mHi) : 0u;
 #pragma unroll
       for (int v = 0; v < NV; v++) out[v] = ((m >> v) & 1u) ? -inner[v] : inner[v];
       return;

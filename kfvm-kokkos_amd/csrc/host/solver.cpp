@@ -16,6 +16,7 @@
 #include <string>
 
 #include "problems.hpp"
+#include "stencil_geom.hpp"
 
 namespace kfvm {
 
@@ -103,6 +104,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   counter_ = dalloc<int>(1, "weno count");
   faceW_ = dalloc<double>(wts_.face.size(), "face weights");
   derivW_ = dalloc<double>(wts_.deriv.size(), "deriv weights");
+  faceWT_ = dalloc<double>(size_t(wts_.F) * wts_.Q * wts_.C, "face weights (axis-major)");
   nPart_ = flux_rk_partials(cfg_.dim, grid_);
   errPart_ = dalloc<double>(nPart_, "err partials");
   posPart_ = dalloc<double>(nPart_, "pos partials");
@@ -110,8 +112,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   vmax_ = dalloc<unsigned long long>(1, "vmax");
   check(hipHostMalloc((void **)&hostRed_, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
 
-  check(hipMemcpy(faceW_, wts_.face.data(), wts_.face.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
-  check(hipMemcpy(derivW_, wts_.deriv.data(), wts_.deriv.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  uploadWeights();
   check(launch_fill(theta_[0], 1.0, nc, stream_), "fill theta");
   check(launch_fill(theta_[1], 1.0, nc, stream_), "fill theta");
   check(launch_fill(wflag_, 0.0, nc, stream_), "fill wflag");
@@ -130,7 +131,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
 Solver::~Solver() {
   profileEnable(false);
   void *bufs[] = {U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_,
-                  faceW_, derivW_, errPart_, posPart_, redOut_, vmax_};
+                  faceW_, derivW_, faceWT_, errPart_, posPart_, redOut_, vmax_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -195,8 +196,11 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
   // theta of the previous RHS evaluation feeds the selector; this one writes the other buffer
   const double *thPrev = theta_[th_];
   double *thNew = theta_[th_ ^ 1];
+  if (hydro) timed("cell_props", [&] { return launch_cell_props(dim, Uin, props_, grid_, stream_); });
   if (useSparseWeno) {
-    timed("linear", [&] { return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, stream_); });
+    timed("linear", [&] {
+      return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, faceWT_, stream_);
+    });
     if (hydro) {
       const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
       check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
@@ -204,13 +208,20 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
         return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
       });
       timed("weno", [&] {
-        return launch_weno(dim, nq, nv_, Uin, FS_, thNew, props_, grid_, faceW_, derivW_, list_, counter_, stream_);
+        return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, list_, counter_, stream_);
+      });
+      timed("weno_post", [&] {
+        return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, list_, counter_, stream_);
       });
     }
   } else {
     timed("weno", [&] {
-      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, props_, grid_, faceW_, derivW_, nullptr, nullptr, stream_);
+      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, nullptr, nullptr, stream_);
     });
+    if (hydro)
+      timed("weno_post", [&] {
+        return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, nullptr, nullptr, stream_);
+      });
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
   if (hydro) th_ ^= 1;
@@ -430,8 +441,22 @@ void Solver::setWeights(const double *face, const double *deriv) {
   std::copy(face, face + wts_.face.size(), wts_.face.begin());
   std::copy(deriv, deriv + wts_.deriv.size(), wts_.deriv.begin());
   check(hipStreamSynchronize(stream_), "sync");
-  check(hipMemcpy(faceW_, wts_.face.data(), wts_.face.size() * sizeof(double), hipMemcpyHostToDevice), "set weights");
-  check(hipMemcpy(derivW_, wts_.deriv.data(), wts_.deriv.size() * sizeof(double), hipMemcpyHostToDevice), "set weights");
+  uploadWeights();
+}
+// Device copies of the weight tables: the reference layouts
+// faceWeights[nSub][F][Q][C] and derivWeights[nSub][I][C], plus the
+// full-stencil face weights regrouped per pass of P face points as
+// [F*Q/P][C][P] (point index fastest) for the linear reconstruction, which
+// walks the stencil in its outer loop.
+void Solver::uploadWeights() {
+  const int C = wts_.C, FQ = wts_.F * wts_.Q, P = linear_pass_points(wts_.dim, wts_.nq);
+  std::vector<double> wt(size_t(FQ) * C);
+  for (int ps = 0; ps < FQ / P; ps++)
+    for (int n = 0; n < C; n++)
+      for (int q = 0; q < P; q++) wt[(size_t(ps) * C + n) * P + q] = wts_.face[size_t(ps * P + q) * C + n];
+  check(hipMemcpy(faceW_, wts_.face.data(), wts_.face.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  check(hipMemcpy(derivW_, wts_.deriv.data(), wts_.deriv.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  check(hipMemcpy(faceWT_, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
 }
 void Solver::sync() const { check(hipStreamSynchronize(stream_), "sync"); }
 

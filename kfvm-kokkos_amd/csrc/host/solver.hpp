@@ -67,6 +67,7 @@ class Solver {
   void harvest();
 
   void pipeline(const double *Uin, const RKArgs &a);
+  void uploadWeights();
   double readSpeed();
 
   Config cfg_;
@@ -86,7 +87,7 @@ class Solver {
   double *theta_[2] = {nullptr, nullptr};  // positivity theta: previous RHS / this RHS
   int th_ = 0;                             // theta_[th_] holds the latest values
   int *list_ = nullptr, *counter_ = nullptr;
-  double *faceW_ = nullptr, *derivW_ = nullptr;
+  double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr;
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;
   unsigned long long *vmax_ = nullptr;
   double *hostRed_ = nullptr;  // pinned: [err, pos, vmax bits, counter]

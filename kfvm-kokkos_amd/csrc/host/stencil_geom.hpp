@@ -157,6 +157,13 @@ constexpr StencilTables<D> make_stencil_tables() {
 }
 
 // Number of monomials of total degree <= t in D dimensions.
+// Face points per pass of the linear reconstruction kernel: both faces of
+// one axis, or one face when that would hold too many accumulators (3D with
+// 3x3 points).  The axis-major weight copy is laid out per pass.
+KFVM_HD constexpr int linear_pass_points(int dim, int nq) {
+  return dim == 3 ? (nq == 3 ? nq * nq : 2 * nq * nq) : 2 * nq;
+}
+
 KFVM_HD constexpr int num_monos(int D, int t) {
   return D == 2 ? (t + 1) * (t + 2) / 2 : (t + 1) * (t + 2) * (t + 3) / 6;
 }
