@@ -141,7 +141,8 @@ def main():
             flagged.append(float(s.flags[0].sum()))
     s.profile(False)
     kern = {}
-    for name in ("cell_props", "linear", "select", "weno", "weno_post", "flux_rk", "stage_pre"):
+    for name in ("cell_props", "linear", "select", "pp_list", "weno", "weno_post", "flux_rk",
+                 "stage_pre"):
         ms, cnt = s.profile_get(name)
         if cnt:
             kern[name] = {"ms_total": ms, "launches": cnt, "ms_avg": ms / cnt}

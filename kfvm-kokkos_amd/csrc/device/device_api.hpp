@@ -46,19 +46,27 @@ struct RKArgs {
 hipError_t launch_cell_props(int dim, const double *U, double *props, const Grid &g, hipStream_t s);
 // Linear reconstruction, KXRCF indicator and positivity limiter (all cells).
 // W0: full-stencil face weights [F*Q][C]; W0T: the same regrouped per pass
-// of linear_pass_points(dim, nq) points as [F*Q/P][C][P].
+// of linear_pass_points(dim, nq) points as [F*Q/P][C][P].  Euler: cells
+// whose states violate a limiter bound are appended to ppList (count in
+// *ppCount, zeroed by the caller) and limited by launch_pp_list.
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
                          double *thetaNew, const double *props, const Grid &g, const double *W0,
-                         const double *W0T, hipStream_t s);
+                         const double *W0T, int *ppList, int *ppCount, hipStream_t s);
+// Euler: limiter on the ppList cells that wflag does not mark for WENO.
+hipError_t launch_pp_list(int dim, int nq, const double *U, double *FS, double *thetaNew,
+                          const double *props, const double *wflag, const Grid &g,
+                          const int *ppList, const int *ppCount, hipStream_t s);
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                          double *wflag, int *list, int *counter, double wThresh, const Grid &g,
                          hipStream_t s);
 // KFVM-WENO reconstruction of the listed cells (list == nullptr: all).  Euler
 // leaves the face states in linearised primitive variables for
 // launch_weno_post; linear advection completes here.
+// FW, DW: reference weight layouts; FWS [nSub][C][F*Q], DWS [nSub][C][I]:
+// stencil-major copies (Euler).
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
-                       const Grid &g, const double *FW, const double *DW, const int *list,
-                       const int *counter, hipStream_t s);
+                       const Grid &g, const double *FW, const double *DW, const double *FWS,
+                       const double *DWS, const int *list, const int *counter, hipStream_t s);
 // Euler: back to conserved variables and positivity limiter on the listed cells.
 hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,
                             const double *props, const Grid &g, const int *list, const int *counter,

@@ -20,9 +20,15 @@ hipError_t launch_cell_props(int dim, const double *U, double *props, const Grid
 
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
                          double *thetaNew, const double *props, const Grid &g, const double *W0,
-                         const double *W0T, hipStream_t s) {
+                         const double *W0T, int *ppList, int *ppCount, hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, W0T, s));
+  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, W0T, ppList, ppCount, s));
+}
+
+hipError_t launch_pp_list(int dim, int nq, const double *U, double *FS, double *thetaNew,
+                          const double *props, const double *wflag, const Grid &g,
+                          const int *ppList, const int *ppCount, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::pp_list(U, FS, thetaNew, props, wflag, g, ppList, ppCount, s));
 }
 
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
@@ -32,10 +38,10 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
 }
 
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
-                       const Grid &g, const double *FW, const double *DW, const int *list,
-                       const int *counter, hipStream_t s) {
+                       const Grid &g, const double *FW, const double *DW, const double *FWS,
+                       const double *DWS, const int *list, const int *counter, hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, g, FW, DW, list, counter, s));
+  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, g, nv == 5 ? FWS : FW, nv == 5 ? DWS : DW, list, counter, s));
 }
 
 hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,

@@ -22,10 +22,10 @@ struct Launch {
   }
   static hipError_t linear(int nv, const double *U, double *FS, double *WI, double *thetaNew,
                            const double *props, const Grid &g, const double *W0, const double *W0T,
-                           hipStream_t s) {
+                           int *ppList, int *ppCount, hipStream_t s) {
     if (nv == 5)
       hipLaunchKernelGGL((dev::k_linear_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, FS, WI,
-                         thetaNew, props, g, W0T);
+                         thetaNew, props, g, W0T, ppList, ppCount);
     else
       hipLaunchKernelGGL((dev::k_linear<D, NQ, 1>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, WI,
                          thetaNew, g, W0);
@@ -47,6 +47,14 @@ struct Launch {
     else
       hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
                          dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
+    return hipGetLastError();
+  }
+  static hipError_t pp_list(const double *U, double *FS, double *thetaNew, const double *props,
+                            const double *wflag, const Grid &g, const int *ppList, const int *ppCount,
+                            hipStream_t s) {
+    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 2048u);
+    hipLaunchKernelGGL((dev::k_pp_list<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, wflag, g,
+                       ppList, ppCount);
     return hipGetLastError();
   }
   static hipError_t weno_post(const double *U, double *FS, double *thetaNew, const double *props,

@@ -7,6 +7,8 @@
 //                   conserved variables of the brick plus its radius-2 halo
 //                   are staged in LDS with coalesced loads (boundary
 //                   conditions applied by the index map of the load).
+//   k_pp_list       the limiter's exact sequence for the cells that failed
+//                   the fast test of k_linear_fused and are not WENO cells.
 //   k_cell_props    per-cell internal energy and sound speed, the two
 //                   neighbourhood quantities of the limiter bounds.
 //   k_wenorec5      Stencil_K.H:164-283, 416-454, 506-589: one lane per
@@ -39,7 +41,9 @@ typedef __attribute__((address_space(4))) double cdouble;
 template <int D>
 struct RBox {
   static constexpr int H = 2;  // stencil radius
-  static constexpr int BX = D == 3 ? 8 : 16, BY = D == 3 ? 8 : 16, BZ = D == 3 ? 4 : 1;
+  // 16 cells along x: a wave (16x4 cells) reads LDS rows of 16 doubles
+  // without bank conflicts and stores whole 128-byte lines
+  static constexpr int BX = 16, BY = D == 3 ? 4 : 16, BZ = D == 3 ? 4 : 1;
   static constexpr int LX = BX + 2 * H, LY = BY + 2 * H, LZ = D == 3 ? BZ + 2 * H : 1;
   static constexpr int N = LX * LY * LZ;   // staged cells per variable
   static constexpr int NT = BX * BY * BZ;  // threads per block
@@ -126,7 +130,7 @@ template <int D, int NQ>
 __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
     const double *__restrict__ U, double *__restrict__ FS, double *__restrict__ WI,
     double *__restrict__ thetaNew, const double *__restrict__ props, Grid g,
-    const double *__restrict__ WT) {
+    const double *__restrict__ WT, int *__restrict__ ppList, int *__restrict__ ppCount) {
   using Bx = RBox<D>;
   constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = SInfo<D>::C, Q = Shape<D, NQ>::Q, H = Bx::H, NP = Bx::N;
@@ -219,9 +223,20 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
       __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
     }
   }
-  double th = 1.0;
-  if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, cell, ref, pb);
-  thetaNew[cell] = th;
+  // Cells whose states violate a bound go to a list for k_pp_list (the
+  // limiter's exact sequence re-reads all states; keeping it out of this
+  // kernel keeps its waves convergent).
+  const bool slow = !(thD == 1.0 && okE);
+  if (!slow) thetaNew[cell] = 1.0;
+  const unsigned long long mask = __ballot(slow);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(ppCount, __popcll(mask));
+    base = __shfl(base, leader);
+    if (slow) ppList[base + __popcll(mask & ((1ull << lane) - 1ull))] = cell;
+  }
 }
 
 template <int D>
@@ -295,18 +310,109 @@ __device__ inline PPBounds pp_bounds_p(const double *__restrict__ U,
 // One lane per (cell, variable): linearised-primitive stencil values,
 // smoothness indicators of the full / centre / biased substencils, WENO-AO
 // weights and the weighted face values, still in linearised primitive
-// variables (k_weno_post converts them back).
+// variables (k_weno_post converts them back).  Every matrix-vector product
+// walks the stencil in its outer loop with one accumulator per output, so
+// each accumulator still sums the stencil cells in ascending order (the
+// reference's unblocked GEMV/GEMM order) while one stencil cell's weights
+// arrive as one contiguous scalar load.  Weight layouts (stencil-major):
+// FT[nSub][C][F*Q], DT[nSub][C][I].
+// Per-substencil pieces as templates, so that each one is a small fully
+// unrolled body with compile-time stencil indices (one fully unrolled loop
+// nest over all substencils exceeds the unroller's limits and would leave the
+// stencil values in scratch).
+template <int D>
+struct StTab {
+  static constexpr StencilTables<D> ST = make_stencil_tables<D>();
+};
+
+template <int D, int S>
+struct SubSten {
+  static constexpr int n = S == 0 ? SInfo<D>::C : (S == 1 ? SInfo<D>::Cc : SInfo<D>::Cb);
+  __device__ static constexpr int idx(int m) {
+    if constexpr (S < 2) return m;
+    else return StTab<D>::ST.sub[S - 2][m];
+  }
+};
+
+// beta_S = sum_i (D_S V_S)_i^2 (findSmoothness, Stencil_K.H:506-554)
+template <int D, int S>
+__device__ __attribute__((always_inline)) inline double weno_beta(const double (&vs)[SInfo<D>::C], const cdouble *dt) {
+  constexpr int C = SInfo<D>::C, I = SInfo<D>::I, ns = SubSten<D, S>::n;
+  double acc[I];
+#pragma unroll
+  for (int ii = 0; ii < I; ii++) acc[ii] = 0.0;
+#pragma unroll
+  for (int m = 0; m < ns; m++) {
+    const double x = vs[SubSten<D, S>::idx(m)];
+#pragma unroll
+    for (int ii = 0; ii < I; ii++) acc[ii] = fma(dt[(S * C + m) * I + ii], x, acc[ii]);
+  }
+  double beta = 0.0;
+#pragma unroll
+  for (int ii = 0; ii < I; ii++) beta = fma(acc[ii], acc[ii], beta);
+  return beta;
+}
+
+// out[h] += omega_S * (W_S V_S)[h] over HF face points (fillRiemannStates,
+// Stencil_K.H:416-454)
+template <int D, int S, int HF, int FQ>
+__device__ __attribute__((always_inline)) inline void weno_faces(const double (&vs)[SInfo<D>::C], const cdouble *fc, double om,
+                                  double (&out)[HF]) {
+  constexpr int C = SInfo<D>::C, ns = SubSten<D, S>::n;
+  double acc[HF];
+#pragma unroll
+  for (int h = 0; h < HF; h++) acc[h] = 0.0;
+#pragma unroll
+  for (int m = 0; m < ns; m++) {
+    const double x = vs[SubSten<D, S>::idx(m)];
+#pragma unroll
+    for (int h = 0; h < HF; h++) acc[h] = fma(fc[(S * C + m) * FQ + h], x, acc[h]);
+  }
+#pragma unroll
+  for (int h = 0; h < HF; h++) out[h] = fma(om, acc[h], out[h]);
+}
+
+template <int D, int S, int N>
+struct WenoSubs {
+  __device__ __attribute__((always_inline)) static inline void beta(const double (&vs)[SInfo<D>::C], const cdouble *dt,
+                                     double *om) {
+    om[S] = weno_beta<D, S>(vs, dt);
+    __builtin_amdgcn_sched_barrier(0);
+    WenoSubs<D, S + 1, N>::beta(vs, dt, om);
+  }
+  template <int HF, int FQ>
+  __device__ __attribute__((always_inline)) static inline void faces(const double (&vs)[SInfo<D>::C], const cdouble *fc,
+                                      const double *om, double (&out)[HF]) {
+    weno_faces<D, S, HF, FQ>(vs, fc, om[S], out);
+    __builtin_amdgcn_sched_barrier(0);
+    WenoSubs<D, S + 1, N>::template faces<HF, FQ>(vs, fc, om, out);
+  }
+};
+template <int D, int N>
+struct WenoSubs<D, N, N> {
+  __device__ static inline void beta(const double (&)[SInfo<D>::C], const cdouble *, double *) {}
+  template <int HF, int FQ>
+  __device__ static inline void faces(const double (&)[SInfo<D>::C], const cdouble *,
+                                      const double *, double (&)[HF]) {}
+};
+
+template <int D, int NQ>
+struct WenoChunk {  // face points per accumulation chunk
+  static constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  static constexpr int H = D == 3 ? (NQ == 2 ? 12 : 9) : FQ;
+};
+
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
                                                   double *__restrict__ FS, Grid g,
-                                                  const double *__restrict__ FW,
-                                                  const double *__restrict__ DW,
+                                                  const double *__restrict__ FT,
+                                                  const double *__restrict__ DT,
                                                   const int *__restrict__ list,
                                                   const int *__restrict__ counter) {
   constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb;
   constexpr int nSub = SInfo<D>::nSub, I = SInfo<D>::I;
-  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  constexpr int FQ = WenoChunk<D, NQ>::FQ, HF = WenoChunk<D, NQ>::H;
   constexpr double gHi = 0.7, gLo = 0.7;
   constexpr double g0 = gHi, g1 = (1.0 - gHi) * gLo, gb = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
   const Group G;
@@ -338,21 +444,14 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
         vs[m] = G.v == 0 ? s0 : (sv - uv * s0) / lp.rho;
       }
     }
+    // weight tables opaque to loop-invariant code motion (see k_linear_fused)
+    const double *dt0 = DT, *ft0 = FT;
+    asm volatile("" : "+s"(dt0), "+s"(ft0));
+    const cdouble *dt = (const cdouble *)dt0;
+    // smoothness indicators (findSmoothness)
     double om[nSub];
-#pragma unroll
-    for (int s = 0; s < nSub; s++) {
-      const int ns = s == 0 ? C : (s == 1 ? Cc : Cb);
-      double beta = 0.0;
-#pragma unroll 1
-      for (int ii = 0; ii < I; ii++) {
-        const double *dd = DW + (s * I + ii) * C;
-        double tt = 0.0;
-#pragma unroll
-        for (int m = 0; m < ns; m++) tt = fma(dd[m], vs[s < 2 ? m : ST.sub[s - 2][m]], tt);
-        beta = fma(tt, tt, beta);
-      }
-      om[s] = beta;
-    }
+    WenoSubs<D, 0, nSub>::beta(vs, dt, om);
+    // createOmegas
     double tot = 0.0;
 #pragma unroll
     for (int s = 0; s < nSub; s++) {
@@ -365,21 +464,45 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
     om[0] /= g0;
 #pragma unroll
     for (int s = 1; s < nSub; s++) om[s] -= om[0] * (s == 1 ? g1 : gb);
-    // rolled over face points: bounds the live weights to one point's 106
+    // fillRiemannStates, HF face points at a time
 #pragma unroll 1
-    for (int fq = 0; fq < FQ; fq++) {
-      double acc = 0.0;
+    for (int h0 = 0; h0 < FQ; h0 += HF) {
+      const double *fc0 = ft0 + h0;
+      asm volatile("" : "+s"(fc0));
+      const cdouble *fc = (const cdouble *)fc0;
+      double out[HF];
 #pragma unroll
-      for (int s = 0; s < nSub; s++) {
-        const int ns = s == 0 ? C : (s == 1 ? Cc : Cb);
-        const double *wq = FW + (s * FQ + fq) * C;
-        double tt = 0.0;
+      for (int h = 0; h < HF; h++) out[h] = 0.0;
+      WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, fc, om, out);
 #pragma unroll
-        for (int m = 0; m < ns; m++) tt = fma(wq[m], vs[s < 2 ? m : ST.sub[s - 2][m]], tt);
-        acc = fma(om[s], tt, acc);
-      }
-      FS[(size_t(fq) * 5 + G.v) * nc + c] = acc;
+      for (int h = 0; h < HF; h++) FS[(size_t(h0 + h) * 5 + G.v) * nc + c] = out[h];
     }
+  }
+}
+
+// Positivity limiter (Common_K.H:299-643) on the listed cells whose linear
+// states failed the fast test, except those the selector handed to WENO
+// (k_weno_post limits their new states).
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_pp_list(const double *__restrict__ U,
+                                                 double *__restrict__ FS,
+                                                 double *__restrict__ thetaNew,
+                                                 const double *__restrict__ props,
+                                                 const double *__restrict__ wflag, Grid g,
+                                                 const int *__restrict__ ppList,
+                                                 const int *__restrict__ ppCount) {
+  const int n = *ppCount;
+  const size_t nc = g.ncell;
+  for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
+    const int c = ppList[t];
+    if (wflag[c] != 0.0) continue;
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    double ref[5];
+#pragma unroll
+    for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
+    const PPBounds pb = pp_bounds_p<D>(U, props, g, i, j, k, ref);
+    thetaNew[c] = pp_apply<D, NQ>(FS, nc, c, ref, pb);
   }
 }
 

@@ -101,10 +101,13 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   theta_[1] = dalloc<double>(nc, "theta");
   wflag_ = dalloc<double>(nc, "wflag");
   list_ = dalloc<int>(nc, "weno list");
-  counter_ = dalloc<int>(1, "weno count");
+  counter_ = dalloc<int>(2, "weno / limiter counts");
+  ppList_ = dalloc<int>(nc, "limiter list");
   faceW_ = dalloc<double>(wts_.face.size(), "face weights");
   derivW_ = dalloc<double>(wts_.deriv.size(), "deriv weights");
-  faceWT_ = dalloc<double>(size_t(wts_.F) * wts_.Q * wts_.C, "face weights (axis-major)");
+  faceWT_ = dalloc<double>(size_t(wts_.F) * wts_.Q * wts_.C, "face weights (pass-major)");
+  faceWS_ = dalloc<double>(wts_.face.size(), "face weights (stencil-major)");
+  derivWS_ = dalloc<double>(wts_.deriv.size(), "deriv weights (stencil-major)");
   nPart_ = flux_rk_partials(cfg_.dim, grid_);
   errPart_ = dalloc<double>(nPart_, "err partials");
   posPart_ = dalloc<double>(nPart_, "pos partials");
@@ -130,8 +133,8 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
 
 Solver::~Solver() {
   profileEnable(false);
-  void *bufs[] = {U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_,
-                  faceW_, derivW_, faceWT_, errPart_, posPart_, redOut_, vmax_};
+  void *bufs[] = {U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -198,17 +201,22 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
   double *thNew = theta_[th_ ^ 1];
   if (hydro) timed("cell_props", [&] { return launch_cell_props(dim, Uin, props_, grid_, stream_); });
   if (useSparseWeno) {
+    if (hydro) check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
     timed("linear", [&] {
-      return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, faceWT_, stream_);
+      return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, faceWT_, ppList_,
+                           counter_ + 1, stream_);
     });
     if (hydro) {
       const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
-      check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
       timed("select", [&] {
         return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
       });
+      timed("pp_list", [&] {
+        return launch_pp_list(dim, nq, Uin, FS_, thNew, props_, wflag_, grid_, ppList_, counter_ + 1, stream_);
+      });
       timed("weno", [&] {
-        return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, list_, counter_, stream_);
+        return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, list_,
+                           counter_, stream_);
       });
       timed("weno_post", [&] {
         return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, list_, counter_, stream_);
@@ -216,7 +224,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     }
   } else {
     timed("weno", [&] {
-      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, nullptr, nullptr, stream_);
+      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr, nullptr, stream_);
     });
     if (hydro)
       timed("weno_post", [&] {
@@ -444,19 +452,29 @@ void Solver::setWeights(const double *face, const double *deriv) {
   uploadWeights();
 }
 // Device copies of the weight tables: the reference layouts
-// faceWeights[nSub][F][Q][C] and derivWeights[nSub][I][C], plus the
-// full-stencil face weights regrouped per pass of P face points as
-// [F*Q/P][C][P] (point index fastest) for the linear reconstruction, which
-// walks the stencil in its outer loop.
+// faceWeights[nSub][F][Q][C] and derivWeights[nSub][I][C]; the full-stencil
+// face weights regrouped per pass of P face points as [F*Q/P][C][P] for the
+// linear reconstruction; and stencil-major copies [nSub][C][F*Q] and
+// [nSub][C][I] for the WENO reconstruction.  The Euler kernels walk the
+// stencil in their outer loop, so one stencil cell's weights are contiguous.
 void Solver::uploadWeights() {
-  const int C = wts_.C, FQ = wts_.F * wts_.Q, P = linear_pass_points(wts_.dim, wts_.nq);
-  std::vector<double> wt(size_t(FQ) * C);
+  const int C = wts_.C, FQ = wts_.F * wts_.Q, I = wts_.I, nSub = wts_.nSub;
+  const int P = linear_pass_points(wts_.dim, wts_.nq);
+  std::vector<double> wt(size_t(FQ) * C), fs(wts_.face.size()), ds(wts_.deriv.size());
   for (int ps = 0; ps < FQ / P; ps++)
     for (int n = 0; n < C; n++)
       for (int q = 0; q < P; q++) wt[(size_t(ps) * C + n) * P + q] = wts_.face[size_t(ps * P + q) * C + n];
+  for (int sb = 0; sb < nSub; sb++)
+    for (int n = 0; n < C; n++) {
+      for (int fq = 0; fq < FQ; fq++)
+        fs[(size_t(sb) * C + n) * FQ + fq] = wts_.face[(size_t(sb) * FQ + fq) * C + n];
+      for (int i = 0; i < I; i++) ds[(size_t(sb) * C + n) * I + i] = wts_.deriv[(size_t(sb) * I + i) * C + n];
+    }
   check(hipMemcpy(faceW_, wts_.face.data(), wts_.face.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(derivW_, wts_.deriv.data(), wts_.deriv.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(faceWT_, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  check(hipMemcpy(faceWS_, fs.data(), fs.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  check(hipMemcpy(derivWS_, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
 }
 void Solver::sync() const { check(hipStreamSynchronize(stream_), "sync"); }
 

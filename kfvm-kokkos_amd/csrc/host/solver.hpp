@@ -86,8 +86,9 @@ class Solver {
   double *props_ = nullptr;  // per-cell internal energy and sound speed [2][ncell]
   double *theta_[2] = {nullptr, nullptr};  // positivity theta: previous RHS / this RHS
   int th_ = 0;                             // theta_[th_] holds the latest values
-  int *list_ = nullptr, *counter_ = nullptr;
-  double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr;
+  int *list_ = nullptr, *counter_ = nullptr;  // WENO list; counter_[0] WENO, [1] limiter count
+  int *ppList_ = nullptr;                      // cells for the limiter's exact sequence
+  double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr, *faceWS_ = nullptr, *derivWS_ = nullptr;
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;
   unsigned long long *vmax_ = nullptr;
   double *hostRed_ = nullptr;  // pinned: [err, pos, vmax bits, counter]
