@@ -141,7 +141,7 @@ def main():
             flagged.append(float(s.flags[0].sum()))
     s.profile(False)
     kern = {}
-    for name in ("cell_props", "linear", "select", "pp_list", "weno", "weno_post", "flux_rk",
+    for name in ("cell_props", "linear", "select", "weno", "weno_post", "pp", "flux_rk",
                  "stage_pre"):
         ms, cnt = s.profile_get(name)
         if cnt:

@@ -48,14 +48,16 @@ hipError_t launch_cell_props(int dim, const double *U, double *props, const Grid
 // W0: full-stencil face weights [F*Q][C]; W0T: the same regrouped per pass
 // of linear_pass_points(dim, nq) points as [F*Q/P][C][P].  Euler: cells
 // whose states violate a limiter bound are appended to ppList (count in
-// *ppCount, zeroed by the caller) and limited by launch_pp_list.
+// *ppCount, zeroed by the caller) and limited by launch_pp_lanes.
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
                          double *thetaNew, const double *props, const Grid &g, const double *W0,
                          const double *W0T, int *ppList, int *ppCount, hipStream_t s);
-// Euler: limiter on the ppList cells that wflag does not mark for WENO.
-hipError_t launch_pp_list(int dim, int nq, const double *U, double *FS, double *thetaNew,
-                          const double *props, const double *wflag, const Grid &g,
-                          const int *ppList, const int *ppCount, hipStream_t s);
+// Euler: the limiter's exact sequence on the cells of launch_linear whose
+// states failed the fast test (list, size *count); those wflag marks for
+// WENO are skipped.
+hipError_t launch_pp_lanes(int dim, int nq, const double *U, double *FS, double *thetaNew,
+                           const double *props, const double *wflag, const Grid &g,
+                           const int *list, const int *count, hipStream_t s);
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                          double *wflag, int *list, int *counter, double wThresh, const Grid &g,
                          hipStream_t s);
@@ -67,7 +69,8 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
                        const Grid &g, const double *FW, const double *DW, const double *FWS,
                        const double *DWS, const int *list, const int *counter, hipStream_t s);
-// Euler: back to conserved variables and positivity limiter on the listed cells.
+// Euler: back to conserved variables and the positivity limiter on the
+// listed cells.
 hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,
                             const double *props, const Grid &g, const int *list, const int *counter,
                             hipStream_t s);

@@ -25,10 +25,10 @@ hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, d
   KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, W0T, ppList, ppCount, s));
 }
 
-hipError_t launch_pp_list(int dim, int nq, const double *U, double *FS, double *thetaNew,
-                          const double *props, const double *wflag, const Grid &g,
-                          const int *ppList, const int *ppCount, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::pp_list(U, FS, thetaNew, props, wflag, g, ppList, ppCount, s));
+hipError_t launch_pp_lanes(int dim, int nq, const double *U, double *FS, double *thetaNew,
+                           const double *props, const double *wflag, const Grid &g,
+                           const int *list, const int *count, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::pp_lanes(U, FS, thetaNew, props, wflag, g, list, count, s));
 }
 
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,

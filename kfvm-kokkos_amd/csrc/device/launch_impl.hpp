@@ -49,12 +49,12 @@ struct Launch {
                          dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
     return hipGetLastError();
   }
-  static hipError_t pp_list(const double *U, double *FS, double *thetaNew, const double *props,
-                            const double *wflag, const Grid &g, const int *ppList, const int *ppCount,
-                            hipStream_t s) {
-    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 2048u);
-    hipLaunchKernelGGL((dev::k_pp_list<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, wflag, g,
-                       ppList, ppCount);
+  static hipError_t pp_lanes(const double *U, double *FS, double *thetaNew, const double *props,
+                             const double *wflag, const Grid &g, const int *list, const int *count,
+                             hipStream_t s) {
+    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 4096u);
+    hipLaunchKernelGGL((dev::k_pp_lanes<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, wflag, g,
+                       list, count);
     return hipGetLastError();
   }
   static hipError_t weno_post(const double *U, double *FS, double *thetaNew, const double *props,

@@ -7,8 +7,8 @@
 //                   conserved variables of the brick plus its radius-2 halo
 //                   are staged in LDS with coalesced loads (boundary
 //                   conditions applied by the index map of the load).
-//   k_pp_list       the limiter's exact sequence for the cells that failed
-//                   the fast test of k_linear_fused and are not WENO cells.
+//   k_pp_lanes      the limiter's exact sequence for the non-WENO cells that
+//                   failed the fast test of k_linear_fused.
 //   k_cell_props    per-cell internal energy and sound speed, the two
 //                   neighbourhood quantities of the limiter bounds.
 //   k_wenorec5      Stencil_K.H:164-283, 416-454, 506-589: one lane per
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
       __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
     }
   }
-  // Cells whose states violate a bound go to a list for k_pp_list (the
+  // Cells whose states violate a bound go to a list for k_pp_lanes (the
   // limiter's exact sequence re-reads all states; keeping it out of this
   // kernel keeps its waves convergent).
   const bool slow = !(thD == 1.0 && okE);
@@ -480,29 +480,113 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
   }
 }
 
-// Positivity limiter (Common_K.H:299-643) on the listed cells whose linear
-// states failed the fast test, except those the selector handed to WENO
-// (k_weno_post limits their new states).
+// Positivity limiter (Common_K.H:299-643), exact sequence, for the cells
+// whose linear states failed the fast test of k_linear_fused, skipping the
+// cells the selector handed to WENO (k_weno_post limits their new states).
+// Most listed cells are skipped, so a group of LPG lanes works on one cell, one lane
+// per face point; the bounds' 3^D neighbourhood is spread over the group and
+// every min / max over points is a group reduction (min and max do not depend
+// on the order of their operands, so the result is that of the sequential
+// loops).
+template <int FQ>
+struct PPGroup {
+  static constexpr int LPG = FQ <= 8 ? 8 : (FQ <= 16 ? 16 : (FQ <= 32 ? 32 : 64));
+};
+
+template <int LPG>
+__device__ inline double grp_min(double x) {
+#pragma unroll
+  for (int o = LPG / 2; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o));
+  return x;
+}
+template <int LPG>
+__device__ inline double grp_max(double x) {
+#pragma unroll
+  for (int o = LPG / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+  return x;
+}
+
 template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_pp_list(const double *__restrict__ U,
-                                                 double *__restrict__ FS,
-                                                 double *__restrict__ thetaNew,
-                                                 const double *__restrict__ props,
-                                                 const double *__restrict__ wflag, Grid g,
-                                                 const int *__restrict__ ppList,
-                                                 const int *__restrict__ ppCount) {
-  const int n = *ppCount;
+__global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
+                                                  double *__restrict__ FS,
+                                                  double *__restrict__ thetaNew,
+                                                  const double *__restrict__ props,
+                                                  const double *__restrict__ wflag, Grid g,
+                                                  const int *__restrict__ list,
+                                                  const int *__restrict__ count) {
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q, LPG = PPGroup<FQ>::LPG;
+  constexpr int NB = D == 3 ? 27 : 9, GPW = 64 / LPG;
+  const int n = *count;
+  const int lane = threadIdx.x & 63, pl = lane % LPG;
+  const int gid = (blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)) * GPW + lane / LPG;
+  const int ngroups = gridDim.x * (BLK / 64) * GPW;
   const size_t nc = g.ncell;
-  for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
-    const int c = ppList[t];
-    if (wflag[c] != 0.0) continue;
+  for (int t = gid; t < n; t += ngroups) {
+    const int c = list[t];
+    if (wflag[c] != 0.0) continue;  // uniform over the group
     int i, j, k;
     cell_ijk<D>(c, g, i, j, k);
     double ref[5];
 #pragma unroll
     for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
-    const PPBounds pb = pp_bounds_p<D>(U, props, g, i, j, k, ref);
-    thetaNew[c] = pp_apply<D, NQ>(FS, nc, c, ref, pb);
+    // bounds (pp_bounds_p with the neighbourhood spread over the group)
+    double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+    double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+    for (int nb = pl; nb < NB; nb += LPG) {
+      const int xo = D == 3 ? nb / 9 - 1 : nb / 3 - 1;
+      const int yo = D == 3 ? (nb / 3) % 3 - 1 : nb % 3 - 1;
+      const int zo = D == 3 ? nb % 3 - 1 : 0;
+      const CellRef r = map_cell<D, 5>(g, i + xo, j + yo, k + zo);
+      const double rho = U[r.idx];
+      maxDens = fmax(maxDens, rho);
+      minDens = fmin(minDens, rho);
+      minEint = fmin(minEint, props[r.idx]);
+      minSdSp = fmin(minSdSp, props[nc + r.idx]);
+    }
+    maxDens = grp_max<LPG>(maxDens);
+    minDens = grp_min<LPG>(minDens);
+    minEint = grp_min<LPG>(minEint);
+    minSdSp = grp_min<LPG>(minSdSp);
+    const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
+                       load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
+    const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
+                       load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
+    double dzW;
+    if (D == 3)
+      dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
+            load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
+    else
+      dzW = ref[3] / ref[0] - ref[3] / ref[0];
+    const double divV = (dxU + dyV + dzW) / 2.0;
+    double kappa = 0.2;
+    const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+    kappa *= (1.0 - eta);
+    PPBounds pb;
+    pb.rDens = ref[0];
+    pb.maxDens = maxDens * (1.0 + kappa);
+    pb.minDens = minDens * (1.0 - kappa);
+    pb.minEint = minEint * (1.0 - kappa);
+    // pp_apply with one lane per face point
+    const bool own = pl < FQ;
+    double st[5];
+    double *p = FS + (size_t(own ? pl : 0) * 5) * nc + c;
+#pragma unroll
+    for (int w = 0; w < 5; w++) st[w] = own ? p[w * nc] : ref[w];
+    const double thD = grp_min<LPG>(own ? dens_theta(1.0, st[0], pb) : 1.0);
+    const bool mixD = thD > 0.0 && thD < 1.0, repD = !mixD && thD < 1.0;
+#pragma unroll
+    for (int w = 0; w < 5; w++) st[w] = repD ? ref[w] : (mixD ? ref[w] + thD * (st[w] - ref[w]) : st[w]);
+    double thE = 1.0;
+    if (own && !(Hydro::eint(st) > pb.minEint)) thE = eint_pos(pb.minEint, ref, st);
+    const double thP = grp_min<LPG>(thE);
+    const bool mixP = thP > 0.0 && thP < 1.0, repP = !mixP && thP < 1.0;
+#pragma unroll
+    for (int w = 0; w < 5; w++) st[w] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[w] - ref[w]) : st[w]);
+    if (own && (thD < 1.0 || thP < 1.0)) {
+#pragma unroll
+      for (int w = 0; w < 5; w++) p[w * nc] = st[w];
+    }
+    if (pl == 0) thetaNew[c] = fmin(thD, thP);
   }
 }
 
@@ -546,6 +630,8 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
       thD = dens_theta(thD, d, pb);
       okE = okE && Hydro::eint(s) > pb.minEint;
     }
+    // most WENO cells need the exact sequence: kept inline (a separate
+    // lane-parallel pass costs more than the divergence it avoids)
     double th = 1.0;
     if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, c, ref, pb);
     thetaNew[c] = th;

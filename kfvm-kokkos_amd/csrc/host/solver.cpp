@@ -199,20 +199,22 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
   // theta of the previous RHS evaluation feeds the selector; this one writes the other buffer
   const double *thPrev = theta_[th_];
   double *thNew = theta_[th_ ^ 1];
-  if (hydro) timed("cell_props", [&] { return launch_cell_props(dim, Uin, props_, grid_, stream_); });
+  // counter_: [0] WENO cells, [1] cells of the linear pass for the limiter's
+  // exact sequence (ppList_)
+  if (hydro) {
+    check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
+    timed("cell_props", [&] { return launch_cell_props(dim, Uin, props_, grid_, stream_); });
+  }
+  lastRhsFull_ = !useSparseWeno;
   if (useSparseWeno) {
-    if (hydro) check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
     timed("linear", [&] {
-      return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, faceWT_, ppList_,
-                           counter_ + 1, stream_);
+      return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, faceWT_, ppList_, counter_ + 1,
+                           stream_);
     });
     if (hydro) {
       const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
       timed("select", [&] {
         return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
-      });
-      timed("pp_list", [&] {
-        return launch_pp_list(dim, nq, Uin, FS_, thNew, props_, wflag_, grid_, ppList_, counter_ + 1, stream_);
       });
       timed("weno", [&] {
         return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, list_,
@@ -221,10 +223,14 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
       timed("weno_post", [&] {
         return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, list_, counter_, stream_);
       });
+      timed("pp", [&] {
+        return launch_pp_lanes(dim, nq, Uin, FS_, thNew, props_, wflag_, grid_, ppList_, counter_ + 1, stream_);
+      });
     }
   } else {
     timed("weno", [&] {
-      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr, nullptr, stream_);
+      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr,
+                         nullptr, stream_);
     });
     if (hydro)
       timed("weno_post", [&] {
@@ -322,7 +328,7 @@ int Solver::takeStep() {
     std::memcpy(&maxVel, &hostRed_[2], sizeof(double));
     int cnt;
     std::memcpy(&cnt, &hostRed_[3], sizeof(int));
-    lastNWeno = cnt;
+    lastNWeno = lastRhsFull_ ? ncell_ : cnt;
 
     const double errNew = 1.0 / std::sqrt(errSum / nDofs);
     double dtfac = std::pow(errNew, rk_.ep1) * std::pow(errEst, rk_.ep2);

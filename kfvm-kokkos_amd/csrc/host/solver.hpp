@@ -93,6 +93,7 @@ class Solver {
   unsigned long long *vmax_ = nullptr;
   double *hostRed_ = nullptr;  // pinned: [err, pos, vmax bits, counter]
   int nPart_ = 0;
+  bool lastRhsFull_ = false;  // the last RHS evaluation ran WENO on every cell
   bool stateInCur_ = false;  // last step was not accepted: the solution is in U_[cur_]
   int stagePos_ = 0;         // position inside a step for advanceStages
 
