@@ -108,6 +108,16 @@ int kfvm_weight_sizes(int dim, int nq, int64_t *nface, int64_t *nderiv);
  * reconstruction weights for length scale lfac and cell sizes dx, dy, dz. */
 int kfvm_build_weights(int dim, int nq, double lfac, double dx, double dy, double dz,
                        double *face, int64_t nface, double *deriv, int64_t nderiv);
+/* Stencil::cellWeights [nsub][nq^dim][C] (src/stencil/Stencil.H:99,
+ * filled at src/stencil/Stencil.cpp:287-844): GP point values at the cell's
+ * interior Gauss points, point index (a*nq + b)*nq + c over (x, y, z).
+ * ncell must equal nsub * nq^dim * C. */
+int kfvm_build_cell_weights(int dim, int nq, double lfac, double dx, double dy, double dz,
+                            double *cell, int64_t ncell);
+/* Numeric::DiffMat (src/numeric/Numeric.H:59-106): the [nq][nq+2]
+ * differentiation matrix of the Lagrange basis on {-1/2, Gauss points, +1/2},
+ * row = Gauss point, column 0 = -1/2 face, nq+1 = +1/2 face.  n = nq*(nq+2). */
+int kfvm_diff_matrix(int nq, double *dm, int64_t n);
 /* ProblemSetup::setFromFile on text (src/ProblemSetup.H:142-205): writes the
  * parsed configuration as a JSON object into json[len]. */
 int kfvm_parse_init(const char *init_text, char *json, int64_t len);

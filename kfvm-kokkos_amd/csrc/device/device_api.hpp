@@ -21,7 +21,7 @@ struct Grid {
   double wt[3];      // 1D Gauss weights on (-1/2, 1/2)
   double vel[3];     // linear advection velocity
   int haveVel;       // 0: velocity (cos gamma, sin gamma, 0) with no z flux
-  int rs;            // 0 LLF, 1 HLL, 2 HLLC, 3 exact upwind
+  int rs;            // 0 LLF, 1 HLL, 2 HLLC, 3 exact upwind, 4 KEPES (GLM-MHD)
 };
 
 enum RKMode : int { RK_PRIME = 0, RK_STAGE = 1, RK_FSAL = 2 };
@@ -39,6 +39,8 @@ struct RKArgs {
   double *errPartial;   // per-block error sums (FSAL mode)
   double *posPartial;   // per-block positivity minima (FSAL mode)
   unsigned long long *vmaxBits;  // running max wave speed (bit pattern of a non-negative double)
+  const unsigned long long *chBits;  // GLM-MHD: {max(c_f + |u|), max |u|} of this evaluation
+  const double *src;                 // GLM-MHD: source terms at the interior points
 };
 
 // Euler: per-cell internal energy and sound speed, props [2][ncell], read by
@@ -74,6 +76,24 @@ hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, dou
 hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,
                             const double *props, const Grid &g, const int *list, const int *counter,
                             hipStream_t s);
+// GLM-MHD (9 variables).  SRC [(p * 9 + v)][ncell]: states, then source
+// terms, at the NQ^D interior points; CW: interior weights [nSub][NQ^D][C];
+// Dm: differentiation matrix [NQ][NQ + 2]; chBits: the two maxima of the
+// cleaning speed (zeroed by the caller before launch_mhd_speed).
+hipError_t launch_mhd_linear(int dim, int nq, const double *U, double *FS, double *SRC, double *WI,
+                             const Grid &g, const double *W0, const double *CW, hipStream_t s);
+hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI, const double *theta,
+                             double *wflag, int *list, int *counter, double wThresh, const Grid &g,
+                             hipStream_t s);
+hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
+                           const double *FW, const double *DW, const double *CW, const int *list,
+                           const int *counter, hipStream_t s);
+hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
+                         const Grid &g, hipStream_t s);
+hipError_t launch_mhd_speed(int dim, int nq, const double *FS, const Grid &g, unsigned long long *chBits,
+                            hipStream_t s);
+hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
+                              const double *theta, const Grid &g, const double *Dm, hipStream_t s);
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
                           const Grid &g, const RKArgs &a, hipStream_t s);
 int flux_rk_partials(int dim, const Grid &g);

@@ -50,9 +50,36 @@ hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double
   KFVM_SELECT(dim, nq, L::weno_post(U, FS, thetaNew, props, g, list, counter, s));
 }
 
+hipError_t launch_mhd_linear(int dim, int nq, const double *U, double *FS, double *SRC, double *WI,
+                             const Grid &g, const double *W0, const double *CW, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_linear(U, FS, SRC, WI, g, W0, CW, s));
+}
+hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI, const double *theta,
+                             double *wflag, int *list, int *counter, double wThresh, const Grid &g,
+                             hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_select(U, WI, theta, wflag, list, counter, wThresh, g, s));
+}
+hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
+                           const double *FW, const double *DW, const double *CW, const int *list,
+                           const int *counter, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FW, DW, CW, list, counter, s));
+}
+hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
+                         const Grid &g, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_pp(U, FS, SRC, thetaNew, g, s));
+}
+hipError_t launch_mhd_speed(int dim, int nq, const double *FS, const Grid &g, unsigned long long *chBits,
+                            hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_speed(FS, g, chBits, s));
+}
+hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
+                              const double *theta, const Grid &g, const double *Dm, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_sources(U, FS, SRC, theta, g, Dm, s));
+}
+
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
                           const Grid &g, const RKArgs &a, hipStream_t s) {
-  if (nv != 5 && nv != 1) return hipErrorInvalidValue;
+  if (nv != 5 && nv != 1 && nv != 9) return hipErrorInvalidValue;
   KFVM_SELECT(dim, nq, L::flux_rk(nv, U, FS, g, a, s));
 }
 
@@ -68,6 +95,9 @@ hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double
                        K, U, Uhat, Utmp, betaDt, bhatDt, ncell);
   else if (nv == 1)
     hipLaunchKernelGGL((dev::k_stage_pre<1>), dim3(stage_blocks(ncell)), dim3(dev::BLK), 0, s, Uprev,
+                       K, U, Uhat, Utmp, betaDt, bhatDt, ncell);
+  else if (nv == 9)
+    hipLaunchKernelGGL((dev::k_stage_pre<9>), dim3(stage_blocks(ncell)), dim3(dev::BLK), 0, s, Uprev,
                        K, U, Uhat, Utmp, betaDt, bhatDt, ncell);
   else
     return hipErrorInvalidValue;

@@ -64,6 +64,46 @@ struct Launch {
                        counter);
     return hipGetLastError();
   }
+  // ---- GLM-MHD
+  static hipError_t mhd_linear(const double *U, double *FS, double *SRC, double *WI, const Grid &g,
+                               const double *W0, const double *CW, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_mhd_linear<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
+                       g, W0, CW);
+    hipLaunchKernelGGL((dev::k_mhd_indicator<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, FS, WI,
+                       g);
+    return hipGetLastError();
+  }
+  static hipError_t mhd_select(const double *U, const double *WI, const double *theta, double *wflag,
+                               int *list, int *counter, double wThresh, const Grid &g, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_select<D, NQ, dev::Mhd>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, WI,
+                       theta, wflag, list, counter, wThresh, g);
+    return hipGetLastError();
+  }
+  static hipError_t mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FW,
+                             const double *DW, const double *CW, const int *list, const int *counter,
+                             hipStream_t s) {
+    const unsigned nb = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
+    hipLaunchKernelGGL((dev::k_mhd_weno<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, SRC, g, FW, DW, CW, list,
+                       counter);
+    return hipGetLastError();
+  }
+  static hipError_t mhd_pp(const double *U, double *FS, double *SRC, double *thetaNew, const Grid &g,
+                           hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_mhd_pp<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
+                       thetaNew, g);
+    return hipGetLastError();
+  }
+  static hipError_t mhd_speed(const double *FS, const Grid &g, unsigned long long *chBits, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_mhd_speed<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, FS, g,
+                       chBits);
+    return hipGetLastError();
+  }
+  static hipError_t mhd_sources(const double *U, const double *FS, double *SRC, const double *theta,
+                                const Grid &g, const double *Dm, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_mhd_sources<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
+                       theta, g, Dm);
+    return hipGetLastError();
+  }
   static dim3 flux_grid(const Grid &g) {
     using Bk = dev::Brick<D>;
     return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X), unsigned((g.ny + Bk::Y - 1) / Bk::Y),
@@ -75,6 +115,8 @@ struct Launch {
     const dim3 blk(Bk::X, Bk::Y, Bk::Z);
     if (nv == 5)
       hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 5>), flux_grid(g), blk, 0, s, U, FS, g, a);
+    else if (nv == 9)
+      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 9>), flux_grid(g), blk, 0, s, U, FS, g, a);
     else
       hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 1>), flux_grid(g), blk, 0, s, U, FS, g, a);
     return hipGetLastError();

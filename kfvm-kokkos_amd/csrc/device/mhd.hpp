@@ -1,0 +1,463 @@
+// Stage kernels of the GLM-MHD equations (9 variables: rho, mx, my, mz, E,
+// Bx, By, Bz, psi).  One thread per cell; the face states (FS) and the
+// interior quadrature-point states (SRC, needed by the GLM source terms) are
+// planes [(point * 9 + var)][cell].
+//
+//   k_mhd_linear     KernelLinearRecon_K incl. the source branch
+//                    (Stencil_K.H:32-118)
+//   k_mhd_indicator  p/rho^gamma of every unlimited face state (the KXRCF
+//                    input, Numeric_K.H:315-379)
+//   k_mhd_weno       KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504)
+//                    with VarConvert<MHD_GLM> (MHD_GLM_K.H:24-69)
+//   k_mhd_pp         PositivityPreserve_K with interior points
+//                    (Common_K.H:299-643)
+//   k_mhd_speed      SpeedEstimate_K (MHD_GLM_K.H:213-310): the cleaning speed
+//                    c_h = max(c_f + |u|) - max(|u|), x-components on every face
+//   k_mhd_sources    SourceTerms_K (MHD_GLM_K.H:316-547): div B and psi-gradient
+//                    terms at the interior points
+#pragma once
+
+namespace kfvm {
+namespace dev {
+
+template <int D, int NQ>
+struct MShape {
+  static constexpr int F = 2 * D, Q = D == 3 ? NQ * NQ : NQ, FQ = F * Q;
+  static constexpr int NI = D == 3 ? NQ * NQ * NQ : NQ * NQ;  // interior points
+  static constexpr int NZ = D == 3 ? NQ : 1;
+};
+
+// linearised primitive context about the centre cell (VarConvert<MHD_GLM>)
+struct MhdLin {
+  double gamm, rho, u, v, w, bx, by, bz, rPsi, vsq;
+  __device__ inline MhdLin(const double *r, double gamma)
+      : gamm(gamma - 1.0), rho(r[0]), u(r[1] / r[0]), v(r[2] / r[0]), w(r[3] / r[0]), bx(r[5]),
+        by(r[6]), bz(r[7]), rPsi(r[8]), vsq(0.0) {
+    vsq = (u * u + v * v + w * w) / 2.0;
+  }
+};
+
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U,
+                                                    double *__restrict__ FS,
+                                                    double *__restrict__ SRC, Grid g,
+                                                    const double *__restrict__ W,
+                                                    const double *__restrict__ CW) {
+  constexpr auto ST = make_stencil_tables<D>();
+  constexpr int C = SInfo<D>::C, FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+#pragma unroll 1
+  for (int v = 0; v < 9; v++) {
+    double vs[C];
+#pragma unroll
+    for (int n = 0; n < C; n++)
+      vs[n] = load_u<D, 9>(U, g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2], v);
+#pragma unroll 2
+    for (int fq = 0; fq < FQ; fq++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int n = 0; n < C; n++) acc = fma(W[fq * C + n], vs[n], acc);
+      FS[(size_t(fq) * 9 + v) * nc + c] = acc;
+    }
+#pragma unroll 2
+    for (int p = 0; p < NI; p++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int n = 0; n < C; n++) acc = fma(CW[p * C + n], vs[n], acc);
+      SRC[(size_t(p) * 9 + v) * nc + c] = acc;
+    }
+  }
+}
+
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_indicator(const double *__restrict__ FS,
+                                                       double *__restrict__ WI, Grid g) {
+  constexpr int FQ = MShape<D, NQ>::FQ;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  const size_t nc = g.ncell;
+#pragma unroll 1
+  for (int fq = 0; fq < FQ; fq++) {
+    double s[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
+    WI[size_t(fq) * nc + c] = Mhd::weno(g.gamma, s);
+  }
+}
+
+// One output point of the WENO combination: sum_s omega_s (W_s . V_s).
+template <int D>
+__device__ inline double weno_point(const double (&vs)[SInfo<D>::C], const double *om,
+                                    const double *__restrict__ Wt, int stride, int row) {
+  constexpr auto &ST = StTab<D>::ST;
+  constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb, nSub = SInfo<D>::nSub;
+  double acc = 0.0;
+#pragma unroll
+  for (int s = 0; s < nSub; s++) {
+    const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
+    const double *w = Wt + (size_t(s) * stride + row) * C;
+    double t = 0.0;
+#pragma unroll
+    for (int m = 0; m < n; m++) t = fma(w[m], vs[s < 2 ? m : ST.sub[s - 2][m]], t);
+    acc = fma(om[s], t, acc);
+  }
+  return acc;
+}
+
+// Back to conserved variables (linPrim2Cons) accumulated in the output planes:
+// the variables are processed in the order 0,1,2,3,5,6,7,4,8, which is the
+// operand order of the reference's energy sum
+//   E = vsq d + rho u mx + rho v my + rho w mz + bx Bx + by By + bz Bz
+//       + e / (gamma - 1) + psi0 psi.
+__device__ inline void mhd_store(double *__restrict__ P, size_t nc, int v, double acc,
+                                 const MhdLin &lp) {
+  double *e = P + 4 * nc;
+  if (v == 0) {
+    P[0] = acc;
+    *e = lp.vsq * acc;
+  } else if (v < 4) {
+    const double uv = v == 1 ? lp.u : (v == 2 ? lp.v : lp.w);
+    P[v * nc] = lp.rho * acc + uv * P[0];
+    *e = *e + lp.rho * uv * acc;
+  } else if (v < 8 && v > 4) {
+    const double bv = v == 5 ? lp.bx : (v == 6 ? lp.by : lp.bz);
+    P[v * nc] = acc;
+    *e = *e + bv * acc;
+  } else if (v == 4) {
+    *e = *e + acc / lp.gamm;
+  } else {
+    P[8 * nc] = acc;
+    *e = *e + lp.rPsi * acc;
+  }
+}
+
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_weno(const double *__restrict__ U,
+                                                  double *__restrict__ FS,
+                                                  double *__restrict__ SRC, Grid g,
+                                                  const double *__restrict__ FW,
+                                                  const double *__restrict__ DW,
+                                                  const double *__restrict__ CW,
+                                                  const int *__restrict__ list,
+                                                  const int *__restrict__ counter) {
+  constexpr auto ST = make_stencil_tables<D>();
+  constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb;
+  constexpr int nSub = SInfo<D>::nSub, I = SInfo<D>::I;
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+  constexpr double gHi = 0.7, gLo = 0.7;
+  constexpr double g0 = gHi, g1 = (1.0 - gHi) * gLo, gb = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
+  const int nt = list ? *counter : g.ncell;
+  const size_t nc = g.ncell;
+  for (int t = blockIdx.x * BLK + threadIdx.x; t < nt; t += gridDim.x * BLK) {
+    const int c = list ? list[t] : t;
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    double ref[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) ref[v] = U[size_t(v) * nc + c];
+    const MhdLin lp(ref, g.gamma);
+#pragma unroll 1
+    for (int vi = 0; vi < 9; vi++) {
+      const int v = vi < 4 ? vi : (vi < 7 ? vi + 1 : (vi == 7 ? 4 : 8));
+      const double uv = v == 1 ? lp.u : (v == 2 ? lp.v : lp.w);
+      // stencil values of variable v, linearised primitive (cons2LinPrim)
+      double vs[C];
+#pragma unroll
+      for (int n = 0; n < C; n++) {
+        const CellRef r = map_cell<D, 9>(g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2]);
+        if (v == 4) {
+          double s[9];
+#pragma unroll
+          for (int w = 0; w < 9; w++) s[w] = ld(U, nc, r, w);
+          vs[n] = lp.gamm * (lp.vsq * s[0] + s[4] - lp.u * s[1] - lp.v * s[2] - lp.w * s[3] -
+                             lp.bx * s[5] - lp.by * s[6] - lp.bz * s[7] - lp.rPsi * s[8]);
+        } else if (v >= 1 && v <= 3) {
+          vs[n] = (ld(U, nc, r, v) - uv * ld(U, nc, r, 0)) / lp.rho;
+        } else {
+          vs[n] = ld(U, nc, r, v);
+        }
+      }
+      // smoothness indicators and WENO-AO weights
+      double om[nSub];
+#pragma unroll
+      for (int s = 0; s < nSub; s++) {
+        const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
+        double beta = 0.0;
+#pragma unroll 1
+        for (int ii = 0; ii < I; ii++) {
+          const double *dd = DW + (s * I + ii) * C;
+          double tt = 0.0;
+#pragma unroll
+          for (int m = 0; m < n; m++) tt = fma(dd[m], vs[s < 2 ? m : ST.sub[s - 2][m]], tt);
+          beta = fma(tt, tt, beta);
+        }
+        om[s] = beta;
+      }
+      double tot = 0.0;
+#pragma unroll
+      for (int s = 0; s < nSub; s++) {
+        const double gs = s == 0 ? g0 : (s == 1 ? g1 : gb);
+        om[s] = gs / (om[s] * om[s] + 1.e-40);
+        tot += om[s];
+      }
+#pragma unroll
+      for (int s = 0; s < nSub; s++) om[s] /= tot;
+      om[0] /= g0;
+#pragma unroll
+      for (int s = 1; s < nSub; s++) om[s] -= om[0] * (s == 1 ? g1 : gb);
+      // face points, then interior points
+#pragma unroll 1
+      for (int fq = 0; fq < FQ; fq++)
+        mhd_store(FS + size_t(fq) * 9 * nc + c, nc, v, weno_point<D>(vs, om, FW, FQ, fq), lp);
+#pragma unroll 1
+      for (int p = 0; p < NI; p++)
+        mhd_store(SRC + size_t(p) * 9 * nc + c, nc, v, weno_point<D>(vs, om, CW, NI, p), lp);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ limiter
+__device__ inline double mhd_eint_pos(double emin, const double *ref, const double *s) {
+  double a = 0.0, b = 1.0, c;
+  double st[9];
+  for (int it = 0; it < 10; it++) {
+    c = 0.5 * (a + b);
+#pragma unroll
+    for (int v = 0; v < 9; v++) st[v] = ref[v] + c * (s[v] - ref[v]);
+    const double dec = Mhd::eint(st) - emin;
+    if (dec > 0.0) a = c;
+    else b = c;
+  }
+  return a;
+}
+
+template <int D, int NQ>
+__device__ inline void mhd_scale(double *__restrict__ FS, double *__restrict__ SRC, size_t nc,
+                                 int c, const double *ref, double th, bool replace) {
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+#pragma unroll 1
+  for (int p = 0; p < FQ + NI; p++) {
+    double *b = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
+#pragma unroll
+    for (int v = 0; v < 9; v++) b[v * nc] = replace ? ref[v] : ref[v] + th * (b[v * nc] - ref[v]);
+  }
+}
+
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
+                                                double *__restrict__ FS,
+                                                double *__restrict__ SRC,
+                                                double *__restrict__ thetaNew, Grid g) {
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+  double ref[9];
+#pragma unroll
+  for (int v = 0; v < 9; v++) ref[v] = U[size_t(v) * nc + c];
+  // ranges over the 3^D neighbourhood (Common_K.H:329-387)
+  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
+#pragma unroll 1
+  for (int xo = -1; xo <= 1; xo++)
+#pragma unroll 1
+    for (int yo = -1; yo <= 1; yo++)
+#pragma unroll 1
+      for (int zo = zl; zo <= zh; zo++) {
+        const CellRef r = map_cell<D, 9>(g, i + xo, j + yo, k + zo);
+        double s[9];
+#pragma unroll
+        for (int v = 0; v < 9; v++) s[v] = ld(U, nc, r, v);
+        maxDens = fmax(maxDens, s[0]);
+        minDens = fmin(minDens, s[0]);
+        minEint = fmin(minEint, Mhd::eint(s));
+        minSdSp = fmin(minSdSp, Mhd::sound(g.gamma, s));
+      }
+  const double dxU = load_u<D, 9>(U, g, i + 1, j, k, 1) / load_u<D, 9>(U, g, i + 1, j, k, 0) -
+                     load_u<D, 9>(U, g, i - 1, j, k, 1) / load_u<D, 9>(U, g, i - 1, j, k, 0);
+  const double dyV = load_u<D, 9>(U, g, i, j + 1, k, 2) / load_u<D, 9>(U, g, i, j + 1, k, 0) -
+                     load_u<D, 9>(U, g, i, j - 1, k, 2) / load_u<D, 9>(U, g, i, j - 1, k, 0);
+  double dzW;
+  if (D == 3)
+    dzW = load_u<D, 9>(U, g, i, j, k + 1, 3) / load_u<D, 9>(U, g, i, j, k + 1, 0) -
+          load_u<D, 9>(U, g, i, j, k - 1, 3) / load_u<D, 9>(U, g, i, j, k - 1, 0);
+  else
+    dzW = ref[3] / ref[0] - ref[3] / ref[0];
+  const double divV = (dxU + dyV + dzW) / 2.0;
+  double kappa = 0.2;
+  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+  kappa *= (1.0 - eta);
+  maxDens *= (1.0 + kappa);
+  minDens *= (1.0 - kappa);
+  minEint *= (1.0 - kappa);
+  PPBounds b;
+  b.rDens = ref[0];
+  b.maxDens = maxDens;
+  b.minDens = minDens;
+  b.minEint = minEint;
+  // density: faces then interior points
+  double thD = 1.0;
+#pragma unroll 1
+  for (int p = 0; p < FQ + NI; p++) {
+    const double rho = p < FQ ? FS[size_t(p) * 9 * nc + c] : SRC[size_t(p - FQ) * 9 * nc + c];
+    thD = dens_theta(thD, rho, b);
+  }
+  if (thD > 0.0 && thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, false);
+  else if (thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, true);
+  // internal energy
+  double thP = 1.0;
+#pragma unroll 1
+  for (int p = 0; p < FQ + NI; p++) {
+    const double *q = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
+    double s[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) s[v] = q[v * nc];
+    const double ei = Mhd::eint(s);
+    thP = ei > minEint ? thP : fmin(thP, mhd_eint_pos(minEint, ref, s));
+  }
+  if (thP > 0.0 && thP < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thP, false);
+  else if (thP < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thP, true);
+  thetaNew[c] = fmin(thD, thP);
+}
+
+// ------------------------------------------------------------ cleaning speed
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_speed(const double *__restrict__ FS, Grid g,
+                                                   unsigned long long *__restrict__ chBits) {
+  constexpr int FQ = MShape<D, NQ>::FQ;
+  __shared__ double sh[BLK];
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  double maxLam = 0.0, maxVel = 0.0;
+  if (c < g.ncell) {
+    const size_t nc = g.ncell;
+#pragma unroll 1
+    for (int fq = 0; fq < FQ; fq++) {
+      double s[9];
+#pragma unroll
+      for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
+      const double vel = fabs(s[1] / s[0]);
+      const double lam = Mhd::fast(g.gamma, s, s[5]) + vel;
+      maxVel = vel > maxVel ? vel : maxVel;
+      maxLam = lam > maxLam ? lam : maxLam;
+    }
+  }
+  const double bl = block_max(maxLam, sh, threadIdx.x);
+  const double bv = block_max(maxVel, sh, threadIdx.x);
+  if (threadIdx.x == 0) {
+    if (bl > 0.0) atomicMax(&chBits[0], (unsigned long long)__double_as_longlong(bl));
+    if (bv > 0.0) atomicMax(&chBits[1], (unsigned long long)__double_as_longlong(bv));
+  }
+}
+
+__device__ inline double glm_ch(const unsigned long long *chBits) {
+  return __longlong_as_double((long long)chBits[0]) - __longlong_as_double((long long)chBits[1]);
+}
+
+// -------------------------------------------------------------- GLM sources
+// Dm: differentiation matrix [NQ][NQ+2] over the points (-1/2, interior, +1/2)
+// of one cell (DiffMatLUT, Numeric.cpp:47-99).
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_sources(const double *__restrict__ U,
+                                                     const double *__restrict__ FS,
+                                                     double *__restrict__ SRC,
+                                                     const double *__restrict__ theta, Grid g,
+                                                     const double *__restrict__ Dm) {
+  constexpr int Q = MShape<D, NQ>::Q, NI = MShape<D, NQ>::NI, NZ = MShape<D, NQ>::NZ;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+  auto own = [&](int f, int q, int v) { return FS[(size_t(f * Q + q) * 9 + v) * nc + c]; };
+  // normal field averaged over the two states of each face point
+  auto bav = [&](int f, int q) {
+    const int ax = f >> 1, hi = f & 1;
+    double in[9], out[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) in[v] = own(f, q, v);
+    const int ni = i + (ax == 0 ? (hi ? 1 : -1) : 0), nj = j + (ax == 1 ? (hi ? 1 : -1) : 0);
+    const int nk = k + (ax == 2 ? (hi ? 1 : -1) : 0);
+    face_state<D, NQ, 9>(FS, g, ni, nj, nk, f ^ 1, q, in, out);
+    return 0.5 * (in[5 + ax] + out[5 + ax]);
+  };
+  double divB = 0.0;
+#pragma unroll 1
+  for (int nQ = 0; nQ < NQ; nQ++)
+#pragma unroll 1
+    for (int nS = 0; nS < NZ; nS++) {
+      const int q = nS * NQ + nQ;
+      const double twt = D == 2 ? g.wt[nQ] : g.wt[nQ] * g.wt[nS];
+      divB += twt * (bav(1, q) - bav(0, q)) / g.dx;
+      divB += twt * (bav(3, q) - bav(2, q)) / g.dy;
+      if (D == 3) divB += twt * (bav(5, q) - bav(4, q)) / g.dz;
+    }
+  auto src = [&](int p, int v) -> double & { return SRC[(size_t(p) * 9 + v) * nc + c]; };
+  // u . grad(psi) at the interior points
+  double gpsiDu[NI];
+  const bool fd = theta[c] < 0.75;
+#pragma unroll 1
+  for (int nQ = 0; nQ < NQ; nQ++)
+#pragma unroll 1
+    for (int nR = 0; nR < NQ; nR++)
+#pragma unroll 1
+      for (int nS = 0; nS < NZ; nS++) {
+        const int p = (nQ * NQ + nR) * NZ + nS;
+        const double rho = src(p, 0);
+        const double vx = src(p, 1) / rho, vy = src(p, 2) / rho, vz = src(p, 3) / rho;
+        double dpx, dpy, dpz = 0.0;
+        if (fd) {
+          const double tdx = 1.0 / (2.0 * g.dx), tdy = 1.0 / (2.0 * g.dy);
+          dpx = tdx * (load_u<D, 9>(U, g, i + 1, j, k, 8) - load_u<D, 9>(U, g, i - 1, j, k, 8));
+          dpy = tdy * (load_u<D, 9>(U, g, i, j + 1, k, 8) - load_u<D, 9>(U, g, i, j - 1, k, 8));
+          if (D == 3) {
+            const double tdz = 1.0 / (2.0 * g.dz);
+            dpz = tdz * (load_u<D, 9>(U, g, i, j, k + 1, 8) - load_u<D, 9>(U, g, i, j, k - 1, 8));
+          }
+        } else {
+          const int qx = D == 2 ? nR : nR * NQ + nS, qy = D == 2 ? nQ : nQ * NQ + nS;
+          dpx = Dm[nQ * (NQ + 2)] * own(0, qx, 8) + Dm[nQ * (NQ + 2) + NQ + 1] * own(1, qx, 8);
+          for (int n = 1; n <= NQ; n++) dpx += Dm[nQ * (NQ + 2) + n] * src(((n - 1) * NQ + nR) * NZ + nS, 8);
+          dpx /= g.dx;
+          dpy = Dm[nR * (NQ + 2)] * own(2, qy, 8) + Dm[nR * (NQ + 2) + NQ + 1] * own(3, qy, 8);
+          for (int n = 1; n <= NQ; n++) dpy += Dm[nR * (NQ + 2) + n] * src((nQ * NQ + n - 1) * NZ + nS, 8);
+          dpy /= g.dy;
+          if (D == 3) {
+            const int qz = nQ * NQ + nR;
+            dpz = Dm[nS * (NQ + 2)] * own(4, qz, 8) + Dm[nS * (NQ + 2) + NQ + 1] * own(5, qz, 8);
+            for (int n = 1; n <= NQ; n++) dpz += Dm[nS * (NQ + 2) + n] * src((nQ * NQ + nR) * NZ + n - 1, 8);
+            dpz /= g.dz;
+          }
+        }
+        gpsiDu[p] = vx * dpx + vy * dpy + vz * dpz;
+      }
+  // source terms (no user forcing for the registry problems)
+#pragma unroll
+  for (int p = 0; p < NI; p++) {
+    double st[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) st[v] = src(p, v);
+    const double vx = st[1] / st[0], vy = st[2] / st[0], vz = st[3] / st[0];
+    const double uDb = vx * st[5] + vy * st[6] + vz * st[7];
+    const double f0 = 0.0;
+    src(p, 0) = f0;
+    src(p, 1) = f0 - divB * st[5];
+    src(p, 2) = f0 - divB * st[6];
+    src(p, 3) = f0 - divB * st[7];
+    const double eForce = divB * uDb + gpsiDu[p] * st[8];
+    src(p, 4) = f0 - eForce;
+    src(p, 5) = f0 - divB * vx;
+    src(p, 6) = f0 - divB * vy;
+    src(p, 7) = f0 - divB * vz;
+    src(p, 8) = -gpsiDu[p];
+  }
+}
+
+}  // namespace dev
+}  // namespace kfvm

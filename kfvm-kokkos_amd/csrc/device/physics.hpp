@@ -9,6 +9,8 @@
 //   HLLC             src/physics/hydro/Hydro_Riemann.H:97-187
 //   HLL / LLF        src/physics/Common_K.H:23-63
 //   LinAdv exact     src/physics/linadv/LinAdv_Riemann.H:63-90
+//   GLM-MHD          src/physics/mhd_glm/MHD_GLM_EOS.H:15-144,
+//                    MHD_GLM_Riemann.H:18-231
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -134,6 +136,165 @@ struct Hydro {
     } else {
       flux<DIR>(uR, vR, F);
     }
+    return sMax;
+  }
+};
+
+// GLM-MHD, variables (rho, mx, my, mz, E, Bx, By, Bz, psi)
+// (MHD_GLM_EOS.H:15-144, MHD_GLM_Riemann.H:18-231).  DIR 0/1/2 rotates the
+// normal direction as ord = {DIR, DIR+1, DIR+2} (mod 3).
+struct Mhd {
+  static constexpr int NV = 9;
+  __device__ static inline double ekin(const double *U) {
+    return 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0];
+  }
+  __device__ static inline double prsb(const double *U) {
+    return 0.5 * (U[5] * U[5] + U[6] * U[6] + U[7] * U[7]);
+  }
+  __device__ static inline double eint(const double *U) {
+    return (U[4] - prsb(U) - ekin(U) - 0.5 * U[8] * U[8]);
+  }
+  __device__ static inline double pres(double g, const double *U) { return (g - 1.0) * eint(U); }
+  __device__ static inline double sound(double g, const double *U) {
+    return sqrt(g * pres(g, U) / U[0]);
+  }
+  __device__ static inline double weno(double g, const double *U) {
+    return pres(g, U) / pow(U[0], g);
+  }
+  __device__ static inline double fast(double g, const double *U, double bl) {
+    const double asq = g * pres(g, U) / U[0];
+    const double bsq = 2.0 * prsb(U) / U[0];
+    const double blsq = bl * bl / U[0];
+    const double absq = asq + bsq;
+    return sqrt(0.5 * (absq + sqrt(absq * absq - 4.0 * asq * blsq)));
+  }
+  __device__ static inline void cons2prim(double g, const double *U, double *V) {
+    V[0] = U[0];
+    V[1] = U[1] / U[0];
+    V[2] = U[2] / U[0];
+    V[3] = U[3] / U[0];
+    V[4] = pres(g, U);
+    V[5] = U[5];
+    V[6] = U[6];
+    V[7] = U[7];
+    V[8] = U[8];
+  }
+  template <int DIR>
+  __device__ static inline void flux(double ch, const double *U, const double *V, double *F) {
+    constexpr int v0 = 1 + DIR, v1 = 1 + (DIR + 1) % 3, v2 = 1 + (DIR + 2) % 3;
+    constexpr int b0 = 5 + DIR, b1 = 5 + (DIR + 1) % 3, b2 = 5 + (DIR + 2) % 3;
+    const double mx = U[v0];
+    const double vx = V[v0], vy = V[v1], vz = V[v2];
+    const double bx = U[b0], by = U[b1], bz = U[b2];
+    const double et = U[4], pg = V[4];
+    const double chpsi = ch * U[8], hpsisq = 0.5 * U[8] * U[8];
+    const double uDb = vx * bx + vy * by + vz * bz, bsq = bx * bx + by * by + bz * bz;
+    const double pt = pg + 0.5 * bsq;
+    F[0] = mx;
+    F[v0] = mx * vx + pt - bx * bx;
+    F[v1] = mx * vy - bx * by;
+    F[v2] = mx * vz - bx * bz;
+    F[4] = vx * (et + pt - hpsisq) + bx * (chpsi - uDb);
+    F[b0] = chpsi;
+    F[b1] = vx * by - vy * bx;
+    F[b2] = vx * bz - vz * bx;
+    F[8] = ch * bx;
+  }
+  // Davis estimates with c_h, extremised with the Wu-Shu positivity estimates
+  template <int DIR>
+  __device__ static inline double speeds(double g, double ch, const double *uL, const double *uR,
+                                         const double *vL, const double *vR, double &sL,
+                                         double &sR) {
+    constexpr int v0 = 1 + DIR, b0 = 5 + DIR, b1 = 5 + (DIR + 1) % 3, b2 = 5 + (DIR + 2) % 3;
+    const double cfL = fast(g, uL, uL[b0]);
+    const double cfR = fast(g, uR, uR[b0]);
+    const double sdL = fmin(vL[v0] - fmax(ch, cfL), vR[v0] - fmax(ch, cfR));
+    const double sdR = fmax(vL[v0] + fmax(ch, cfL), vR[v0] + fmax(ch, cfR));
+    const double rtL = sqrt(uL[0]), rtR = sqrt(uR[0]);
+    const double rtSum = rtL + rtR, vSum = (rtL * vL[v0] + rtR * vR[v0]) / rtSum;
+    const double bd0 = uL[b0] - uR[b0], bd1 = uL[b1] - uR[b1], bd2 = uL[b2] - uR[b2];
+    const double bnDiff = sqrt(bd0 * bd0 + bd1 * bd1 + bd2 * bd2) / rtSum;
+    const double aL = fmin(vL[v0], vSum) - cfL - bnDiff;
+    const double aR = fmin(vR[v0], vSum) + cfR + bnDiff;
+    sL = fmin(sdL, aL);
+    sR = fmax(sdR, aR);
+    return fmax(fabs(sL), fabs(sR));
+  }
+  // generic HLL / LLF (Common_K.H:23-63)
+  template <int DIR>
+  __device__ static inline double solve_hll(double g, double ch, bool llf, const double *uL,
+                                            const double *uR, double *F) {
+    double vL[9], vR[9], FR[9];
+    cons2prim(g, uL, vL);
+    cons2prim(g, uR, vR);
+    double sL, sR;
+    const double sMax = speeds<DIR>(g, ch, uL, uR, vL, vR, sL, sR);
+    if (llf) {
+      sL = -sMax;
+      sR = sMax;
+    }
+    const double tR = (fmin(0.0, sR) - fmin(0.0, sL)) / (sR - sL), tL = 1.0 - tR;
+    const double tD = 0.5 * (sR * fabs(sL) - sL * fabs(sR)) / (sR - sL);
+    flux<DIR>(ch, uL, vL, F);
+    flux<DIR>(ch, uR, vR, FR);
+#pragma unroll
+    for (int v = 0; v < 9; v++) F[v] = tL * F[v] + tR * FR[v] - tD * (uR[v] - uL[v]);
+    return sMax;
+  }
+  __device__ static inline double logMean(double a, double b) {
+    const double zeta = a / b, f = (zeta - 1.0) / (zeta + 1.0), u = f * f;
+    const double F = u < 0.01 ? 1.0 + u / 3.0 + u * u / 5.0 + u * u * u / 7.0 : 0.5 * log(zeta) / f;
+    return 0.5 * (a + b) / F;
+  }
+  __device__ static inline double am(double a, double b) { return 0.5 * (a + b); }
+  // entropy-conserving flux plus LLF dissipation (MHD_GLM_Riemann.H:118-231)
+  template <int DIR>
+  __device__ static inline double solve_kepes(double g, double ch, const double *uL,
+                                              const double *uR, double *F) {
+    constexpr int v0 = 1 + DIR, v1 = 1 + (DIR + 1) % 3, v2 = 1 + (DIR + 2) % 3;
+    constexpr int b0 = 5 + DIR, b1 = 5 + (DIR + 1) % 3, b2 = 5 + (DIR + 2) % 3;
+    double vL[9], vR[9];
+    cons2prim(g, uL, vL);
+    cons2prim(g, uR, vR);
+    double sL, sR;
+    const double sMax = speeds<DIR>(g, ch, uL, uR, vL, vR, sL, sR);
+    const double rhoLn = logMean(uL[0], uR[0]);
+    const double rhoAv = am(uL[0], uR[0]);
+    const double uAv = am(vL[v0], vR[v0]), vAv = am(vL[v1], vR[v1]), wAv = am(vL[v2], vR[v2]);
+    const double uuAv = am(vL[v0] * vL[v0], vR[v0] * vR[v0]);
+    const double vvAv = am(vL[v1] * vL[v1], vR[v1] * vR[v1]);
+    const double wwAv = am(vL[v2] * vL[v2], vR[v2] * vR[v2]);
+    const double bxAv = am(uL[b0], uR[b0]), byAv = am(uL[b1], uR[b1]), bzAv = am(uL[b2], uR[b2]);
+    const double bxxAv = am(uL[b0] * uL[b0], uR[b0] * uR[b0]);
+    const double byyAv = am(uL[b1] * uL[b1], uR[b1] * uR[b1]);
+    const double bzzAv = am(uL[b2] * uL[b2], uR[b2] * uR[b2]);
+    const double uBxAv = am(vL[v0] * uL[b0], vR[v0] * uR[b0]);
+    const double vByAv = am(vL[v1] * uL[b1], vR[v1] * uR[b1]);
+    const double wBzAv = am(vL[v2] * uL[b2], vR[v2] * uR[b2]);
+    const double uBxxAv = am(vL[v0] * uL[b0] * uL[b0], vR[v0] * uR[b0] * uR[b0]);
+    const double uByyAv = am(vL[v0] * uL[b1] * uL[b1], vR[v0] * uR[b1] * uR[b1]);
+    const double uBzzAv = am(vL[v0] * uL[b2] * uL[b2], vR[v0] * uR[b2] * uR[b2]);
+    const double psiAv = am(uL[8], uR[8]);
+    const double bxPsiAv = am(uL[b0] * uL[8], uR[b0] * uR[8]);
+    const double betaL = uL[0] / (2.0 * vL[4]);
+    const double betaR = uR[0] / (2.0 * vR[4]);
+    const double betaLn = logMean(betaL, betaR);
+    const double betaAv = am(betaL, betaR);
+    const double pTil = 0.5 * rhoAv / betaAv;
+    const double pHatTot = pTil + 0.5 * (bxxAv + byyAv + bzzAv);
+    F[0] = rhoLn * uAv;
+    F[v0] = rhoLn * uAv * uAv + pHatTot - bxAv * bxAv;
+    F[v1] = rhoLn * uAv * vAv - bxAv * byAv;
+    F[v2] = rhoLn * uAv * wAv - bxAv * bzAv;
+    F[b0] = ch * psiAv;
+    F[b1] = uAv * byAv - vAv * bxAv;
+    F[b2] = uAv * bzAv - wAv * bxAv;
+    F[8] = ch * bxAv;
+    F[4] = (1.0 / (2.0 * (g - 1.0) * betaLn) - 0.5 * (uuAv + vvAv + wwAv)) * F[0] + uAv * F[v0] +
+           vAv * F[v1] + wAv * F[v2] + bxAv * F[b0] + byAv * F[b1] + bzAv * F[b2] + psiAv * F[8] -
+           0.5 * (uBxxAv + uByyAv + uBzzAv) + bxAv * (uBxAv + vByAv + wBzAv) - ch * bxPsiAv;
+#pragma unroll
+    for (int v = 0; v < 9; v++) F[v] -= sMax * (uR[v] - uL[v]) / 2.0;
     return sMax;
   }
 };

@@ -74,7 +74,7 @@ __device__ inline CellRef map_cell(const Grid &g, int i, int j, int k) {
   CellRef r;
   r.idx = (size_t(kk) * g.ny + jj) * g.nx + ii;
   r.neg = 0u;
-  if (NV == 5) {
+  if (NV >= 5) {
     r.neg = side_mask(sx, g.negmask[0], g.negmask[1]) ^ side_mask(sy, g.negmask[2], g.negmask[3]);
     if (D == 3) r.neg ^= side_mask(sz, g.negmask[4], g.negmask[5]);
   }
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
 }
 
 // ------------------------------------------------------------ KXRCF selector
-template <int D, int NQ>
+template <int D, int NQ, class P = Hydro>
 __global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
                                                 const double *__restrict__ WI,
                                                 const double *__restrict__ theta,
@@ -319,10 +319,10 @@ __global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
         maxJump = fmax(maxJump, fabs(wout - win));
       }
     }
-    double uc[5];
+    double uc[P::NV];
 #pragma unroll
-    for (int v = 0; v < 5; v++) uc[v] = U[size_t(v) * nc + c];
-    const double ind = maxJump / Hydro::weno(g.gamma, uc);
+    for (int v = 0; v < P::NV; v++) uc[v] = U[size_t(v) * nc + c];
+    const double ind = maxJump / P::weno(g.gamma, uc);
     flagged = ind >= wThresh || theta[c] < 1.0;
     wflag[c] = flagged ? 1.0 : 0.0;
   }
@@ -504,8 +504,12 @@ __global__ __launch_bounds__(BLK) void k_weno(const double *__restrict__ U,
 
 // -------------------------------------------- Riemann + divergence + RK update
 template <int NV, int DIR>
-__device__ inline double riemann(const Grid &g, const double *uL, const double *uR, double *Fl) {
-  if (NV == 5) {
+__device__ inline double riemann(const Grid &g, double ch, const double *uL, const double *uR,
+                                 double *Fl) {
+  if (NV == 9) {
+    if (g.rs == 4) return Mhd::solve_kepes<DIR>(g.gamma, ch, uL, uR, Fl);
+    return Mhd::solve_hll<DIR>(g.gamma, ch, g.rs == 0, uL, uR, Fl);
+  } else if (NV == 5) {
     if (g.rs == 2) return Hydro::solve_hllc<DIR>(g.gamma, uL, uR, Fl);
     return Hydro::solve_hll<DIR>(g.gamma, g.rs == 0, uL, uR, Fl);
   } else {
@@ -585,7 +589,7 @@ __device__ inline void face_state(const double *__restrict__ FS, const Grid &g, 
     if ((low ? bcLo : bcHi) != 0) {
       const unsigned mLo = ax == 0 ? g.negmask[0] : (ax == 1 ? g.negmask[2] : g.negmask[4]);
       const unsigned mHi = ax == 0 ? g.negmask[1] : (ax == 1 ? g.negmask[3] : g.negmask[5]);
-      const unsigned m = NV == 5 ? (low ? mLo : mHi) : 0u;
+      const unsigned m = NV >= 5 ? (low ? mLo : mHi) : 0u;
 #pragma unroll
       for (int v = 0; v < NV; v++) out[v] = ((m >> v) & 1u) ? -inner[v] : inner[v];
       return;
@@ -616,6 +620,10 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   const size_t nc = g.ncell;
   const size_t c = live ? (size_t(ck) * g.ny + cj) * g.nx + ci : 0;
   double vmax = 0.0, errLoc = 0.0, posLoc = 1.0;
+  // GLM cleaning speed of this evaluation (NV == 9)
+  const double ch = NV == 9 ? __longlong_as_double((long long)a.chBits[0]) -
+                                  __longlong_as_double((long long)a.chBits[1])
+                            : 0.0;
   double rhs[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) rhs[v] = 0.0;
@@ -666,9 +674,9 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
           face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, uL, uR);
         }
         double sp;
-        if (dir == 0) sp = riemann<NV, 0>(g, uL, uR, Fl);
-        else if (dir == 1) sp = riemann<NV, 1>(g, uL, uR, Fl);
-        else sp = riemann<NV, 2>(g, uL, uR, Fl);
+        if (dir == 0) sp = riemann<NV, 0>(g, ch, uL, uR, Fl);
+        else if (dir == 1) sp = riemann<NV, 1>(g, ch, uL, uR, Fl);
+        else sp = riemann<NV, 2>(g, ch, uL, uR, Fl);
         vmax = sp > vmax ? sp : vmax;
       }
 #pragma unroll
@@ -700,6 +708,21 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
     __syncthreads();
   }
 
+  if (live && NV == 9) {
+    // source terms at the interior points (IntegrateRHS_K)
+    constexpr int NZ = D == 3 ? NQ : 1;
+#pragma unroll 1
+    for (int nQ = 0; nQ < NQ; nQ++)
+#pragma unroll 1
+      for (int nR = 0; nR < NQ; nR++)
+#pragma unroll 1
+        for (int nS = 0; nS < NZ; nS++) {
+          const int p = (nQ * NQ + nR) * NZ + nS;
+          const double w = D == 3 ? g.wt[nQ] * g.wt[nR] * g.wt[nS] : g.wt[nQ] * g.wt[nR];
+#pragma unroll
+          for (int v = 0; v < NV; v++) rhs[v] += w * a.src[(size_t(p) * NV + v) * nc + c];
+        }
+  }
   if (live) {
     if (a.mode == RK_PRIME) {
 #pragma unroll
@@ -732,7 +755,8 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
         const double e = (u - uh) / (a.atol + a.rtol * fmax(fabs(u), fabs(uh)));
         errLoc += pow(e, 2.0);
       }
-      posLoc = hasNan ? -1.0 : (NV == 5 ? Hydro::positivity(st) : 1.0);
+      posLoc = hasNan ? -1.0
+                      : (NV == 5 ? Hydro::positivity(st) : (NV == 9 ? fmin(st[0], Mhd::eint(st)) : 1.0));
     }
   }
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -752,6 +776,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
 }  // namespace kfvm
 
 #include "recon5.hpp"
+#include "mhd.hpp"
 
 namespace kfvm {
 namespace dev {

@@ -306,6 +306,29 @@ int kfvm_build_weights(int dim, int nq, double lfac, double dx, double dy, doubl
   });
 }
 
+int kfvm_build_cell_weights(int dim, int nq, double lfac, double dx, double dy, double dz,
+                            double *cell, int64_t ncell) {
+  if ((dim != 2 && dim != 3) || (nq != 2 && nq != 3)) return fail("dim must be 2|3 and nq 2|3");
+  const int nSub = dim == 3 ? 8 : 6, C = dim == 3 ? 33 : 13;
+  const int64_t n = int64_t(nSub) * (dim == 3 ? nq * nq * nq : nq * nq) * C;
+  if (!cell || ncell != n) return fail("cell weight buffer has the wrong size");
+  return guarded([&] {
+    kfvm::WeightTable w = kfvm::build_weights(dim, nq, lfac, dx, dy, dz);
+    std::memcpy(cell, w.cell.data(), w.cell.size() * sizeof(double));
+    return 0;
+  });
+}
+
+int kfvm_diff_matrix(int nq, double *dm, int64_t n) {
+  if (nq != 2 && nq != 3) return fail("nq must be 2|3");
+  if (!dm || n != int64_t(nq) * (nq + 2)) return fail("differentiation matrix buffer has the wrong size");
+  return guarded([&] {
+    const std::vector<double> D = kfvm::diff_matrix(nq);
+    std::memcpy(dm, D.data(), D.size() * sizeof(double));
+    return 0;
+  });
+}
+
 int kfvm_parse_init(const char *init_text, char *json, int64_t len) {
   if (!init_text || !json || len <= 0) return fail("null argument");
   return guarded([&] {

@@ -39,6 +39,37 @@ QuadRule quad_rule(int nq) {
   return q;
 }
 
+std::vector<double> diff_matrix(int nq) {
+  const QuadRule qr = quad_rule(nq);
+  // nodes in extended precision; the quadrature nodes are the rounded doubles
+  std::vector<long double> X;
+  X.push_back(-0.5L);
+  for (int q = 0; q < nq; q++) X.push_back((long double)qr.ab[q]);
+  X.push_back(0.5L);
+  const int n = nq + 2;
+  std::vector<double> D(size_t(nq) * n, 0.0);
+  for (int q = 0; q < nq; q++) {
+    const int k = q + 1;
+    for (int m = 0; m < n; m++) {
+      long double v;
+      if (m == k) {
+        v = 0.0L;
+        for (int j = 0; j < n; j++)
+          if (j != k) v += 1.0L / (X[k] - X[j]);
+      } else {
+        long double num = 1.0L, den = 1.0L;
+        for (int j = 0; j < n; j++) {
+          if (j != m && j != k) num *= X[k] - X[j];
+          if (j != m) den *= X[m] - X[j];
+        }
+        v = num / den;
+      }
+      D[size_t(q) * n + m] = double(v);
+    }
+  }
+  return D;
+}
+
 bool RKTable::needsTmp() const {
   for (int s = 0; s < nStages; s++)
     if (gam2[s] != 0.0 || delta[s] != 0.0) return true;
@@ -295,6 +326,7 @@ bool apply_line(const std::string &category, const std::string &setting, Config 
         else if (val(1) == "hll") c.rs = Riemann::HLL;
         else if (val(1) == "llf") c.rs = Riemann::LLF;
         else if (val(1) == "exact") c.rs = Riemann::Exact;
+        else if (val(1) == "kepes") c.rs = Riemann::KEPES;
         else return false;
       } else if (key == "rk") {
         if (val(1) == "ssp4_3_2") c.rk = RK::SSP4_3_2;

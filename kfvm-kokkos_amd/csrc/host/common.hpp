@@ -17,7 +17,7 @@
 namespace kfvm {
 
 enum class Eq : int { Hydro = 0, LinAdv = 1, MhdGlm = 2 };
-enum class Riemann : int { LLF = 0, HLL = 1, HLLC = 2, Exact = 3 };
+enum class Riemann : int { LLF = 0, HLL = 1, HLLC = 2, Exact = 3, KEPES = 4 };
 enum class RK : int { ThreeStarP = 0, FourStarP = 1, SSP4_3_2 = 2, SSP10_4_3 = 3 };
 enum class BC : int { Periodic = 0, Outflow = 1, Reflecting = 2, Noslip = 3 };
 enum Face : int { West = 0, East, South, North, Bottom, Top };
@@ -39,6 +39,12 @@ struct QuadRule {
   double wt[5] = {0, 0, 0, 0, 0};
 };
 QuadRule quad_rule(int nq);
+
+// Differentiation matrix of one cell in cell units (DiffMatLUT,
+// Numeric.cpp:47-99): row q holds the derivative weights at interior node
+// ab[q] of the Lagrange interpolant through the nodes
+// (-1/2, ab[0], ..., ab[nq-1], +1/2).  Returned as [nq][nq + 2].
+std::vector<double> diff_matrix(int nq);
 
 // Low-storage RK-FSAL coefficient table (Numeric.H:117-282).
 struct RKTable {

@@ -74,6 +74,65 @@ void ic_smooth_wave(const Config &c, uint64_t, double x, double y, double z, dou
             1.0 + 0.1 * std::cos(ph), s);
 }
 
+// GLM-MHD states (rho, mx, my, mz, E, Bx, By, Bz, psi)
+void mhd_state(double gamma, double rho, double u, double v, double w, double p, double bx,
+               double by, double bz, double *s) {
+  s[0] = rho;
+  s[1] = rho * u;
+  s[2] = rho * v;
+  s[3] = rho * w;
+  s[4] = p / (gamma - 1.0) + rho * (u * u + v * v + w * w) / 2.0 + (bx * bx + by * by + bz * bz) / 2.0;
+  s[5] = bx;
+  s[6] = by;
+  s[7] = bz;
+  s[8] = 0.0;
+}
+// 3D vortex: rho = gamma^2, p = gamma, v = (-sin 2pi y, sin 2pi x, 0.1 sin 2pi z),
+// B = (-sin 2pi y, sin 4pi x, 0) / sqrt(4 pi)
+void ic_mhd_vortex3d(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  const double g = c.gamma, b0 = 1.0 / std::sqrt(4.0 * M_PI);
+  mhd_state(g, g * g, -std::sin(2.0 * M_PI * y), std::sin(2.0 * M_PI * x), 0.1 * std::sin(2.0 * M_PI * z), g,
+            -std::sin(2.0 * M_PI * y) * b0, std::sin(4.0 * M_PI * x) * b0, 0.0, s);
+}
+// Orszag-Tang vortex (exec/MHD/OrszagTang/InitialCondition.H)
+void ic_orszag_tang(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  const double g = c.gamma, eps = 0.2;
+  const double u = -(1.0 + eps * std::sin(2.0 * M_PI * z)) * std::sin(2.0 * M_PI * y);
+  const double v = (1.0 + eps * std::sin(2.0 * M_PI * z)) * std::sin(2.0 * M_PI * x);
+  const double w = eps * std::sin(2.0 * M_PI * z);
+  mhd_state(g, g * g, u, v, w, g, -std::sin(2.0 * M_PI * y), std::sin(4.0 * M_PI * x), 0.0, s);
+}
+// Circularly polarised Alfven wave along a tilted direction
+// (exec/MHD/CircAlfven/InitialCondition.H), periodic on [0,3] x [0,1.5]^2
+void ic_circ_alfven(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  const double sina = 2.0 / 3.0, sinb = 2.0 / std::sqrt(5.0);
+  const double cosa = std::sqrt(5.0) / 3.0, cosb = 1.0 / std::sqrt(5.0);
+  const double x1 = x * cosa * cosb + y * cosa * sinb + z * sina;
+  const double u1 = 0.0, u2 = 0.1 * std::sin(2.0 * M_PI * x1), u3 = 0.1 * std::cos(2.0 * M_PI * x1);
+  const double b1 = 1.0, b2 = 0.1 * std::sin(2.0 * M_PI * x1), b3 = 0.1 * std::cos(2.0 * M_PI * x1);
+  const double u = u1 * cosa * cosb - u2 * sinb - u3 * sina * cosb;
+  const double v = u1 * cosa * sinb + u2 * cosb - u3 * sina * sinb;
+  const double w = u1 * sina + u3 * cosa;
+  const double bx = b1 * cosa * cosb - b2 * sinb - b3 * sina * cosb;
+  const double by = b1 * cosa * sinb + b2 * cosb - b3 * sina * sinb;
+  const double bz = b1 * sina + b3 * cosa;
+  mhd_state(c.gamma, 1.0, u, v, w, 0.1, bx, by, bz, s);
+}
+// uniform state: user p0..p7 = rho, u, v, w, p, Bx, By, Bz
+void ic_mhd_uniform(const Config &c, uint64_t, double, double, double, double *s) {
+  mhd_state(c.gamma, c.user[0], c.user[1], c.user[2], c.user[3], c.user[4], c.user[5], c.user[6],
+            c.user[7], s);
+}
+// blast: pressure 10 inside radius p0 of the box centre, 0.1 outside;
+// rho = 1, B = (p1, p2, 0)
+void ic_mhd_blast(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  const double xc = 0.5 * (c.xLo + c.xUp), yc = 0.5 * (c.yLo + c.yUp);
+  const double zc = c.dim == 3 ? 0.5 * (c.zLo + c.zUp) : z;
+  const double r2 = (x - xc) * (x - xc) + (y - yc) * (y - yc) + (z - zc) * (z - zc);
+  mhd_state(c.gamma, 1.0, 0.0, 0.0, 0.0, r2 < c.user[0] * c.user[0] ? 10.0 : 0.1, c.user[1], c.user[2],
+            0.0, s);
+}
+
 struct Entry {
   const char *name;
   Eq eq;
@@ -87,6 +146,11 @@ const Entry kRegistry[] = {
     {"quadrant2d", Eq::Hydro, ic_quadrant2d},
     {"uniform", Eq::Hydro, ic_uniform},
     {"smooth_wave", Eq::Hydro, ic_smooth_wave},
+    {"mhd_vortex3d", Eq::MhdGlm, ic_mhd_vortex3d},
+    {"orszag_tang", Eq::MhdGlm, ic_orszag_tang},
+    {"circ_alfven", Eq::MhdGlm, ic_circ_alfven},
+    {"mhd_uniform", Eq::MhdGlm, ic_mhd_uniform},
+    {"mhd_blast", Eq::MhdGlm, ic_mhd_blast},
 };
 
 }  // namespace

@@ -37,7 +37,6 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   if (cfg_.dim != 2 && cfg_.dim != 3) throw std::runtime_error("problem:dim must be 2 or 3");
   if (cfg_.rad != 2) throw std::runtime_error("only stencil radius 2 is built");
   if (cfg_.nQuad != 2 && cfg_.nQuad != 3) throw std::runtime_error("problem:nquad must be 2 or 3");
-  if (cfg_.eq == Eq::MhdGlm) throw std::runtime_error("mhd_glm is not built yet");
   if (cfg_.dim == 2) cfg_.nZ = 1;
   const int64_t nmin = cfg_.dim == 3 ? std::min({cfg_.nX, cfg_.nY, cfg_.nZ}) : std::min(cfg_.nX, cfg_.nY);
   if (nmin < 2) throw std::runtime_error("every direction needs at least 2 cells");
@@ -46,8 +45,10 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
       throw std::runtime_error("periodic boundaries must be set on both sides of an axis");
   if (cfg_.eq == Eq::LinAdv && cfg_.rs != Riemann::Exact)
     throw std::runtime_error("linear advection uses problem:riemann exact");
-  if (cfg_.eq == Eq::Hydro && cfg_.rs == Riemann::Exact)
+  if (cfg_.eq == Eq::Hydro && (cfg_.rs == Riemann::Exact || cfg_.rs == Riemann::KEPES))
     throw std::runtime_error("hydro needs problem:riemann hllc, hll or llf");
+  if (cfg_.eq == Eq::MhdGlm && (cfg_.rs == Riemann::Exact || cfg_.rs == Riemann::HLLC))
+    throw std::runtime_error("mhd_glm needs problem:riemann llf, hll or kepes");
   if (!problem_known(cfg_.problem, cfg_.eq))
     throw std::runtime_error("unknown problem '" + cfg_.problem + "' for this equation");
   if (cfg_.cfl <= 0.0) cfg_.cfl = cfg_.cflValue();
@@ -65,8 +66,12 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   for (int f = 0; f < 6; f++) {
     grid_.bc[f] = int(cfg_.bc[f]);
     const int ax = f / 2;
-    grid_.negmask[f] = cfg_.bc[f] == BC::Reflecting ? (1u << (1 + ax))
-                       : cfg_.bc[f] == BC::Noslip   ? 0xEu
+    // mirrored walls negate the normal momentum (reflecting) or all momenta
+    // (noslip), and for GLM-MHD the normal field (Hydro_BCCoeffs.H:15-21,
+    // MHD_GLM_BCCoeffs.H:16-22)
+    const unsigned bn = cfg_.eq == Eq::MhdGlm ? (1u << (5 + ax)) : 0u;
+    grid_.negmask[f] = cfg_.bc[f] == BC::Reflecting ? ((1u << (1 + ax)) | bn)
+                       : cfg_.bc[f] == BC::Noslip   ? (0xEu | bn)
                                                     : 0u;
   }
   grid_.dx = cfg_.dx();
@@ -97,6 +102,16 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   FS_ = dalloc<double>(nc * FQ * nv_, "face states");
   WI_ = dalloc<double>(nc * FQ, "indicator");
   props_ = dalloc<double>(nc * 2, "cell properties");
+  if (cfg_.eq == Eq::MhdGlm) {
+    const QuadRule qr2 = quad_rule(cfg_.nQuad);
+    const int NI = wts_.nqCell;
+    SRC_ = dalloc<double>(nc * NI * nv_, "interior point states");
+    cellW_ = dalloc<double>(wts_.cell.size(), "interior weights");
+    diffM_ = dalloc<double>(size_t(qr2.n) * (qr2.n + 2), "differentiation matrix");
+    chBits_ = dalloc<unsigned long long>(2, "cleaning speed");
+    const std::vector<double> Dm = diff_matrix(cfg_.nQuad);
+    check(hipMemcpy(diffM_, Dm.data(), Dm.size() * sizeof(double), hipMemcpyHostToDevice), "copy diff matrix");
+  }
   theta_[0] = dalloc<double>(nc, "theta");
   theta_[1] = dalloc<double>(nc, "theta");
   wflag_ = dalloc<double>(nc, "wflag");
@@ -133,7 +148,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
 
 Solver::~Solver() {
   profileEnable(false);
-  void *bufs[] = {U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
+  void *bufs[] = {SRC_, cellW_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
                   faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
@@ -193,7 +208,48 @@ bool Solver::profileGet(const std::string &name, double &ms, int64_t &count) {
   return true;
 }
 
+// GLM-MHD evaluation (Solver.cpp:410-530 with sources): reconstruction of
+// face and interior states, selection, WENO, limiter on every cell, cleaning
+// speed, source terms, then the Riemann / divergence / RK kernel.
+void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
+  const int dim = cfg_.dim, nq = cfg_.nQuad;
+  const double *thPrev = theta_[th_];
+  double *thNew = theta_[th_ ^ 1];
+  lastRhsFull_ = !useSparseWeno;
+  check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
+  if (useSparseWeno) {
+    timed("linear", [&] {
+      return launch_mhd_linear(dim, nq, Uin, FS_, SRC_, WI_, grid_, faceW_, cellW_, stream_);
+    });
+    const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
+    timed("select", [&] {
+      return launch_mhd_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
+    });
+    timed("weno", [&] {
+      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceW_, derivW_, cellW_, list_, counter_,
+                             stream_);
+    });
+  } else {
+    timed("weno", [&] {
+      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceW_, derivW_, cellW_, nullptr, nullptr,
+                             stream_);
+    });
+    useSparseWeno = cfg_.wenoThresh > 0.0;
+  }
+  timed("pp", [&] { return launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thNew, grid_, stream_); });
+  th_ ^= 1;
+  check(hipMemsetAsync(chBits_, 0, 2 * sizeof(unsigned long long), stream_), "clear speeds");
+  timed("speed", [&] { return launch_mhd_speed(dim, nq, FS_, grid_, chBits_, stream_); });
+  timed("sources", [&] { return launch_mhd_sources(dim, nq, Uin, FS_, SRC_, thNew, grid_, diffM_, stream_); });
+  RKArgs a = a0;
+  a.chBits = chBits_;
+  a.src = SRC_;
+  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, a, stream_); });
+  nRhsEval++;
+}
+
 void Solver::pipeline(const double *Uin, const RKArgs &a) {
+  if (nv_ == 9) return pipelineMhd(Uin, a);
   const int dim = cfg_.dim, nq = cfg_.nQuad;
   const bool hydro = nv_ == 5;
   // theta of the previous RHS evaluation feeds the selector; this one writes the other buffer
@@ -481,6 +537,9 @@ void Solver::uploadWeights() {
   check(hipMemcpy(faceWT_, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(faceWS_, fs.data(), fs.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(derivWS_, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  if (cellW_)
+    check(hipMemcpy(cellW_, wts_.cell.data(), wts_.cell.size() * sizeof(double), hipMemcpyHostToDevice),
+          "copy weights");
 }
 void Solver::sync() const { check(hipStreamSynchronize(stream_), "sync"); }
 

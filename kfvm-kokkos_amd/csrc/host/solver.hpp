@@ -67,6 +67,7 @@ class Solver {
   void harvest();
 
   void pipeline(const double *Uin, const RKArgs &a);
+  void pipelineMhd(const double *Uin, const RKArgs &a);
   void uploadWeights();
   double readSpeed();
 
@@ -88,6 +89,10 @@ class Solver {
   int th_ = 0;                             // theta_[th_] holds the latest values
   int *list_ = nullptr, *counter_ = nullptr;  // WENO list; counter_[0] WENO, [1] limiter count
   int *ppList_ = nullptr;                      // cells for the limiter's exact sequence
+  // GLM-MHD: interior point states / sources, interior weights, differentiation
+  // matrix, cleaning-speed maxima
+  double *SRC_ = nullptr, *cellW_ = nullptr, *diffM_ = nullptr;
+  unsigned long long *chBits_ = nullptr;
   double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr, *faceWS_ = nullptr, *derivWS_ = nullptr;
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;
   unsigned long long *vmax_ = nullptr;

@@ -29,7 +29,8 @@ EXPORTS = [
     "kfvm_set_flags", "kfvm_get_time", "kfvm_set_time", "kfvm_get_info", "kfvm_get_counters",
     "kfvm_force_full_weno", "kfvm_get_weights", "kfvm_set_weights", "kfvm_profile_enable",
     "kfvm_profile_get", "kfvm_profile_reset", "kfvm_sync", "kfvm_weight_sizes",
-    "kfvm_build_weights", "kfvm_parse_init", "kfvm_last_error",
+    "kfvm_build_weights", "kfvm_build_cell_weights", "kfvm_diff_matrix", "kfvm_parse_init",
+    "kfvm_last_error",
 ]
 
 
@@ -81,6 +82,8 @@ def load_library(path=None):
         "kfvm_sync": [P],
         "kfvm_weight_sizes": [ctypes.c_int, ctypes.c_int, ctypes.POINTER(I64), ctypes.POINTER(I64)],
         "kfvm_build_weights": [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [D, I64, D, I64],
+        "kfvm_build_cell_weights": [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [D, I64],
+        "kfvm_diff_matrix": [ctypes.c_int, D, I64],
         "kfvm_parse_init": [ctypes.c_char_p, ctypes.c_char_p, I64],
     }
     for name, args in sig.items():
@@ -120,6 +123,24 @@ def build_weights(dim, nq, lfac, dx, dy, dz=1.0):
     _check(lib, lib.kfvm_build_weights(dim, nq, lfac, dx, dy, dz, _dptr(f), f.size, _dptr(d), d.size),
            "kfvm_build_weights")
     return f, d
+
+
+def build_cell_weights(dim, nq, lfac, dx, dy, dz=1.0):
+    """Interior-point GP weights [nsub, nq^dim, C] (point (a*nq+b)*nq+c)."""
+    lib = load_library()
+    nsub, C = (8, 33) if dim == 3 else (6, 13)
+    w = np.empty((nsub, nq ** dim, C))
+    _check(lib, lib.kfvm_build_cell_weights(dim, nq, lfac, dx, dy, dz, _dptr(w), w.size),
+           "kfvm_build_cell_weights")
+    return w
+
+
+def diff_matrix(nq):
+    """Lagrange differentiation matrix [nq, nq+2] on {-1/2, Gauss points, +1/2}."""
+    lib = load_library()
+    m = np.empty((nq, nq + 2))
+    _check(lib, lib.kfvm_diff_matrix(nq, _dptr(m), m.size), "kfvm_diff_matrix")
+    return m
 
 
 def parse_init(text):

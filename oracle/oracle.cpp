@@ -329,6 +329,137 @@ struct Hydro {
 };
 
 // ------------------------------------------------------------------ solver
+// ---- GLM-MHD physics (MHD_GLM_EOS.H:15-144, MHD_GLM_Riemann.H:18-231)
+struct Mhd {
+  static double ekin(const double *U) { return 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]; }
+  static double prsb(const double *U) { return 0.5 * (U[5] * U[5] + U[6] * U[6] + U[7] * U[7]); }
+  static double eint(const double *U) { return (U[4] - prsb(U) - ekin(U) - 0.5 * U[8] * U[8]); }
+  static double pres(double g, const double *U) { return (g - 1.0) * eint(U); }
+  static double sound(double g, const double *U) { return std::sqrt(g * pres(g, U) / U[0]); }
+  static double weno(double g, const double *U) { return pres(g, U) / std::pow(U[0], g); }
+  static double positivity(const double *U) { return std::fmin(U[0], eint(U)); }
+  static double fast(double g, const double *U, double bl) {
+    const double asq = g * pres(g, U) / U[0];
+    const double bsq = 2.0 * prsb(U) / U[0];
+    const double blsq = bl * bl / U[0];
+    const double absq = asq + bsq;
+    return std::sqrt(0.5 * (absq + std::sqrt(absq * absq - 4.0 * asq * blsq)));
+  }
+  static void cons2prim(double g, const double *U, double *V) {
+    V[0] = U[0];
+    V[1] = U[1] / U[0];
+    V[2] = U[2] / U[0];
+    V[3] = U[3] / U[0];
+    V[4] = pres(g, U);
+    for (int v = 5; v < 9; v++) V[v] = U[v];
+  }
+  static void flux(double ch, const int *ord, const double *U, const double *V, double *F) {
+    const int v0 = 1 + ord[0], v1 = 1 + ord[1], v2 = 1 + ord[2];
+    const int b0 = 5 + ord[0], b1 = 5 + ord[1], b2 = 5 + ord[2];
+    const double mx = U[v0];
+    const double vx = V[v0], vy = V[v1], vz = V[v2];
+    const double bx = U[b0], by = U[b1], bz = U[b2];
+    const double et = U[4], pg = V[4];
+    const double chpsi = ch * U[8], hpsisq = 0.5 * U[8] * U[8];
+    const double uDb = vx * bx + vy * by + vz * bz, bsq = bx * bx + by * by + bz * bz;
+    const double pt = pg + 0.5 * bsq;
+    F[0] = mx;
+    F[v0] = mx * vx + pt - bx * bx;
+    F[v1] = mx * vy - bx * by;
+    F[v2] = mx * vz - bx * bz;
+    F[4] = vx * (et + pt - hpsisq) + bx * (chpsi - uDb);
+    F[b0] = chpsi;
+    F[b1] = vx * by - vy * bx;
+    F[b2] = vx * bz - vz * bx;
+    F[8] = ch * bx;
+  }
+  static double speeds(double g, double ch, const int *ord, const double *uL, const double *uR,
+                       const double *vL, const double *vR, double &sL, double &sR) {
+    const int v0 = 1 + ord[0], b0 = 5 + ord[0], b1 = 5 + ord[1], b2 = 5 + ord[2];
+    const double cfL = fast(g, uL, uL[b0]);
+    const double cfR = fast(g, uR, uR[b0]);
+    const double sdL = std::fmin(vL[v0] - std::fmax(ch, cfL), vR[v0] - std::fmax(ch, cfR));
+    const double sdR = std::fmax(vL[v0] + std::fmax(ch, cfL), vR[v0] + std::fmax(ch, cfR));
+    const double rtL = std::sqrt(uL[0]), rtR = std::sqrt(uR[0]);
+    const double rtSum = rtL + rtR, vSum = (rtL * vL[v0] + rtR * vR[v0]) / rtSum;
+    const double bd[] = {uL[b0] - uR[b0], uL[b1] - uR[b1], uL[b2] - uR[b2]};
+    const double bnDiff = std::sqrt(bd[0] * bd[0] + bd[1] * bd[1] + bd[2] * bd[2]) / rtSum;
+    const double aL = std::fmin(vL[v0], vSum) - cfL - bnDiff;
+    const double aR = std::fmin(vR[v0], vSum) + cfR + bnDiff;  // fmin as in the reference
+    sL = std::fmin(sdL, aL);
+    sR = std::fmax(sdR, aR);
+    return std::fmax(std::fabs(sL), std::fabs(sR));
+  }
+  static double solve_hll(double g, double ch, bool llf, int dir, const double *uL, const double *uR,
+                          const double *vL, const double *vR, double *F) {
+    const int ord[3] = {dir, (dir + 1) % 3, (dir + 2) % 3};
+    double sL, sR;
+    const double sMax = speeds(g, ch, ord, uL, uR, vL, vR, sL, sR);
+    if (llf) {
+      sL = -sMax;
+      sR = sMax;
+    }
+    const double tR = (std::fmin(0.0, sR) - std::fmin(0.0, sL)) / (sR - sL), tL = 1.0 - tR;
+    const double tD = 0.5 * (sR * std::fabs(sL) - sL * std::fabs(sR)) / (sR - sL);
+    double FR[9];
+    flux(ch, ord, uL, vL, F);
+    flux(ch, ord, uR, vR, FR);
+    for (int v = 0; v < 9; v++) F[v] = tL * F[v] + tR * FR[v] - tD * (uR[v] - uL[v]);
+    return sMax;
+  }
+  static double logMean(double a, double b) {
+    const double zeta = a / b, f = (zeta - 1.0) / (zeta + 1.0), u = f * f;
+    const double F = u < 0.01 ? 1.0 + u / 3.0 + u * u / 5.0 + u * u * u / 7.0 : 0.5 * std::log(zeta) / f;
+    return 0.5 * (a + b) / F;
+  }
+  static double am(double a, double b) { return 0.5 * (a + b); }
+  static double solve_kepes(double g, double ch, int dir, const double *uL, const double *uR,
+                            const double *vL, const double *vR, double *F) {
+    const int ord[3] = {dir, (dir + 1) % 3, (dir + 2) % 3};
+    const int v0 = 1 + ord[0], v1 = 1 + ord[1], v2 = 1 + ord[2];
+    const int b0 = 5 + ord[0], b1 = 5 + ord[1], b2 = 5 + ord[2];
+    double sL, sR;
+    const double sMax = speeds(g, ch, ord, uL, uR, vL, vR, sL, sR);
+    const double rhoLn = logMean(uL[0], uR[0]);
+    const double rhoAv = am(uL[0], uR[0]);
+    const double uAv = am(vL[v0], vR[v0]), vAv = am(vL[v1], vR[v1]), wAv = am(vL[v2], vR[v2]);
+    const double uuAv = am(vL[v0] * vL[v0], vR[v0] * vR[v0]);
+    const double vvAv = am(vL[v1] * vL[v1], vR[v1] * vR[v1]);
+    const double wwAv = am(vL[v2] * vL[v2], vR[v2] * vR[v2]);
+    const double bxAv = am(uL[b0], uR[b0]), byAv = am(uL[b1], uR[b1]), bzAv = am(uL[b2], uR[b2]);
+    const double bxxAv = am(uL[b0] * uL[b0], uR[b0] * uR[b0]);
+    const double byyAv = am(uL[b1] * uL[b1], uR[b1] * uR[b1]);
+    const double bzzAv = am(uL[b2] * uL[b2], uR[b2] * uR[b2]);
+    const double uBxAv = am(vL[v0] * uL[b0], vR[v0] * uR[b0]);
+    const double vByAv = am(vL[v1] * uL[b1], vR[v1] * uR[b1]);
+    const double wBzAv = am(vL[v2] * uL[b2], vR[v2] * uR[b2]);
+    const double uBxxAv = am(vL[v0] * uL[b0] * uL[b0], vR[v0] * uR[b0] * uR[b0]);
+    const double uByyAv = am(vL[v0] * uL[b1] * uL[b1], vR[v0] * uR[b1] * uR[b1]);
+    const double uBzzAv = am(vL[v0] * uL[b2] * uL[b2], vR[v0] * uR[b2] * uR[b2]);
+    const double psiAv = am(uL[8], uR[8]);
+    const double bxPsiAv = am(uL[b0] * uL[8], uR[b0] * uR[8]);
+    const double betaL = uL[0] / (2.0 * vL[4]);
+    const double betaR = uR[0] / (2.0 * vR[4]);
+    const double betaLn = logMean(betaL, betaR);
+    const double betaAv = am(betaL, betaR);
+    const double pTil = 0.5 * rhoAv / betaAv;
+    const double pHatTot = pTil + 0.5 * (bxxAv + byyAv + bzzAv);
+    F[0] = rhoLn * uAv;
+    F[v0] = rhoLn * uAv * uAv + pHatTot - bxAv * bxAv;
+    F[v1] = rhoLn * uAv * vAv - bxAv * byAv;
+    F[v2] = rhoLn * uAv * wAv - bxAv * bzAv;
+    F[b0] = ch * psiAv;
+    F[b1] = uAv * byAv - vAv * bxAv;
+    F[b2] = uAv * bzAv - wAv * bxAv;
+    F[8] = ch * bxAv;
+    F[4] = (1.0 / (2.0 * (g - 1.0) * betaLn) - 0.5 * (uuAv + vvAv + wwAv)) * F[0] + uAv * F[v0] +
+           vAv * F[v1] + wAv * F[v2] + bxAv * F[b0] + byAv * F[b1] + bzAv * F[b2] + psiAv * F[8] -
+           0.5 * (uBxxAv + uByyAv + uBzzAv) + bxAv * (uBxAv + vByAv + wBzAv) - ch * bxPsiAv;
+    for (int v = 0; v < 9; v++) F[v] -= sMax * (uR[v] - uL[v]) / 2.0;
+    return sMax;
+  }
+};
+
 struct Base {
   virtual ~Base() {}
   virtual void set_state(const double *) = 0;
@@ -340,6 +471,7 @@ struct Base {
   virtual void set_flags(const double *) = 0;
   virtual void prime() = 0;
   virtual int take_step() = 0;
+  virtual void set_mhd_tables(const double *, const double *) {}
   orc_config cfg{};
   double time = 0.0, dt = 0.0, errEst = 1.0;
   int64_t nTS = 1, nRhsEval = 0, nWeno = 0;
@@ -359,6 +491,11 @@ struct Solver : Base {
   std::vector<double> fx, fy, fz;
   std::vector<char> wflag;  // cells needing WENO in sparse mode
   RKT rk;
+  // GLM-MHD: interior quadrature points (states, then sources) [cell][NI][NV],
+  // interior weights [nSub][NI][C], differentiation matrix [NQ][NQ+2], c_h
+  int NI = 0, NZ = 1;
+  std::vector<double> src, cellW, Dm;
+  double ch = 0.0;
 
   explicit Solver(const orc_config &c, const double *fW, const double *dW) : st(D) {
     cfg = c;
@@ -407,6 +544,9 @@ struct Solver : Base {
     fy.assign(size_t(nx) * (ny + 1) * nz * 2 * Q * NV, 0.0);
     if (D == 3) fz.assign(size_t(nx) * ny * (nz + 1) * 2 * Q * NV, 0.0);
     wflag.assign(nc, 0);
+    NZ = D == 3 ? NQ : 1;
+    NI = NQ * NQ * NZ;
+    if (NV == 9) src.assign(nc * NI * NV, 0.0);
     rk = rk_coeffs(c.rk);
     useSparseWeno = c.wenoThresh > 0.0;
   }
@@ -442,11 +582,21 @@ struct Solver : Base {
 
   // ---- physics dispatch
   static constexpr bool hydro = NV == 5;
+  static constexpr bool mhd = NV == 9;
   double gam() const { return cfg.gamma; }
+  static double eintP(const double *s) { return mhd ? Mhd::eint(s) : Hydro::eint(s); }
+  double soundP(const double *s) const { return mhd ? Mhd::sound(gam(), s) : Hydro::sound(gam(), s); }
+  double wenoP(const double *s) const { return mhd ? Mhd::weno(gam(), s) : Hydro::weno(gam(), s); }
+  double *SP(int64_t i, int64_t j, int64_t k, int p) { return &src[(ci(i, j, k) * NI + p) * NV]; }
+  void set_mhd_tables(const double *cw, const double *dm) override {
+    cellW.assign(cw, cw + size_t(st.nSub) * NI * st.C);
+    Dm.assign(dm, dm + size_t(NQ) * (NQ + 2));
+  }
 
   // reflecting / noslip sign of variable v for axis a
   double bc_sign(int b, int a, int v) const {
-    if (!hydro) return 1.0;
+    if (!hydro && !mhd) return 1.0;
+    if (mhd && v == 5 + a) return -1.0;                   // normal field (MHD_GLM_BCCoeffs.H)
     if (b == 3) return (v >= 1 && v <= 3) ? -1.0 : 1.0;  // noslip
     return (v == 1 + a) ? -1.0 : 1.0;                     // reflecting
   }
@@ -597,6 +747,21 @@ struct Solver : Base {
                 o[v] = acc;
               }
             }
+    if (mhd) {  // interior points of the source terms (Stencil_K.H:92-118)
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+          for (int64_t k = 0; k < nz; k++)
+            for (int p = 0; p < NI; p++) {
+              const double *w = &cellW[size_t(p) * st.C];
+              double *o = SP(i, j, k, p);
+              for (int v = 0; v < NV; v++) {
+                double acc = 0.0;
+                for (int c = 0; c < st.C; c++) acc = std::fma(w[c], cellp(V, i, j, k, c)[v], acc);
+                o[v] = acc;
+              }
+            }
+    }
   }
 
   // ---- KFVM-WENO for one cell (KernelWenoRecon_K)
@@ -605,7 +770,7 @@ struct Solver : Base {
     double Vf[33][NV];
     const double *ref = &V[uc(i, j, k)];
     // linearised-primitive context about the centre cell (Hydro_K.H:25-65)
-    double gamm = 0, rho = 1, u = 0, vv = 0, w = 0, vsq = 0;
+    double gamm = 0, rho = 1, u = 0, vv = 0, w = 0, vsq = 0, bx = 0, by = 0, bz = 0, rPsi = 0;
     if (hydro) {
       gamm = gam() - 1.0;
       rho = ref[0];
@@ -614,11 +779,30 @@ struct Solver : Base {
       w = ref[3] / rho;
       vsq = 0.5 * (u * u + vv * vv + w * w);
     }
+    if (mhd) {  // VarConvert<MHD_GLM> (MHD_GLM_K.H:24-69)
+      gamm = gam() - 1.0;
+      rho = ref[0];
+      u = ref[1] / rho;
+      vv = ref[2] / rho;
+      w = ref[3] / rho;
+      bx = ref[5];
+      by = ref[6];
+      bz = ref[7];
+      rPsi = ref[8];
+      vsq = (u * u + vv * vv + w * w) / 2.0;
+    }
     for (int c = 0; c < C; c++) {
       const double *s = cellp(V, i, j, k, c);
       for (int v = 0; v < NV; v++) Vf[c][v] = s[v];
       if (hydro) {
         Vf[c][4] = gamm * (vsq * Vf[c][0] - u * Vf[c][1] - vv * Vf[c][2] - w * Vf[c][3] + Vf[c][4]);
+        Vf[c][1] = (Vf[c][1] - u * Vf[c][0]) / rho;
+        Vf[c][2] = (Vf[c][2] - vv * Vf[c][0]) / rho;
+        Vf[c][3] = (Vf[c][3] - w * Vf[c][0]) / rho;
+      }
+      if (mhd) {
+        Vf[c][4] = gamm * (vsq * Vf[c][0] + Vf[c][4] - u * Vf[c][1] - vv * Vf[c][2] - w * Vf[c][3] -
+                           bx * Vf[c][5] - by * Vf[c][6] - bz * Vf[c][7] - rPsi * Vf[c][8]);
         Vf[c][1] = (Vf[c][1] - u * Vf[c][0]) / rho;
         Vf[c][2] = (Vf[c][2] - vv * Vf[c][0]) / rho;
         Vf[c][3] = (Vf[c][3] - w * Vf[c][0]) / rho;
@@ -681,9 +865,42 @@ struct Solver : Base {
           o[2] = (rho * o[2] + vv * o[0]);
           o[3] = (rho * o[3] + w * o[0]);
         }
+        if (mhd) lin2cons(o, gamm, rho, u, vv, w, bx, by, bz, rPsi, vsq);
         double *dst = own(f, i, j, k, q);
         for (int v = 0; v < NV; v++) dst[v] = o[v];
       }
+    if (mhd) {  // fillInterior (Stencil_K.H:456-504)
+      for (int p = 0; p < NI; p++) {
+        double o[NV];
+        for (int v = 0; v < NV; v++) {
+          double acc = 0.0;
+          for (int s = 0; s < nSub; s++) {
+            const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
+            const double *wq = &cellW[(size_t(s) * NI + p) * C];
+            double t = 0.0;
+            for (int c = 0; c < n; c++) {
+              const int cc = s < 2 ? c : st.sub[s - 2][c];
+              t = std::fma(wq[c], Vf[cc][v], t);
+            }
+            acc = std::fma(om[s][v], t, acc);
+          }
+          o[v] = acc;
+        }
+        lin2cons(o, gamm, rho, u, vv, w, bx, by, bz, rPsi, vsq);
+        double *dst = SP(i, j, k, p);
+        for (int v = 0; v < NV; v++) dst[v] = o[v];
+      }
+    }
+  }
+
+  // linPrim2Cons of VarConvert<MHD_GLM>
+  static void lin2cons(double *o, double gamm, double rho, double u, double vv, double w, double bx,
+                       double by, double bz, double rPsi, double vsq) {
+    o[4] = (vsq * o[0] + rho * u * o[1] + rho * vv * o[2] + rho * w * o[3] + bx * o[5] + by * o[6] +
+            bz * o[7] + o[4] / gamm + rPsi * o[8]);
+    o[1] = (rho * o[1] + u * o[0]);
+    o[2] = (rho * o[2] + vv * o[0]);
+    o[3] = (rho * o[3] + w * o[0]);
   }
 
   // ---- KXRCF selector (WenoSelect_K); returns number flagged
@@ -699,8 +916,8 @@ struct Solver : Base {
             for (int nQ = 0; nQ < NQ; nQ++) {
               const int q = nR * NQ + nQ;
               auto jump = [&](const double *in, const double *out) {
-                if (!hydro) return 0.0;
-                return std::fabs(Hydro::weno(gam(), out) - Hydro::weno(gam(), in));
+                if (!hydro && !mhd) return 0.0;
+                return std::fabs(wenoP(out) - wenoP(in));
               };
               maxJump = std::fmax(maxJump, jump(FX(i, j, k, 1, q), FX(i, j, k, 0, q)));
               maxJump = std::fmax(maxJump, jump(FX(i + 1, j, k, 0, q), FX(i + 1, j, k, 1, q)));
@@ -712,7 +929,7 @@ struct Solver : Base {
               }
             }
           // weno() of LinAdv is 0: the indicator is 0/0 and never fires
-          const double wc = hydro ? Hydro::weno(gam(), &V[uc(i, j, k)]) : 0.0;
+          const double wc = (hydro || mhd) ? wenoP(&V[uc(i, j, k)]) : 0.0;
           const double ind = maxJump / wc;
           const size_t c = ci(i, j, k);
           if (ind >= wThresh || flag[2 * c + 1] < 1.0) {
@@ -729,8 +946,7 @@ struct Solver : Base {
 
   // ---- positivity-preserving limiter (PositivityPreserve_K)
   void pos_pres(const std::vector<double> &V) {
-    if (!hydro) return;
-    const double g = gam();
+    if (!hydro && !mhd) return;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int64_t i = 0; i < nx; i++)
       for (int64_t j = 0; j < ny; j++)
@@ -749,8 +965,8 @@ struct Solver : Base {
                                          : &V[ui(i + r + xo, j + r + yo, 0)];
                 maxDens = std::fmax(maxDens, s[0]);
                 minDens = std::fmin(minDens, s[0]);
-                const double ei = Hydro::eint(s);
-                const double sd = Hydro::sound(g, s);
+                const double ei = eintP(s);
+                const double sd = soundP(s);
                 minEint = std::fmin(minEint, ei);
                 minSdSp = std::fmin(minSdSp, sd);
               }
@@ -779,6 +995,11 @@ struct Solver : Base {
               thD = rho > minDens ? thD : std::fmin(thD, dM / (rDens - rho));
               thD = rho < maxDens ? thD : std::fmin(thD, dP / (rho - rDens));
             }
+          for (int p = 0; p < (mhd ? NI : 0); p++) {
+            const double rho = SP(i, j, k, p)[0];
+            thD = rho > minDens ? thD : std::fmin(thD, dM / (rDens - rho));
+            thD = rho < maxDens ? thD : std::fmin(thD, dP / (rho - rDens));
+          }
           if (thD > 0.0 && thD < 1.0) mix(i, j, k, ref, thD);
           else if (thD < 1.0) replace(i, j, k, ref);
           // internal energy theta
@@ -786,9 +1007,14 @@ struct Solver : Base {
           for (int q = 0; q < Q; q++)
             for (int f = 0; f < F; f++) {
               const double *s = own(f, i, j, k, q);
-              const double ei = Hydro::eint(s);
+              const double ei = eintP(s);
               thP = ei > minEint ? thP : std::fmin(thP, eint_pos(minEint, ref, s));
             }
+          for (int p = 0; p < (mhd ? NI : 0); p++) {
+            const double *s = SP(i, j, k, p);
+            const double ei = eintP(s);
+            thP = ei > minEint ? thP : std::fmin(thP, eint_pos(minEint, ref, s));
+          }
           if (thP > 0.0 && thP < 1.0) mix(i, j, k, ref, thP);
           else if (thP < 1.0) replace(i, j, k, ref);
           flag[2 * ci(i, j, k) + 1] = std::fmin(thD, thP);
@@ -800,6 +1026,10 @@ struct Solver : Base {
         double *s = own(f, i, j, k, q);
         for (int v = 0; v < NV; v++) s[v] = ref[v] + th * (s[v] - ref[v]);
       }
+    for (int p = 0; p < (mhd ? NI : 0); p++) {
+      double *s = SP(i, j, k, p);
+      for (int v = 0; v < NV; v++) s[v] = ref[v] + th * (s[v] - ref[v]);
+    }
   }
   void replace(int64_t i, int64_t j, int64_t k, const double *ref) {
     for (int q = 0; q < Q; q++)
@@ -807,14 +1037,18 @@ struct Solver : Base {
         double *s = own(f, i, j, k, q);
         for (int v = 0; v < NV; v++) s[v] = ref[v];
       }
+    for (int p = 0; p < (mhd ? NI : 0); p++) {
+      double *s = SP(i, j, k, p);
+      for (int v = 0; v < NV; v++) s[v] = ref[v];
+    }
   }
   static double eint_pos(double emin, const double *ref, const double *s) {
     double a = 0.0, b = 1.0, c;
-    double st_[5];
+    double st_[NV];
     for (int it = 0; it < 10; it++) {
       c = 0.5 * (a + b);
-      for (int v = 0; v < 5; v++) st_[v] = ref[v] + c * (s[v] - ref[v]);
-      const double dec = Hydro::eint(st_) - emin;
+      for (int v = 0; v < NV; v++) st_[v] = ref[v] + c * (s[v] - ref[v]);
+      const double dec = eintP(st_) - emin;
       if (dec > 0.0) a = c;
       else b = c;
     }
@@ -829,7 +1063,13 @@ struct Solver : Base {
       uR[v] = sR_[v];
     }
     double s = 0.0;
-    if (hydro) {
+    if (mhd) {
+      double vL[9], vR[9];
+      Mhd::cons2prim(gam(), uL, vL);
+      Mhd::cons2prim(gam(), uR, vR);
+      if (cfg.rs == 4) s = Mhd::solve_kepes(gam(), ch, dir, uL, uR, vL, vR, Fl);
+      else s = Mhd::solve_hll(gam(), ch, cfg.rs == 0, dir, uL, uR, vL, vR, Fl);
+    } else if (hydro) {
       double vL[5], vR[5];
       Hydro::cons2prim(gam(), uL, vL);
       Hydro::cons2prim(gam(), uR, vR);
@@ -908,6 +1148,116 @@ struct Solver : Base {
                   rhs[v] += qWt * (FZ(i, j, k, 0, q)[v] - FZ(i, j, k + 1, 1, q)[v]) / dz;
                 }
           }
+          if (mhd) {  // source terms at the interior points
+            for (int a = 0; a < NQ; a++)
+              for (int b = 0; b < NQ; b++)
+                for (int c = 0; c < NZ; c++) {
+                  const double *S = SP(i, j, k, (a * NQ + b) * NZ + c);
+                  const double w = D == 3 ? wt[a] * wt[b] * wt[c] : wt[a] * wt[b];
+                  for (int v = 0; v < NV; v++) rhs[v] += w * S[v];
+                }
+          }
+        }
+  }
+
+  // ---- GLM cleaning speed (SpeedEstimate_K, MHD_GLM_K.H:213-310; Solver.cpp:456-465)
+  double mhd_speed() {
+    double maxLam = 0.0, maxVel = 0.0;
+#pragma omp parallel for collapse(2) reduction(max : maxLam, maxVel)
+    for (int64_t i = 0; i < nx; i++)
+      for (int64_t j = 0; j < ny; j++)
+        for (int64_t k = 0; k < nz; k++)
+          for (int f = 0; f < F; f++)
+            for (int q = 0; q < Q; q++) {
+              const double *s = own(f, i, j, k, q);
+              const double vel = std::fabs(s[1] / s[0]);  // x-components on every face
+              const double lam = Mhd::fast(gam(), s, s[5]) + vel;
+              maxVel = vel > maxVel ? vel : maxVel;
+              maxLam = lam > maxLam ? lam : maxLam;
+            }
+    return maxLam - maxVel;
+  }
+
+  // ---- GLM source terms (SourceTerms_K, MHD_GLM_K.H:316-547)
+  void mhd_sources(const std::vector<double> &V) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t i = 0; i < nx; i++)
+      for (int64_t j = 0; j < ny; j++)
+        for (int64_t k = 0; k < nz; k++) {
+          double divB = 0.0;
+          for (int nQ = 0; nQ < NQ; nQ++)
+            for (int nS = 0; nS < NZ; nS++) {
+              const int q = nS * NQ + nQ;
+              const double twt = D == 2 ? wt[nQ] : wt[nQ] * wt[nS];
+              const double bAvW = 0.5 * (FX(i, j, k, 1, q)[5] + FX(i, j, k, 0, q)[5]);
+              const double bAvE = 0.5 * (FX(i + 1, j, k, 0, q)[5] + FX(i + 1, j, k, 1, q)[5]);
+              divB += twt * (bAvE - bAvW) / dx;
+              const double bAvS = 0.5 * (FY(i, j, k, 1, q)[6] + FY(i, j, k, 0, q)[6]);
+              const double bAvN = 0.5 * (FY(i, j + 1, k, 0, q)[6] + FY(i, j + 1, k, 1, q)[6]);
+              divB += twt * (bAvN - bAvS) / dy;
+              if (D == 3) {
+                const double bAvB = 0.5 * (FZ(i, j, k, 1, q)[7] + FZ(i, j, k, 0, q)[7]);
+                const double bAvT = 0.5 * (FZ(i, j, k + 1, 0, q)[7] + FZ(i, j, k + 1, 1, q)[7]);
+                divB += twt * (bAvT - bAvB) / dz;
+              }
+            }
+          auto at = [&](int xo, int yo, int zo) -> const double * {
+            return D == 3 ? &V[ui(i + r + xo, j + r + yo, k + r + zo)] : &V[ui(i + r + xo, j + r + yo, 0)];
+          };
+          const bool fdiff = flag[2 * ci(i, j, k) + 1] < 0.75;
+          auto dmat = [&](int a, int b) { return Dm[size_t(a) * (NQ + 2) + b]; };
+          double gpsiDu[27];
+          for (int nQ = 0; nQ < NQ; nQ++)
+            for (int nR = 0; nR < NQ; nR++)
+              for (int nS = 0; nS < NZ; nS++) {
+                const int p = (nQ * NQ + nR) * NZ + nS;
+                const double *cv = SP(i, j, k, p);
+                const double rho = cv[0];
+                const double vx = cv[1] / rho, vy = cv[2] / rho, vz = cv[3] / rho;
+                double dpx, dpy, dpz = 0.0;
+                if (fdiff) {
+                  const double tdx = 1.0 / (2.0 * dx), tdy = 1.0 / (2.0 * dy);
+                  dpx = tdx * (at(1, 0, 0)[8] - at(-1, 0, 0)[8]);
+                  dpy = tdy * (at(0, 1, 0)[8] - at(0, -1, 0)[8]);
+                  if (D == 3) {
+                    const double tdz = 1.0 / (2.0 * dz);
+                    dpz = tdz * (at(0, 0, 1)[8] - at(0, 0, -1)[8]);
+                  }
+                } else {
+                  const int qx = D == 2 ? nR : nR * NQ + nS, qy = D == 2 ? nQ : nQ * NQ + nS;
+                  dpx = dmat(nQ, 0) * own(0, i, j, k, qx)[8] + dmat(nQ, NQ + 1) * own(1, i, j, k, qx)[8];
+                  for (int n = 1; n <= NQ; n++) dpx += dmat(nQ, n) * SP(i, j, k, ((n - 1) * NQ + nR) * NZ + nS)[8];
+                  dpx /= dx;
+                  dpy = dmat(nR, 0) * own(2, i, j, k, qy)[8] + dmat(nR, NQ + 1) * own(3, i, j, k, qy)[8];
+                  for (int n = 1; n <= NQ; n++) dpy += dmat(nR, n) * SP(i, j, k, (nQ * NQ + n - 1) * NZ + nS)[8];
+                  dpy /= dy;
+                  if (D == 3) {
+                    const int qz = nQ * NQ + nR;
+                    dpz = dmat(nS, 0) * own(4, i, j, k, qz)[8] + dmat(nS, NQ + 1) * own(5, i, j, k, qz)[8];
+                    for (int n = 1; n <= NQ; n++) dpz += dmat(nS, n) * SP(i, j, k, (nQ * NQ + nR) * NZ + n - 1)[8];
+                    dpz /= dz;
+                  }
+                }
+                gpsiDu[p] = vx * dpx + vy * dpy + vz * dpz;
+              }
+          for (int p = 0; p < NI; p++) {
+            double *S = SP(i, j, k, p);
+            double stt[9];
+            for (int v = 0; v < 9; v++) stt[v] = S[v];
+            const double vx = stt[1] / stt[0], vy = stt[2] / stt[0], vz = stt[3] / stt[0];
+            const double uDb = vx * stt[5] + vy * stt[6] + vz * stt[7];
+            const double f0 = 0.0;
+            S[0] = f0;
+            S[1] = f0 - divB * stt[5];
+            S[2] = f0 - divB * stt[6];
+            S[3] = f0 - divB * stt[7];
+            const double eForce = divB * uDb + gpsiDu[p] * stt[8];
+            S[4] = f0 - eForce;
+            S[5] = f0 - divB * vx;
+            S[6] = f0 - divB * vy;
+            S[7] = f0 - divB * vz;
+            S[8] = -gpsiDu[p];
+          }
         }
   }
 
@@ -936,6 +1286,10 @@ struct Solver : Base {
     }
     pos_pres(V);
     face_bcs();
+    if (mhd) {
+      ch = mhd_speed();
+      mhd_sources(V);
+    }
     const double vmax = riemann_all();
     integrate_rhs();
     return vmax;
@@ -980,6 +1334,8 @@ struct Solver : Base {
   }
   int set_ic(const char *name, const double *user, uint64_t seed) override {
     const int nQZ = D == 3 ? NQ : 1;
+    const double mid[3] = {0.5 * (cfg.lo[0] + cfg.hi[0]), 0.5 * (cfg.lo[1] + cfg.hi[1]),
+                           0.5 * (cfg.lo[2] + cfg.hi[2])};
     int ok = 1;
     for (int64_t i = 0; i < nx; i++)
       for (int64_t j = 0; j < ny; j++)
@@ -996,7 +1352,7 @@ struct Solver : Base {
                 const double px = ab[a] * dx + xc, py = ab[b] * dy + yc;
                 const double pz = D == 3 ? ab[c] * dz + zc : cfg.lo[2];
                 double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                if (!point_ic(name, D, gam(), user, seed, cell, px, py, pz, s)) ok = 0;
+                if (!point_ic(name, D, gam(), user, seed, cell, mid, px, py, pz, s)) ok = 0;
                 const double w = D == 3 ? wt[a] * wt[b] * wt[c] : wt[a] * wt[b];
                 for (int v = 0; v < NV; v++) acc[v] += w * s[v];
               }
@@ -1082,7 +1438,7 @@ struct Solver : Base {
               hasNan = hasNan || std::isnan(u);
               errSum += std::pow((u - uh) / (cfg.atol + cfg.rtol * std::fmax(std::fabs(u), std::fabs(uh))), 2);
             }
-            const double pos = hasNan ? -1.0 : (hydro ? Hydro::positivity(st_) : 1.0);
+            const double pos = hasNan ? -1.0 : (hydro ? Hydro::positivity(st_) : (mhd ? Mhd::positivity(st_) : 1.0));
             posFlag = posFlag > pos ? pos : posFlag;
           }
       const double nDofs = double(nc) * NV;
@@ -1148,6 +1504,9 @@ orc_solver *orc_create(const orc_config *cfg, const double *faceW, const double 
   } else if (cfg->eq == 1) {
     if (cfg->dim == 2) b = new orc::Solver<2, 1>(*cfg, faceW, derivW);
     else b = new orc::Solver<3, 1>(*cfg, faceW, derivW);
+  } else if (cfg->eq == 2) {
+    if (cfg->dim == 2) b = new orc::Solver<2, 9>(*cfg, faceW, derivW);
+    else b = new orc::Solver<3, 9>(*cfg, faceW, derivW);
   } else {
     return nullptr;
   }
@@ -1183,5 +1542,23 @@ int64_t orc_rhs_evals(const orc_solver *s) { return s->impl->nRhsEval; }
 void orc_force_full_weno(orc_solver *s) { s->impl->useSparseWeno = false; }
 int64_t orc_num_weno(const orc_solver *s) { return s->impl->nWeno; }
 int orc_last_step(const orc_solver *s) { return s->impl->lastTimeStep ? 1 : 0; }
+double orc_riemann_point(int eq, int rs, double gamma, double ch, int dir, const double *uL,
+                         const double *uR, double *F) {
+  if (eq == 2) {
+    double vL[9], vR[9];
+    orc::Mhd::cons2prim(gamma, uL, vL);
+    orc::Mhd::cons2prim(gamma, uR, vR);
+    if (rs == 4) return orc::Mhd::solve_kepes(gamma, ch, dir, uL, uR, vL, vR, F);
+    return orc::Mhd::solve_hll(gamma, ch, rs == 0, dir, uL, uR, vL, vR, F);
+  }
+  double vL[5], vR[5];
+  orc::Hydro::cons2prim(gamma, uL, vL);
+  orc::Hydro::cons2prim(gamma, uR, vR);
+  if (rs == 2) return orc::Hydro::solve_hllc(gamma, dir, uL, uR, vL, vR, F);
+  return orc::Hydro::solve_hll(gamma, rs == 0, dir, uL, uR, vL, vR, F);
+}
+void orc_set_mhd_tables(orc_solver *s, const double *cellW, const double *Dm) {
+  s->impl->set_mhd_tables(cellW, Dm);
+}
 
 }  // extern "C"

@@ -19,8 +19,8 @@ extern "C" {
 typedef struct orc_config {
   int dim;        /* 2 or 3 */
   int nq;         /* face Gauss points per direction: 2 or 3 */
-  int eq;         /* 0 hydro, 1 linadv */
-  int rs;         /* 0 LLF, 1 HLL, 2 HLLC, 3 exact (linadv) */
+  int eq;         /* 0 hydro, 1 linadv, 2 GLM-MHD */
+  int rs;         /* 0 LLF, 1 HLL, 2 HLLC, 3 exact (linadv), 4 KEPES (MHD) */
   int rk;         /* 0 ThreeStarP, 1 FourStarP, 2 SSP4_3_2, 3 SSP10_4_3 */
   int bc[6];      /* 0 periodic, 1 outflow, 2 reflecting, 3 noslip */
   int64_t n[3];   /* cells per direction (n[2] = 1 in 2D) */
@@ -62,6 +62,15 @@ int64_t orc_rhs_evals(const orc_solver *s);
 /* Force the next RHS to full WENO (as after an unphysical rejection). */
 void orc_force_full_weno(orc_solver *s);
 int64_t orc_num_weno(const orc_solver *s);
+int orc_last_step(const orc_solver *s);
+/* GLM-MHD tables: interior-point weights [nSub][NQ^D][C] and the
+ * differentiation matrix [NQ][NQ+2]; required before the first RHS. */
+void orc_set_mhd_tables(orc_solver *s, const double *cellW, const double *Dm);
+/* One face-point Riemann solve in direction dir (0 x, 1 y, 2 z): eq 0 hydro
+ * (rs 0 LLF, 1 HLL, 2 HLLC) or 2 GLM-MHD (rs 0 LLF, 1 HLL, 4 KEPES, cleaning
+ * speed ch).  Writes the flux to F and returns the maximum wave speed. */
+double orc_riemann_point(int eq, int rs, double gamma, double ch, int dir, const double *uL,
+                         const double *uR, double *F);
 
 #ifdef __cplusplus
 }

@@ -11,8 +11,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "build", "liborc.so")
 
-EQ = {"hydro": 0, "linadv": 1}
-RS = {"llf": 0, "hll": 1, "hllc": 2, "exact": 3}
+EQ = {"hydro": 0, "linadv": 1, "mhd_glm": 2}
+RS = {"llf": 0, "hll": 1, "hllc": 2, "exact": 3, "kepes": 4}
 RK = {"threestarp": 0, "fourstarp": 1, "ssp4_3_2": 2, "ssp10_4_3": 3}
 BC = {"periodic": 0, "outflow": 1, "reflecting": 2, "noslip": 3}
 
@@ -74,12 +74,27 @@ def lib():
         L.orc_num_weno.restype = I64
         L.orc_last_step.argtypes = [P]
         L.orc_last_step.restype = ctypes.c_int
+        L.orc_set_mhd_tables.argtypes = [P, D, D]
+        L.orc_set_mhd_tables.restype = None
+        L.orc_riemann_point.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, D, D, D]
+        L.orc_riemann_point.restype = ctypes.c_double
         _lib = L
     return _lib
 
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def riemann_point(eq, rs, gamma, ch, direction, uL, uR):
+    """One face-point Riemann solve: (flux, max wave speed)."""
+    L = lib()
+    a = np.ascontiguousarray(uL, dtype=np.float64)
+    b = np.ascontiguousarray(uR, dtype=np.float64)
+    F = np.zeros_like(a)
+    s = L.orc_riemann_point(EQ[eq], RS[rs], gamma, ch, direction, _dp(a), _dp(b), _dp(F))
+    return F, s
 
 
 class Oracle:
@@ -89,7 +104,7 @@ class Oracle:
                  bc="outflow", lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), gamma=1.4,
                  weno_thresh=1.0, adv_vel=None, cfl=None, final_time=1.0, min_dt=1e-10,
                  atol=1e-4, rtol=1e-4, rejection_thresh=0.8, rejection_limit=4, max_steps=1000,
-                 fixed_dt=-1.0, threads=0):
+                 fixed_dt=-1.0, threads=0, cell_w=None, diff_m=None):
         self.L = lib()
         c = OrcConfig()
         c.dim, c.nq, c.eq, c.rs, c.rk = dim, nq, EQ[eq], RS[rs], RK[rk]
@@ -110,13 +125,22 @@ class Oracle:
         c.rejectionThresh, c.fixedDt = rejection_thresh, fixed_dt
         c.rejectionLimit, c.maxSteps, c.nThreads = rejection_limit, max_steps, threads
         self.cfg = c
-        self.nv = 5 if eq == "hydro" else 1
+        self.nv = {"hydro": 5, "linadv": 1, "mhd_glm": 9}[eq]
         self.shape = (self.nv, nn[2] if dim == 3 else 1, nn[1], nn[0])
         self._fw = np.ascontiguousarray(face_w, dtype=np.float64)
         self._dw = np.ascontiguousarray(deriv_w, dtype=np.float64)
         self.h = self.L.orc_create(ctypes.byref(c), _dp(self._fw), _dp(self._dw))
         if not self.h:
             raise RuntimeError("orc_create failed")
+        if eq == "mhd_glm":
+            # interior-point weights [nsub][nq^dim][C] and the [nq][nq+2] differentiation matrix
+            if cell_w is None or diff_m is None:
+                raise ValueError("mhd_glm needs cell_w and diff_m")
+            self._cw = np.ascontiguousarray(cell_w, dtype=np.float64)
+            self._dm = np.ascontiguousarray(diff_m, dtype=np.float64)
+            nsub, C = (8, 33) if dim == 3 else (6, 13)
+            assert self._cw.size == nsub * nq ** dim * C and self._dm.size == nq * (nq + 2)
+            self.L.orc_set_mhd_tables(self.h, _dp(self._cw), _dp(self._dm))
 
     def __del__(self):
         if getattr(self, "h", None):

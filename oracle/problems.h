@@ -35,11 +35,66 @@ inline void hydro_prim2cons(double gamma, double rho, double u, double v, double
   s[4] = p / gamm + rho * (u * u + v * v + w * w) / 2.0;
 }
 
+/* GLM-MHD conserved state (rho, mx, my, mz, E, Bx, By, Bz, psi = 0). */
+inline void mhd_prim2cons(double gamma, double rho, double u, double v, double w, double p,
+                          double bx, double by, double bz, double *s) {
+  s[0] = rho;
+  s[1] = rho * u;
+  s[2] = rho * v;
+  s[3] = rho * w;
+  s[4] = p / (gamma - 1.0) + rho * (u * u + v * v + w * w) / 2.0 + (bx * bx + by * by + bz * bz) / 2.0;
+  s[5] = bx;
+  s[6] = by;
+  s[7] = bz;
+  s[8] = 0.0;
+}
+
 /* Returns false for an unknown problem.  cell = global interior cell index
- * (used by problems with per-cell random data). */
+ * (used by problems with per-cell random data); mid = centre of the box. */
 inline bool point_ic(const char *name, int dim, double gamma, const double *user, uint64_t seed,
-                     uint64_t cell, double x, double y, double z, double *s) {
-  (void)dim;
+                     uint64_t cell, const double *mid, double x, double y, double z, double *s) {
+  const double tp = 2.0 * M_PI;
+  if (!std::strcmp(name, "mhd_vortex3d")) {
+    /* rho = gamma^2, p = gamma, v = (-sin 2pi y, sin 2pi x, 0.1 sin 2pi z),
+       B = (-sin 2pi y, sin 4pi x, 0) / sqrt(4 pi) */
+    const double b0 = 1.0 / std::sqrt(4.0 * M_PI);
+    mhd_prim2cons(gamma, gamma * gamma, -std::sin(tp * y), std::sin(tp * x), 0.1 * std::sin(tp * z), gamma,
+                  -std::sin(tp * y) * b0, std::sin(2.0 * tp * x) * b0, 0.0, s);
+    return true;
+  }
+  if (!std::strcmp(name, "orszag_tang")) {
+    const double e = 0.2;
+    mhd_prim2cons(gamma, gamma * gamma, -(1.0 + e * std::sin(tp * z)) * std::sin(tp * y),
+                  (1.0 + e * std::sin(tp * z)) * std::sin(tp * x), e * std::sin(tp * z), gamma,
+                  -std::sin(tp * y), std::sin(2.0 * tp * x), 0.0, s);
+    return true;
+  }
+  if (!std::strcmp(name, "circ_alfven")) {
+    /* wave along x1 = x cos(a) cos(b) + y cos(a) sin(b) + z sin(a),
+       sin a = 2/3, sin b = 2/sqrt 5 */
+    const double sa = 2.0 / 3.0, sb = 2.0 / std::sqrt(5.0), ca = std::sqrt(5.0) / 3.0, cb = 1.0 / std::sqrt(5.0);
+    const double x1 = x * ca * cb + y * ca * sb + z * sa;
+    const double par = 0.0, p2 = 0.1 * std::sin(tp * x1), p3 = 0.1 * std::cos(tp * x1);
+    /* rotate (parallel, perp2, perp3) back to (x, y, z) for v and for B = (1, p2, p3) */
+    const double u = par * ca * cb - p2 * sb - p3 * sa * cb;
+    const double v = par * ca * sb + p2 * cb - p3 * sa * sb;
+    const double w = par * sa + p3 * ca;
+    const double bx = 1.0 * ca * cb - p2 * sb - p3 * sa * cb;
+    const double by = 1.0 * ca * sb + p2 * cb - p3 * sa * sb;
+    const double bz = 1.0 * sa + p3 * ca;
+    mhd_prim2cons(gamma, 1.0, u, v, w, 0.1, bx, by, bz, s);
+    return true;
+  }
+  if (!std::strcmp(name, "mhd_uniform")) {
+    mhd_prim2cons(gamma, user[0], user[1], user[2], user[3], user[4], user[5], user[6], user[7], s);
+    return true;
+  }
+  if (!std::strcmp(name, "mhd_blast")) {
+    const double zc = dim == 3 ? mid[2] : z;
+    const double r2 = (x - mid[0]) * (x - mid[0]) + (y - mid[1]) * (y - mid[1]) + (z - zc) * (z - zc);
+    mhd_prim2cons(gamma, 1.0, 0.0, 0.0, 0.0, r2 < user[0] * user[0] ? 10.0 : 0.1, user[1], user[2], 0.0, s);
+    return true;
+  }
   if (!std::strcmp(name, "linadv_sine")) {
     s[0] = std::sin(2.0 * M_PI * (user[0] * x + user[1] * y + user[2] * z));
     return true;

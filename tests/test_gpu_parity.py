@@ -62,6 +62,28 @@ CASES = {
     "linadv3d_sparse": config(n=(16, 16, 16), equation="linadv", riemann="exact",
                               problem="linadv_sine", user=(1.0, 1.0, 1.0), bc="periodic",
                               advvel=(1.0, 0.5, -0.25), fixeddt=0.01),
+    # GLM-MHD: interior source points, cleaning speed, KEPES / HLL / LLF
+    "mhd_vortex3d_llf_sparse": config(n=(16, 16, 16), equation="mhd_glm", riemann="llf",
+                                      problem="mhd_vortex3d", user=(), bc="periodic",
+                                      gamma=5.0 / 3.0, fixeddt=0.002),
+    "mhd_vortex3d_kepes_full": config(n=(16, 16, 16), equation="mhd_glm", riemann="kepes",
+                                      problem="mhd_vortex3d", user=(), bc="periodic",
+                                      gamma=5.0 / 3.0, wenothresh=-1.0, fixeddt=0.002),
+    "mhd_vortex3d_nq3_hll": config(n=(12, 12, 12), nq=3, equation="mhd_glm", riemann="hll",
+                                   problem="mhd_vortex3d", user=(), bc="periodic",
+                                   gamma=5.0 / 3.0, fixeddt=0.002),
+    "orszag_tang2d_hll": config(dim=2, n=(48, 48, 1), equation="mhd_glm", riemann="hll",
+                                problem="orszag_tang", user=(), bc="periodic", gamma=5.0 / 3.0,
+                                fixeddt=0.002),
+    "orszag_tang2d_kepes_nq3": config(dim=2, n=(32, 32, 1), nq=3, equation="mhd_glm",
+                                      riemann="kepes", problem="orszag_tang", user=(),
+                                      bc="periodic", gamma=5.0 / 3.0, fixeddt=0.002),
+    "mhd_blast2d_reflect": config(dim=2, n=(40, 40, 1), equation="mhd_glm", riemann="llf",
+                                  problem="mhd_blast", user=(0.1, 0.70710678118654752, 0.70710678118654752),
+                                  bc="reflecting", gamma=5.0 / 3.0, fixeddt=0.0005),
+    "mhd_blast3d_kepes_outflow": config(n=(16, 16, 16), equation="mhd_glm", riemann="kepes",
+                                        problem="mhd_blast", user=(0.2, 0.5, 0.0), bc="outflow",
+                                        gamma=5.0 / 3.0, fixeddt=0.0005),
 }
 
 
