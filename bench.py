@@ -106,7 +106,21 @@ def main():
     ap.add_argument("--wenothresh", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-stages", type=int, default=4)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (5 launches), the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        import torch
+        n = a.calibrate_copy * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the domain decomposition over GPUs is design-only)", file=sys.stderr)

@@ -1,7 +1,8 @@
 // Device physics for the stage kernels: equation of state, linearised
 // primitive map, Riemann solvers.  Every expression keeps the operand order of
 // the reference so that, with contraction disabled, results agree with a CPU
-// evaluation of the same formulas to the last bit (library pow/exp aside).
+// evaluation of the same formulas to the last bit (library pow/exp aside; the
+// HLLC shear factor uses powu, a few ulp from the library pow).
 //
 //   Hydro EOS        src/physics/hydro/Hydro_EOS.H:15-93
 //   VarConvert       src/physics/hydro/Hydro_K.H:25-65
@@ -14,6 +15,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "powu.hpp"
 
 namespace kfvm {
 namespace dev {
@@ -111,10 +114,11 @@ struct Hydro {
     const double psiL = uL[0] * (sL - vL[v0]), psiR = uR[0] * (sR - vR[v0]),
                  psiFac = psiR * psiL / (psiR - psiL);
     const double presRat = fmin(vL[4] / vR[4], vR[4] / vL[4]);
-    const double gFac = 1.0 - pow(presRat, M);
+    const double gFac = 1.0 - powu(presRat, M);  // presRat in (0, 1], M >= 0
     const double shFacL = psiFac * gFac * sL / (sL - sS), shFacR = psiFac * gFac * sR / (sR - sS);
     M = fmin(M, 1.0);
-    const double fM = M * sqrt(4.0 + pow(1.0 - M * M, 2.0)) / (1.0 + M * M);
+    const double oneMsq = 1.0 - M * M;  // pow(x, 2.0) is the correctly rounded x * x
+    const double fM = M * sqrt(4.0 + oneMsq * oneMsq) / (1.0 + M * M);
     const double pFacL = psiFac * (fM - 1.0) * uJump + sL * pS / (sL - sS);
     const double pFacR = psiFac * (fM - 1.0) * uJump + sR * pS / (sR - sS);
     if (0 <= sL) {

@@ -564,10 +564,12 @@ __device__ inline double block_max(double x, double *sh, int tid) {
   return r;
 }
 
-// Brick of cells handled by one workgroup of the flux kernel.
+// Brick of cells handled by one workgroup of the flux kernel.  16 cells in x
+// make every row of a face-state plane one full 128-byte line, so no line is
+// fetched twice by the two bricks that share it.
 template <int D>
 struct Brick {
-  static constexpr int X = D == 3 ? 8 : 16, Y = D == 3 ? 8 : 16, Z = D == 3 ? 4 : 1;
+  static constexpr int X = 16, Y = D == 3 ? 4 : 16, Z = D == 3 ? 4 : 1;
   static constexpr int NXF = (X + 1) * Y * Z, NYF = X * (Y + 1) * Z, NZF = D == 3 ? X * Y * (Z + 1) : 0;
   static constexpr int NF = NXF + NYF + NZF;
 };
