@@ -1,19 +1,24 @@
-"""Throughput of the KFVM-WENO per-stage update on one MI355X.
+"""Throughput of the KFVM-WENO update on one MI355X, plain vs learned indicator.
 
 Workload: 3D Euler (gamma 1.4) on [0,1]^3 at 256^3 cells, FP64, planar
 Riemann problem across x+y+z = 1.5 with (rho,u,v,w,p) = (1,0,0,0,1) /
 (0.125,0,0,0,0.1), outflow boundaries, HLLC, KFVM-WENO radius 2 with 2x2
-Gauss points per face, SSP(4,3,2) stages, sparse (KXRCF) WENO with threshold
-1.0 -- the plain method.  One bench "step" is one RK stage: reconstruction,
-selection, WENO, positivity limiter, Riemann solves, flux divergence,
-register update (and the error norm on the last stage of each RK step), over
-every cell, with the state resident in HBM.
+Gauss points per face, SSP(4,3,2) with the adaptive controller, to t = 0.2.
+It runs twice: plain (KXRCF selection of the WENO cells) and with the learned
+troubled-cell indicator (weights trained on the GPU by
+kfvm_amd.train_indicator on other planar problems at 64^3).  Each run warms
+up to t = 0.1 untimed, times K whole RK steps (take_step: reconstruction,
+selection, WENO, limiter, Riemann solves, divergence, register update, error
+norm, controller, and for the learned run the admissibility check), then
+finishes to t = 0.2 for the L1 density error against the exact (Sod)
+solution along the plane normal.  The state is resident in HBM throughout.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--retrain] [--plain-only]
 
-Prints one JSON line (metric, value in Mzone-updates/s, roofline of the
-dominant kernel from HIP events, CPU baseline from the oracle on a bounded
-sample of the same problem).
+Prints one JSON line: value = zone-updates per RK stage per second of the
+learned run (plain and learned figures and the L1 ratio alongside), the
+roofline of the dominant kernel from HIP events, and the CPU baseline (the
+oracle on a bounded sample of the same problem).
 """
 import argparse
 import json
@@ -63,6 +68,9 @@ _SUBSUM = 33 + 7 + 6 * 11                      # 106 stencil columns over all su
 FLOP_WENO = 2 * 19 * 5 * _SUBSUM + 2 * 8 * 19 * 5 + 2 * 6 * 4 * 5 * _SUBSUM  # 47,100
 FLOP_LINEAR = 2 * 6 * 4 * 5 * 33               # 7,920
 BYTES_STAGE = 40 * 5                           # read U_s, Uprev, Uhat; write U_s+1, Uhat
+# HBM bytes per launch of a kernel from the FETCH_SIZE / WRITE_SIZE counters at
+# 256^3 (profiles/flux_rk_traffic_riemann3d_256.md); None where not measured.
+TRAFFIC = {}
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -97,107 +105,150 @@ def cpu_baseline(seconds_target=15.0):
                       f"problem, {stages} RK stages ({steps} fixed-dt steps) in {el:.1f} s"}
 
 
+def _mode_run(n, weights, a, profile):
+    """One 256^3 run of the planar Riemann problem: untimed warm-up into the
+    developed flow, a timed window of whole RK steps (take_step: stages,
+    error norm, controller, admissibility check), then the rest of the run to
+    t = 0.2 for the L1 error."""
+    import numpy as np
+    import kfvm_amd
+    from kfvm_amd.exact import planar_riemann_l1
+    s = kfvm_amd.Solver(init_text=init_text(n, a.wenothresh))
+    if weights is not None:
+        s.set_indicator(weights, 0.0)
+    if a.warmup >= 0:
+        for _ in range(a.warmup):
+            s.take_step()
+    else:
+        while s.time[0] < 0.1 and not s.time[0] >= 0.2:
+            s.take_step()
+    s.sync()
+    c0 = s.counters
+    t0 = time.perf_counter()
+    nweno = 0
+    for _ in range(a.steps):
+        s.take_step()
+        nweno += s.counters["last_nweno"]
+    s.sync()
+    el = time.perf_counter() - t0
+    stages = s.counters["rhs_evals"] - c0["rhs_evals"]
+    value = n ** 3 * stages / el / 1e6
+    kern = {}
+    if profile:
+        s.profile(True)
+        for _ in range(profile):
+            s.take_step()
+        s.profile(False)
+        for name in ("cell_props", "linear", "select", "weno", "weno_post", "pp", "flux_rk", "stage_pre", "nad"):
+            ms, cnt = s.profile_get(name)
+            if cnt:
+                kern[name] = {"ms_total": ms, "launches": cnt, "ms_avg": ms / cnt}
+    while s.time[0] < 0.2 - 1e-12:
+        if s.take_step() not in (0, 1, 2):
+            break
+    l1 = planar_riemann_l1(s.state[0], s.time[0])
+    out = {"value": value, "ms_per_step": el / a.steps * 1e3, "stages": stages, "seconds": el,
+           "weno_fraction": nweno / a.steps / n ** 3, "l1_rho_t0.2": l1, "t_final": s.time[0],
+           "steps_total": s.time[2] - 1}
+    if weights is not None:
+        out.update(s.indicator_counters)
+    s.close()
+    return out, kern
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=10, help="timed RK steps per method")
+    ap.add_argument("--warmup", type=int, default=-1,
+                    help="untimed RK steps first; -1: until t = 0.1 (the developed flow)")
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--wenothresh", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-stages", type=int, default=4)
+    ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--retrain", action="store_true", help="train the indicator on the GPU first")
+    ap.add_argument("--profile-steps", type=int, default=1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.calibrate_copy:
         import torch
-        n = a.calibrate_copy * (1 << 20) // 8
-        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        nn = a.calibrate_copy * (1 << 20) // 8
+        src = torch.rand(nn, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         for _ in range(5):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 5}))
+        print(json.dumps({"copy_bytes_read": nn * 8, "copy_bytes_written": nn * 8, "launches": 5}))
         return
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the domain decomposition over GPUs is design-only)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import kfvm_amd
+    from kfvm_amd import train_indicator
 
     n = a.size
-    t_setup = time.perf_counter()
-    s = kfvm_amd.Solver(init_text=init_text(n, a.wenothresh))
-    _, dt0, _ = s.time
-    dt = 100.0 * dt0  # cfl * dmin / vmax of the initial state
-    print(f"[bench] setup {time.perf_counter() - t_setup:.1f} s, {n}^3, dt {dt:.4g}", flush=True)
-
-    s.advance_stages(a.warmup, dt)
-    s.sync()
-    t0 = time.perf_counter()
-    s.advance_stages(a.steps, dt)
-    s.sync()
-    el = time.perf_counter() - t0
+    weights = None
+    if not a.plain_only:
+        if a.retrain:
+            weights, _ = train_indicator.train("euler3d", out_dir=os.path.join(ROOT, "gpurun_out", "weights"),
+                                               log=lambda m: print(m, flush=True))
+        else:
+            weights = train_indicator.load("euler3d")
+    plain, kp = _mode_run(n, None, a, a.profile_steps if weights is None else 0)
+    print(f"[bench] plain: {plain['value']:.1f} Mzone-updates/s, WENO {plain['weno_fraction']:.3f}, "
+          f"L1 {plain['l1_rho_t0.2']:.5e}", flush=True)
+    learned, kl = (None, {})
+    if weights is not None:
+        learned, kl = _mode_run(n, weights, a, a.profile_steps)
+        print(f"[bench] learned: {learned['value']:.1f} Mzone-updates/s, WENO {learned['weno_fraction']:.3f}, "
+              f"L1 {learned['l1_rho_t0.2']:.5e}, redone steps {learned['nad_fallbacks']}", flush=True)
+    head = learned if learned is not None else plain
+    kern = kl if learned is not None else kp
     ncell = n ** 3
-    value = ncell * a.steps / el / 1e6
-    print(f"[bench] {a.steps} stages in {el:.3f} s -> {value:.1f} Mzone-updates/s", flush=True)
-
-    # per-kernel device time (HIP events on the solver's stream), separate pass
-    s.profile(True)
-    flagged = []
-    for _ in range(a.profile_stages):
-        s.advance_stages(1, dt)
-        if a.wenothresh > 0:
-            flagged.append(float(s.flags[0].sum()))
-    s.profile(False)
-    kern = {}
-    for name in ("cell_props", "linear", "select", "weno", "weno_post", "pp", "flux_rk",
-                 "stage_pre"):
-        ms, cnt = s.profile_get(name)
-        if cnt:
-            kern[name] = {"ms_total": ms, "launches": cnt, "ms_avg": ms / cnt}
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
     d = kern[dom]
     if dom == "weno":
-        cells = (sum(flagged) / len(flagged)) if flagged else ncell
-        flops = FLOP_WENO * cells
-        roof = {"bound": "mfma", "achieved": flops / (d["ms_avg"] * 1e-3) / 1e12,
+        cells = head["weno_fraction"] * ncell
+        roof = {"bound": "mfma", "achieved": FLOP_WENO * cells / (d["ms_avg"] * 1e-3) / 1e12,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
     elif dom == "linear":
-        flops = FLOP_LINEAR * ncell
-        roof = {"bound": "mfma", "achieved": flops / (d["ms_avg"] * 1e-3) / 1e12,
+        roof = {"bound": "mfma", "achieved": FLOP_LINEAR * ncell / (d["ms_avg"] * 1e-3) / 1e12,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
     else:
         # flux/RK kernel: HBM roof on its algorithmic register traffic
-        byts = BYTES_STAGE * ncell
-        roof = {"bound": "hbm", "achieved": byts / (d["ms_avg"] * 1e-3) / 1e9,
+        roof = {"bound": "hbm", "achieved": BYTES_STAGE * ncell / (d["ms_avg"] * 1e-3) / 1e9,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = TRAFFIC.get(dom)
     roof["kernel"] = dom
-    frac_flagged = (sum(flagged) / len(flagged) / ncell) if flagged else 1.0
 
     cpu = None if a.no_cpu else cpu_baseline()
+    method = "learned indicator" if learned is not None else "plain (KXRCF)"
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "Mzone-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": el / a.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3D Euler planar Riemann {n}^3, HLLC, KFVM-WENO r=2 NQ=2, "
-                               f"SSP(4,3,2), plain (KXRCF sparse WENO, threshold {a.wenothresh})",
-                   "cells": ncell, "weno_fraction": frac_flagged, "step": "one RK stage"},
+        "config": {"workload": f"3D Euler planar Riemann {n}^3 (x+y+z=1.5, (1,0,0,0,1)/(0.125,0,0,0,0.1)), "
+                               f"HLLC, KFVM-WENO r=2 NQ=2, SSP(4,3,2) adaptive, to t=0.2; value: {method}",
+                   "cells": ncell, "step": "one RK step (4 stages); value counts zone-updates per stage"},
+        "plain": plain,
+        "learned": learned,
+        "speedup_learned_vs_plain": (learned["value"] / plain["value"]) if learned else None,
+        "l1_ratio_learned_vs_plain": (learned["l1_rho_t0.2"] / plain["l1_rho_t0.2"]) if learned else None,
         "roofline": roof,
         "kernels_ms_avg": {k: v["ms_avg"] for k, v in kern.items()},
         "cpu_baseline": cpu,

@@ -99,7 +99,53 @@ int kfvm_profile_get(kfvm_solver *s, const char *name, double *ms, int64_t *coun
 int kfvm_profile_reset(kfvm_solver *s);
 int kfvm_sync(kfvm_solver *s);
 
+/* ---- learned troubled-cell indicator (Euler) ----
+ * None of these has a counterpart in the reference: they add a small network
+ * (features -> 32 -> 32 -> logit, bfloat16 weights, FP32 sums) that replaces
+ * the KXRCF selection (WenoSelect_K, src/numeric/Numeric_K.H:292-398) of which
+ * cells get WENO, trained on the GPU on labels the solver generates. */
+typedef struct kfvm_indicator kfvm_indicator;
+
+/* Switch the solver to the learned selection with n bfloat16 parameters
+ * (kfvm_indicator_sizes) and logit threshold; n = 0 returns to KXRCF.  While
+ * on, each accepted step is checked for numerical admissibility: rho and p of
+ * every cell inside its neighbourhood's previous range widened by a tolerance
+ * (kfvm_set_nad).  Cells that fail get WENO in every stage of the next step;
+ * a step in which more than max_frac of the cells fail is redone with KXRCF. */
+int kfvm_set_indicator(kfvm_solver *s, const uint16_t *weights, int64_t n, double threshold);
+/* Admissibility check of learned steps: on/off, tolerance
+ * delta = max(value_frac |bound|, range_frac (max - min)) and the fraction of
+ * failing cells that makes a step be redone; defaults on, 0.05, 1e-3, 0.02. */
+int kfvm_set_nad(kfvm_solver *s, int on, double range_frac, double value_frac, double max_frac);
+/* Steps redone with KXRCF, violating cells summed over the checked steps. */
+int kfvm_get_indicator_counters(kfvm_solver *s, int64_t *nad_fallbacks, int64_t *nad_cells, int *learned);
+
+int kfvm_indicator_create(int dim, kfvm_indicator **out);
+int kfvm_indicator_destroy(kfvm_indicator *p);
+/* Append the samples of the solver's current state: features of every cell and
+ * label 1 where the full-WENO face states (after the limiter) differ from the
+ * linear ones by more than tau times the stencil's range of rho or E (a range
+ * below 1e-3 of the cell's value counts as that floor). */
+int kfvm_indicator_collect(kfvm_indicator *p, kfvm_solver *s, double tau, int64_t *nsamples,
+                           int64_t *npositive);
+int kfvm_indicator_samples(kfvm_indicator *p, float *X, float *Y, int64_t offset, int64_t n);
+int kfvm_indicator_init(kfvm_indicator *p, uint64_t seed);
+/* iters Adam steps (lr, betas 0.9/0.999) on batches drawn from the samples;
+ * class weight pos_weight on label-1 samples; loss = last batch's mean loss. */
+int kfvm_indicator_train(kfvm_indicator *p, int iters, int batch, double lr, double pos_weight, uint64_t seed,
+                         double *loss);
+int kfvm_indicator_get_params(kfvm_indicator *p, float *params, int64_t n);
+int kfvm_indicator_set_params(kfvm_indicator *p, const float *params, int64_t n);
+int kfvm_indicator_get_weights(kfvm_indicator *p, uint16_t *weights, int64_t n);
+/* Mean loss and its gradient over n explicit samples at params (as given). */
+int kfvm_indicator_gradient(kfvm_indicator *p, const float *params, const float *X, const float *Y, int64_t n,
+                            double pos_weight, float *grad, double *loss);
+int kfvm_indicator_logits(kfvm_indicator *p, const float *params, const float *X, int64_t n, float *z);
+
 /* ---- host-only entry points (no device work; usable without a GPU) ---- */
+
+/* Feature and parameter counts of the learned indicator's network. */
+int kfvm_indicator_sizes(int dim, int64_t *ninputs, int64_t *nparams);
 
 /* Shapes of the weight tables for (dim, nq): nface = nsub*2*dim*Q*C,
  * nderiv = nsub*I*C. */

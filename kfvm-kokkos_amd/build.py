@@ -29,7 +29,7 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC"
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 HOST_CXX = ["common.cpp", "weights.cpp", "problems.cpp"]
-HOST_HIP = ["solver.cpp", "capi.cpp"]
+HOST_HIP = ["solver.cpp", "indicator.cpp", "capi.cpp"]
 DEVICE = ["dispatch.hip", "inst_d2q2.hip", "inst_d2q3.hip", "inst_d3q2.hip", "inst_d3q3.hip"]
 
 

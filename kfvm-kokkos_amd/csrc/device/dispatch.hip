@@ -110,6 +110,47 @@ hipError_t launch_finalize(const double *errPartial, const double *posPartial, i
   return hipGetLastError();
 }
 
+int net_inputs(int dim) { return dim == 3 ? dev::Net<3>::NIN : dev::Net<2>::NIN; }
+int net_params(int dim) { return dim == 3 ? dev::Net<3>::NP : dev::Net<2>::NP; }
+int net_train_block() { return dev::TB; }
+
+hipError_t launch_learned_select(int dim, const double *U, const double *eint, const double *theta,
+                                 const double *nadFlag, const float *P, float thr, double *wflag, int *list,
+                                 int *counter, const Grid &g, hipStream_t s) {
+  KFVM_SELECT(dim, 2, L::learned_select(U, eint, theta, nadFlag, P, thr, wflag, list, counter, g, s));
+}
+hipError_t launch_features(int dim, const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {
+  KFVM_SELECT(dim, 2, L::features(U, eint, X, g, s));
+}
+hipError_t launch_labels(int dim, int nq, const double *U, const double *FSL, const double *FSW, float tau,
+                         float *Y, const Grid &g, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::labels(U, FSL, FSW, tau, Y, g, s));
+}
+hipError_t launch_logits(int dim, const float *X, const float *P, float *Z, int n, hipStream_t s) {
+  KFVM_SELECT(dim, 2, L::logits(X, P, Z, n, s));
+}
+hipError_t launch_train_grad(int dim, const float *X, const float *Y, const int *idx, int n, int nb,
+                             uint64_t seed, const float *P, float posW, float *part, float *lossPart,
+                             hipStream_t s) {
+  KFVM_SELECT(dim, 2, L::train_grad(X, Y, idx, n, nb, seed, P, posW, part, lossPart, s));
+}
+hipError_t launch_train_reduce(const float *part, const float *lossPart, int nblk, int np, int nb, float *grad,
+                               float *loss, hipStream_t s) {
+  hipLaunchKernelGGL(dev::k_train_reduce, dim3(stage_blocks(np)), dim3(dev::BLK), 0, s, part, lossPart, nblk, np,
+                     nb, grad, loss);
+  return hipGetLastError();
+}
+hipError_t launch_adam(float *M, float *Pbf, const float *grad, float *m1, float *m2, int np, float lr,
+                       float b1, float b2, float eps, float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(dev::k_adam, dim3(stage_blocks(np)), dim3(dev::BLK), 0, s, M, Pbf, grad, m1, m2, np, lr, b1,
+                     b2, eps, c1, c2);
+  return hipGetLastError();
+}
+hipError_t launch_nad(int dim, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
+                      int *count, const Grid &g, hipStream_t s) {
+  KFVM_SELECT(dim, 2, L::nad(Uold, Unew, fr, fv, nadFlag, count, g, s));
+}
+
 hipError_t launch_fill(double *p, double v, size_t n, hipStream_t s) {
   const unsigned nb = unsigned(std::min<size_t>((n + dev::BLK - 1) / dev::BLK, 4096));
   hipLaunchKernelGGL(dev::k_fill, dim3(nb ? nb : 1), dim3(dev::BLK), 0, s, p, v, n);

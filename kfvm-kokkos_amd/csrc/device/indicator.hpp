@@ -1,0 +1,420 @@
+// Learned troubled-cell indicator for the Euler equations: features, the
+// network, training labels, training (forward / backward / Adam) and the
+// a-posteriori numerical-admissibility (NAD) check.  The reference has no such
+// component; it replaces the KXRCF selector (WenoSelect_K, Numeric_K.H:292-398)
+// as the decision "linear GP reconstruction suffices / full WENO needed".
+//
+// Features of a cell (Net<D>::NIN = 2C floats), from the C cells of its
+// radius-2 stencil in the stencil order, density rho and pressure p, with o
+// the centre cell and m = 0..C-2 running over the other cells in order:
+//   x[m]        = (rho_m - rho_o) / s_rho
+//   x[C-1+m]    = (p_m - p_o) / s_p
+//   x[2C-2]     = min(s_rho / |rho_o|, 4),   x[2C-1] = min(s_p / |p_o|, 4)
+// with s_q = max_m |q_m - q_o| + 1e-300: the stencil's shape in [-1, 1] and
+// its relative range.
+//
+// Network: x -> 32 (ReLU) -> 32 (ReLU) -> logit z.  Weights are bfloat16
+// values (held as floats), sums in FP32 fma order over ascending inputs.
+// Parameter layout (floats): W1[H][NIN], b1[H], W2[H][H], b2[H], w3[H], b3.
+//
+// Label of a cell: 1 if the full-WENO face states (after the limiter) differ
+// from the linear ones by more than tau times the stencil's range of rho or E,
+// ranges below 1e-3 of the cell's value counting as that floor (so grid-scale
+// noise far below the state's magnitude is not a reason for WENO):
+//   max_{f,q, v in {rho, E}} |F_weno - F_lin| / (range_v + 1e-3 |U_v| + 1e-300) > tau
+#pragma once
+
+namespace kfvm {
+namespace dev {
+
+template <int D>
+struct Net {
+  static constexpr int C = SInfo<D>::C, NIN = 2 * C, H = 32;
+  static constexpr int oW1 = 0, oB1 = H * NIN, oW2 = oB1 + H, oB2 = oW2 + H * H, oW3 = oB2 + H,
+                       oB3 = oW3 + H, NP = oB3 + 1;
+};
+
+// Index of the centre cell in the stencil order.
+template <int D>
+constexpr int centre_index() {
+  for (int n = 0; n < SInfo<D>::C; n++)
+    if (StTab<D>::ST.off[n][0] == 0 && StTab<D>::ST.off[n][1] == 0 && StTab<D>::ST.off[n][2] == 0) return n;
+  return -1;
+}
+
+// Features of cell (i,j,k) into x[NIN] (FP32).  eint: per-cell internal
+// energy E - |m|^2 / (2 rho) (the limiter's cell properties), so p = (g-1) eint
+// is the pressure of Hydro_EOS.H bit for bit; both are even in the momenta, so
+// mirrored ghost cells need no sign.
+template <int D>
+__device__ inline void cell_features(const double *__restrict__ U, const double *__restrict__ eint,
+                                     const Grid &g, int i, int j, int k, float *x) {
+  constexpr auto &ST = StTab<D>::ST;
+  constexpr int C = SInfo<D>::C, O = centre_index<D>();
+  static_assert(O >= 0, "stencil without its centre");
+  const double gm1 = g.gamma - 1.0;
+  double rho[C], pr[C];
+  // cells whose stencil stays inside the grid index it directly
+  const bool inner = i >= 2 && i < g.nx - 2 && j >= 2 && j < g.ny - 2 && (D == 2 || (k >= 2 && k < g.nz - 2));
+  const int c = (k * g.ny + j) * g.nx + i;
+#pragma unroll
+  for (int n = 0; n < C; n++) {
+    size_t idx;
+    if (inner)
+      idx = size_t(c + ST.off[n][0] + g.nx * (ST.off[n][1] + (D == 3 ? g.ny * ST.off[n][2] : 0)));
+    else
+      idx = map_cell<D, 5>(g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2]).idx;
+    rho[n] = U[idx];
+    pr[n] = gm1 * eint[idx];
+  }
+  double sr = 0.0, sp = 0.0;
+#pragma unroll
+  for (int n = 0; n < C; n++) {
+    if (n == O) continue;
+    sr = fmax(sr, fabs(rho[n] - rho[O]));
+    sp = fmax(sp, fabs(pr[n] - pr[O]));
+  }
+  const double ir = 1.0 / (sr + 1e-300), ip = 1.0 / (sp + 1e-300);
+#pragma unroll
+  for (int n = 0; n < C; n++) {
+    if (n == O) continue;
+    const int m = n < O ? n : n - 1;
+    x[m] = float((rho[n] - rho[O]) * ir);
+    x[C - 1 + m] = float((pr[n] - pr[O]) * ip);
+  }
+  x[2 * C - 2] = float(fmin(sr / fabs(rho[O]), 4.0));
+  x[2 * C - 1] = float(fmin(sp / fabs(pr[O]), 4.0));
+}
+
+// Logit of the network for features x; params in the layout above.
+template <int D>
+__device__ inline float net_logit(const float *__restrict__ P, const float *x) {
+  using N = Net<D>;
+  float h1[N::H], h2[N::H];
+#pragma unroll 4
+  for (int j = 0; j < N::H; j++) {
+    float a = P[N::oB1 + j];
+#pragma unroll
+    for (int i = 0; i < N::NIN; i++) a = fmaf(P[N::oW1 + j * N::NIN + i], x[i], a);
+    h1[j] = fmaxf(a, 0.0f);
+  }
+#pragma unroll 4
+  for (int j = 0; j < N::H; j++) {
+    float a = P[N::oB2 + j];
+#pragma unroll
+    for (int i = 0; i < N::H; i++) a = fmaf(P[N::oW2 + j * N::H + i], h1[i], a);
+    h2[j] = fmaxf(a, 0.0f);
+  }
+  float z = P[N::oB3];
+#pragma unroll
+  for (int i = 0; i < N::H; i++) z = fmaf(P[N::oW3 + i], h2[i], z);
+  return z;
+}
+
+__device__ inline void wave_append(bool flagged, int c, int *__restrict__ list, int *__restrict__ counter) {
+  const unsigned long long mask = __ballot(flagged);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int cnt = __popcll(mask);
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, cnt);
+  base = __shfl(base, leader);
+  if (flagged) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = c;
+}
+
+// Learned selection: WENO where the network's logit exceeds thr, where the
+// previous RHS evaluation limited the cell (theta < 1, as the KXRCF selector),
+// or where the last step's admissibility check failed (nadFlag > 0).
+template <int D>
+__global__ __launch_bounds__(BLK) void k_learned_select(const double *__restrict__ U,
+                                                        const double *__restrict__ eint,
+                                                        const double *__restrict__ theta,
+                                                        const double *__restrict__ nadFlag,
+                                                        const float *__restrict__ P, float thr,
+                                                        double *__restrict__ wflag,
+                                                        int *__restrict__ list,
+                                                        int *__restrict__ counter, Grid g) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  bool flagged = false;
+  if (c < g.ncell) {
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    float x[Net<D>::NIN];
+    cell_features<D>(U, eint, g, i, j, k, x);
+    flagged = net_logit<D>(P, x) > thr || theta[c] < 1.0 || nadFlag[c] > 0.0;
+    wflag[c] = flagged ? 1.0 : 0.0;
+  }
+  wave_append(flagged, c, list, counter);
+}
+
+// Features of every cell, feature-major rows: X[c][NIN] (training data).
+template <int D>
+__global__ __launch_bounds__(BLK) void k_features(const double *__restrict__ U, const double *__restrict__ eint,
+                                                  float *__restrict__ X, Grid g) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  float x[Net<D>::NIN];
+  cell_features<D>(U, eint, g, i, j, k, x);
+#pragma unroll
+  for (int n = 0; n < Net<D>::NIN; n++) X[size_t(c) * Net<D>::NIN + n] = x[n];
+}
+
+// Network logits of n feature rows (inference check).
+template <int D>
+__global__ __launch_bounds__(BLK) void k_logits(const float *__restrict__ X, const float *__restrict__ P,
+                                                float *__restrict__ Z, int n) {
+  const int s = blockIdx.x * BLK + threadIdx.x;
+  if (s >= n) return;
+  float x[Net<D>::NIN];
+#pragma unroll
+  for (int i = 0; i < Net<D>::NIN; i++) x[i] = X[size_t(s) * Net<D>::NIN + i];
+  Z[s] = net_logit<D>(P, x);
+}
+
+// Labels from the linear (FSL) and full-WENO (FSW) face states of every cell.
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_labels(const double *__restrict__ U, const double *__restrict__ FSL,
+                                                const double *__restrict__ FSW, float tau,
+                                                float *__restrict__ Y, Grid g) {
+  constexpr auto &ST = StTab<D>::ST;
+  constexpr int C = SInfo<D>::C, FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+  double lo[2] = {1.7976931348623157e308, 1.7976931348623157e308}, hi[2] = {-1.7976931348623157e308,
+                                                                            -1.7976931348623157e308};
+#pragma unroll
+  for (int n = 0; n < C; n++) {
+    const CellRef r = map_cell<D, 5>(g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2]);
+    const double d = ld(U, nc, r, 0), e = ld(U, nc, r, 4);
+    lo[0] = fmin(lo[0], d);
+    hi[0] = fmax(hi[0], d);
+    lo[1] = fmin(lo[1], e);
+    hi[1] = fmax(hi[1], e);
+  }
+  const double sc[2] = {hi[0] - lo[0] + 1e-3 * fabs(U[c]) + 1e-300,
+                        hi[1] - lo[1] + 1e-3 * fabs(U[4 * nc + c]) + 1e-300};
+  double d = 0.0;
+#pragma unroll 1
+  for (int fq = 0; fq < FQ; fq++) {
+    const size_t o0 = (size_t(fq) * 5 + 0) * nc + c, o4 = (size_t(fq) * 5 + 4) * nc + c;
+    d = fmax(d, fabs(FSW[o0] - FSL[o0]) / sc[0]);
+    d = fmax(d, fabs(FSW[o4] - FSL[o4]) / sc[1]);
+  }
+  Y[c] = d > double(tau) ? 1.0f : 0.0f;
+}
+
+// ------------------------------------------------------------------ training
+constexpr int TB = 64;  // samples per training block (one wave)
+
+__device__ inline uint32_t mix32(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return uint32_t((x ^ (x >> 31)) >> 32);
+}
+
+// Sample index b of a batch: explicit (idx != nullptr) or drawn uniformly from
+// [0, n) by a counter hash of (seed, b).
+__device__ inline int batch_index(const int *__restrict__ idx, int b, int n, uint64_t seed) {
+  return idx ? idx[b] : int(uint64_t(mix32(seed * 0x100000001B3ull + uint64_t(b))) % uint64_t(n));
+}
+
+// Per-block partial gradient of the class-weighted logistic loss
+//   L_s = w_s * softplus(z_s (1 - 2 y_s)),  w_s = posW if y_s = 1 else 1,
+// over TB samples of a batch of nb: part[blk][NP] gradient sums, lossPart[blk].
+template <int D>
+__global__ __launch_bounds__(TB) void k_train_grad(const float *__restrict__ X, const float *__restrict__ Y,
+                                                   const int *__restrict__ idx, int n, int nb,
+                                                   uint64_t seed, const float *__restrict__ P,
+                                                   float posW, float *__restrict__ part,
+                                                   float *__restrict__ lossPart) {
+  using N = Net<D>;
+  __shared__ float xs[TB][N::NIN + 1];
+  __shared__ float h1s[TB][N::H + 1], h2s[TB][N::H + 1], d1s[TB][N::H + 1], d2s[TB][N::H + 1];
+  __shared__ float dzs[TB], ls[TB];
+  const int t = threadIdx.x;
+  const int b = blockIdx.x * TB + t;
+  const bool live = b < nb;
+  const int s = live ? batch_index(idx, b, n, seed) : 0;
+  float x[N::NIN];
+#pragma unroll
+  for (int i = 0; i < N::NIN; i++) {
+    x[i] = live ? X[size_t(s) * N::NIN + i] : 0.0f;
+    xs[t][i] = x[i];
+  }
+  const float y = live ? Y[s] : 0.0f;
+  // forward
+  float a1[N::H], a2[N::H];
+#pragma unroll 4
+  for (int j = 0; j < N::H; j++) {
+    float a = P[N::oB1 + j];
+#pragma unroll
+    for (int i = 0; i < N::NIN; i++) a = fmaf(P[N::oW1 + j * N::NIN + i], x[i], a);
+    a1[j] = a;
+    h1s[t][j] = fmaxf(a, 0.0f);
+  }
+#pragma unroll 4
+  for (int j = 0; j < N::H; j++) {
+    float a = P[N::oB2 + j];
+#pragma unroll
+    for (int i = 0; i < N::H; i++) a = fmaf(P[N::oW2 + j * N::H + i], fmaxf(a1[i], 0.0f), a);
+    a2[j] = a;
+    h2s[t][j] = fmaxf(a, 0.0f);
+  }
+  float z = P[N::oB3];
+#pragma unroll
+  for (int i = 0; i < N::H; i++) z = fmaf(P[N::oW3 + i], fmaxf(a2[i], 0.0f), z);
+  // loss and its derivative in z
+  const float w = y > 0.5f ? posW : 1.0f;
+  const float m = y > 0.5f ? -z : z;  // softplus argument
+  const float sp = fmaxf(m, 0.0f) + log1pf(expf(-fabsf(m)));
+  const float sig = 1.0f / (1.0f + expf(-z));
+  const float dz = live ? w * (sig - y) : 0.0f;
+  ls[t] = live ? w * sp : 0.0f;
+  dzs[t] = dz;
+  // backward through the hidden layers
+#pragma unroll
+  for (int j = 0; j < N::H; j++) d2s[t][j] = a2[j] > 0.0f ? dz * P[N::oW3 + j] : 0.0f;
+#pragma unroll 4
+  for (int i = 0; i < N::H; i++) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < N::H; j++) acc = fmaf(d2s[t][j], P[N::oW2 + j * N::H + i], acc);
+    d1s[t][i] = a1[i] > 0.0f ? acc : 0.0f;
+  }
+  __syncthreads();
+  // parameter gradients summed over the block's samples (ascending sample order)
+  float *out = part + size_t(blockIdx.x) * N::NP;
+  for (int p = t; p < N::NP; p += TB) {
+    float acc = 0.0f;
+    if (p < N::oB1) {
+      const int j = p / N::NIN, i = p % N::NIN;
+      for (int q = 0; q < TB; q++) acc = fmaf(d1s[q][j], xs[q][i], acc);
+    } else if (p < N::oW2) {
+      const int j = p - N::oB1;
+      for (int q = 0; q < TB; q++) acc += d1s[q][j];
+    } else if (p < N::oB2) {
+      const int j = (p - N::oW2) / N::H, i = (p - N::oW2) % N::H;
+      for (int q = 0; q < TB; q++) acc = fmaf(d2s[q][j], h1s[q][i], acc);
+    } else if (p < N::oW3) {
+      const int j = p - N::oB2;
+      for (int q = 0; q < TB; q++) acc += d2s[q][j];
+    } else if (p < N::oB3) {
+      const int i = p - N::oW3;
+      for (int q = 0; q < TB; q++) acc = fmaf(dzs[q], h2s[q][i], acc);
+    } else {
+      for (int q = 0; q < TB; q++) acc += dzs[q];
+    }
+    out[p] = acc;
+  }
+  if (t == 0) {
+    float l = 0.0f;
+    for (int q = 0; q < TB; q++) l += ls[q];
+    lossPart[blockIdx.x] = l;
+  }
+}
+
+#ifdef KFVM_SMALL_KERNELS
+// grad[p] = sum over blocks of part[blk][p] / nb (fixed order); loss likewise.
+__global__ __launch_bounds__(BLK) void k_train_reduce(const float *__restrict__ part,
+                                                      const float *__restrict__ lossPart, int nblk,
+                                                      int np, int nb, float *__restrict__ grad,
+                                                      float *__restrict__ loss) {
+  const int p = blockIdx.x * BLK + threadIdx.x;
+  if (p < np) {
+    float acc = 0.0f;
+    for (int b = 0; b < nblk; b++) acc += part[size_t(b) * np + p];
+    grad[p] = acc / float(nb);
+  }
+  if (p == 0) {
+    float l = 0.0f;
+    for (int b = 0; b < nblk; b++) l += lossPart[b];
+    *loss = l / float(nb);
+  }
+}
+
+// bfloat16 rounding (to nearest, ties to even) of a float, kept as a float.
+__device__ inline float bf16_round(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xffff0000u);
+}
+
+// Adam step on the FP32 master parameters M; Pbf receives their bfloat16
+// values, which the forward pass uses.
+__global__ __launch_bounds__(BLK) void k_adam(float *__restrict__ M, float *__restrict__ Pbf,
+                                              const float *__restrict__ grad, float *__restrict__ m1,
+                                              float *__restrict__ m2, int np, float lr, float b1,
+                                              float b2, float eps, float c1, float c2) {
+  const int p = blockIdx.x * BLK + threadIdx.x;
+  if (p >= np) return;
+  const float gp = grad[p];
+  const float a = b1 * m1[p] + (1.0f - b1) * gp;
+  const float v = b2 * m2[p] + (1.0f - b2) * gp * gp;
+  m1[p] = a;
+  m2[p] = v;
+  const float th = M[p] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+  M[p] = th;
+  Pbf[p] = bf16_round(th);
+}
+
+#endif  // KFVM_SMALL_KERNELS
+
+// ------------------------------------------------------------- NAD check
+// Numerical admissibility of the new cell averages: rho and p of every cell
+// within [min - delta, max + delta] of its 3^D neighbourhood at the previous
+// step, delta = max(fv max(|min|, |max|), fr (max - min)).  Marks the
+// violating cells in nadFlag (1 / 0) and counts them.
+template <int D>
+__global__ __launch_bounds__(BLK) void k_nad(const double *__restrict__ Uold, const double *__restrict__ Unew,
+                                             double fr, double fv, double *__restrict__ nadFlag,
+                                             int *__restrict__ count, Grid g) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  bool bad = false;
+  if (c < g.ncell) {
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    const size_t nc = g.ncell;
+    const double gm1 = g.gamma - 1.0;
+    double lo[2] = {1.7976931348623157e308, 1.7976931348623157e308};
+    double hi[2] = {-1.7976931348623157e308, -1.7976931348623157e308};
+    constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
+#pragma unroll 1
+    for (int xo = -1; xo <= 1; xo++)
+#pragma unroll 1
+      for (int yo = -1; yo <= 1; yo++)
+#pragma unroll 1
+        for (int zo = zl; zo <= zh; zo++) {
+          const CellRef r = map_cell<D, 5>(g, i + xo, j + yo, k + zo);
+          double s[5];
+#pragma unroll
+          for (int v = 0; v < 5; v++) s[v] = ld(Uold, nc, r, v);
+          const double p = gm1 * (s[4] - 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0]);
+          lo[0] = fmin(lo[0], s[0]);
+          hi[0] = fmax(hi[0], s[0]);
+          lo[1] = fmin(lo[1], p);
+          hi[1] = fmax(hi[1], p);
+        }
+    double s[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) s[v] = Unew[size_t(v) * nc + c];
+    const double val[2] = {s[0], gm1 * (s[4] - 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0])};
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const double del = fmax(fv * fmax(fabs(lo[q]), fabs(hi[q])), fr * (hi[q] - lo[q]));
+      bad = bad || !(val[q] >= lo[q] - del && val[q] <= hi[q] + del);
+    }
+    nadFlag[c] = bad ? 1.0 : 0.0;
+  }
+  const unsigned long long mask = __ballot(bad);
+  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(count, __popcll(mask));
+}
+
+}  // namespace dev
+}  // namespace kfvm

@@ -104,6 +104,41 @@ struct Launch {
                        theta, g, Dm);
     return hipGetLastError();
   }
+  // ---- learned troubled-cell indicator
+  static hipError_t learned_select(const double *U, const double *eint, const double *theta, const double *nadFlag,
+                                   const float *P, float thr, double *wflag, int *list, int *counter,
+                                   const Grid &g, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_learned_select<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint,
+                       theta, nadFlag, P, thr, wflag, list, counter, g);
+    return hipGetLastError();
+  }
+  static hipError_t features(const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_features<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
+    return hipGetLastError();
+  }
+  static hipError_t labels(const double *U, const double *FSL, const double *FSW, float tau, float *Y,
+                           const Grid &g, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_labels<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW, tau,
+                       Y, g);
+    return hipGetLastError();
+  }
+  static hipError_t logits(const float *X, const float *P, float *Z, int n, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_logits<D>), dim3(stage_blocks(n)), dim3(dev::BLK), 0, s, X, P, Z, n);
+    return hipGetLastError();
+  }
+  static hipError_t train_grad(const float *X, const float *Y, const int *idx, int n, int nb, uint64_t seed,
+                               const float *P, float posW, float *part, float *lossPart, hipStream_t s) {
+    const unsigned nblk = unsigned((nb + dev::TB - 1) / dev::TB);
+    hipLaunchKernelGGL((dev::k_train_grad<D>), dim3(nblk), dim3(dev::TB), 0, s, X, Y, idx, n, nb, seed, P, posW,
+                       part, lossPart);
+    return hipGetLastError();
+  }
+  static hipError_t nad(const double *Uold, const double *Unew, double fr, double fv, double *nadFlag, int *count,
+                        const Grid &g, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_nad<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, Uold, Unew, fr, fv,
+                       nadFlag, count, g);
+    return hipGetLastError();
+  }
   static dim3 flux_grid(const Grid &g) {
     using Bk = dev::Brick<D>;
     return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X), unsigned((g.ny + Bk::Y - 1) / Bk::Y),

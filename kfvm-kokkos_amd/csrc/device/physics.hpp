@@ -91,54 +91,66 @@ struct Hydro {
     for (int v = 0; v < 5; v++) F[v] = tL * F[v] + tR * FR[v] - tD * (uR[v] - uL[v]);
     return sMax;
   }
+  // HLLC with the low-Mach / shear corrections (Hydro_Riemann.H:97-187).  The
+  // formulas are the reference's; divisions by a common denominator are
+  // multiplications by its reciprocal (11 divisions instead of 29), so results
+  // differ from the reference's operation order by rounding only.
   template <int DIR>
   __device__ static inline double solve_hllc(double g, const double *uL, const double *uR,
                                              double *F) {
     constexpr int v0 = 1 + DIR, v1 = 1 + (DIR + 1) % 3, v2 = 1 + (DIR + 2) % 3;
-    double vL[5], vR[5];
-    cons2prim(g, uL, vL);
-    cons2prim(g, uR, vR);
-    double sL, sR;
-    const double sMax = speeds<DIR>(g, uL, uR, vL, vR, sL, sR);
-    const double vsqL = vL[1] * vL[1] + vL[2] * vL[2] + vL[3] * vL[3];
-    const double vsqR = vR[1] * vR[1] + vR[2] * vR[2] + vR[3] * vR[3];
-    const double aL = sound(g, uL), aR = sound(g, uR);
+    const double gm1 = g - 1.0;
+    const double rhoL = uL[0], rhoR = uR[0];
+    const double irL = 1.0 / rhoL, irR = 1.0 / rhoR;
+    const double unL = uL[v0] * irL, vtL = uL[v1] * irL, wtL = uL[v2] * irL;
+    const double unR = uR[v0] * irR, vtR = uR[v1] * irR, wtR = uR[v2] * irR;
+    const double vsqL = unL * unL + vtL * vtL + wtL * wtL, vsqR = unR * unR + vtR * vtR + wtR * wtR;
+    const double pL = gm1 * (uL[4] - 0.5 * rhoL * vsqL), pR = gm1 * (uR[4] - 0.5 * rhoR * vsqR);
+    const double aL = sqrt(g * pL * irL), aR = sqrt(g * pR * irR);
+    const double hL = (uL[4] + pL) * irL, hR = (uR[4] + pR) * irR;
+    // Batten speeds from Roe averages
+    const double srL = sqrt(rhoL), srR = sqrt(rhoR), iS = 1.0 / (srL + srR);
+    const double uTil = (unL * srL + unR * srR) * iS, vTil = (vtL * srL + vtR * srR) * iS;
+    const double wTil = (wtL * srL + wtR * srR) * iS, hTil = (hL * srL + hR * srR) * iS;
+    const double vsqTil = uTil * uTil + vTil * vTil + wTil * wTil;
+    const double aTil = sqrt(gm1 * (hTil - 0.5 * vsqTil));
+    const double sL = fmin(unL - aL, uTil - aTil), sR = fmax(unR + aR, uTil + aTil);
+    const double sMax = fmax(fabs(sL), fabs(sR));
     double M = fmax(sqrt(vsqL) / aL, sqrt(vsqR) / aR);
-    const double rhoL = vL[0], rhoR = vR[0], pL = vL[4], pR = vR[4], unL = vL[v0], unR = vR[v0];
-    const double sS = ((pR - pL) + (rhoL * unL * (sL - unL) - rhoR * unR * (sR - unR))) /
-                      (rhoL * (sL - unL) - rhoR * (sR - unR));
     const double psL0 = rhoL * (sL - unL), psR0 = rhoR * (sR - unR);
+    const double sS = ((pR - pL) + (psL0 * unL - psR0 * unR)) / (psL0 - psR0);
     const double pS = 0.5 * ((pL + pR) + psL0 * (sS - unL) + psR0 * (sS - unR));
-    const double sFacL = sS / (sL - sS), sFacR = sS / (sR - sS);
-    const double uJump = vR[v0] - vL[v0], vJump = vR[v1] - vL[v1], wJump = vR[v2] - vL[v2];
-    const double psiL = uL[0] * (sL - vL[v0]), psiR = uR[0] * (sR - vR[v0]),
-                 psiFac = psiR * psiL / (psiR - psiL);
-    const double presRat = fmin(vL[4] / vR[4], vR[4] / vL[4]);
+    const double iL = 1.0 / (sL - sS), iR = 1.0 / (sR - sS);
+    const double sFacL = sS * iL, sFacR = sS * iR;
+    const double uJump = unR - unL, vJump = vtR - vtL, wJump = wtR - wtL;
+    const double psiFac = psR0 * psL0 / (psR0 - psL0);
+    const double presRat = pL < pR ? pL / pR : pR / pL;
     const double gFac = 1.0 - powu(presRat, M);  // presRat in (0, 1], M >= 0
-    const double shFacL = psiFac * gFac * sL / (sL - sS), shFacR = psiFac * gFac * sR / (sR - sS);
+    const double shFacL = psiFac * gFac * sL * iL, shFacR = psiFac * gFac * sR * iR;
     M = fmin(M, 1.0);
-    const double oneMsq = 1.0 - M * M;  // pow(x, 2.0) is the correctly rounded x * x
+    const double oneMsq = 1.0 - M * M;
     const double fM = M * sqrt(4.0 + oneMsq * oneMsq) / (1.0 + M * M);
-    const double pFacL = psiFac * (fM - 1.0) * uJump + sL * pS / (sL - sS);
-    const double pFacR = psiFac * (fM - 1.0) * uJump + sR * pS / (sR - sS);
-    if (0 <= sL) {
-      flux<DIR>(uL, vL, F);
-    } else if (0 <= sS) {
-      flux<DIR>(uL, vL, F);
-      F[0] = sFacL * (sL * uL[0] - F[0]);
-      F[v0] = sFacL * (sL * uL[v0] - F[v0]) + pFacL;
-      F[v1] = sFacL * (sL * uL[v1] - F[v1]) + shFacL * vJump;
-      F[v2] = sFacL * (sL * uL[v2] - F[v2]) + shFacL * wJump;
-      F[4] = sFacL * (sL * uL[4] - F[4]) + sS * pFacL;
-    } else if (0 < sR) {
-      flux<DIR>(uR, vR, F);
-      F[0] = sFacR * (sR * uR[0] - F[0]);
-      F[v0] = sFacR * (sR * uR[v0] - F[v0]) + pFacR;
-      F[v1] = sFacR * (sR * uR[v1] - F[v1]) + shFacR * vJump;
-      F[v2] = sFacR * (sR * uR[v2] - F[v2]) + shFacR * wJump;
-      F[4] = sFacR * (sR * uR[4] - F[4]) + sS * pFacR;
-    } else {
-      flux<DIR>(uR, vR, F);
+    const double pFacL = psiFac * (fM - 1.0) * uJump + sL * pS * iL;
+    const double pFacR = psiFac * (fM - 1.0) * uJump + sR * pS * iR;
+    // regions of the reference: 0 <= sL: F(uL); else 0 <= sS: star L;
+    // else 0 < sR: star R; else F(uR)
+    const bool left = 0.0 <= sL || 0.0 <= sS;
+    const double *u = left ? uL : uR;
+    const double un = left ? unL : unR, vt = left ? vtL : vtR, wt = left ? wtL : wtR;
+    const double p = left ? pL : pR;
+    F[0] = u[v0];
+    F[v0] = u[v0] * un + p;
+    F[v1] = u[v0] * vt;
+    F[v2] = u[v0] * wt;
+    F[4] = un * (u[4] + p);
+    if (left ? !(0.0 <= sL) : 0.0 < sR) {
+      const double sK = left ? sL : sR, sFac = left ? sFacL : sFacR;
+      const double shFac = left ? shFacL : shFacR, pFac = left ? pFacL : pFacR;
+      F[0] = sFac * (sK * u[0] - F[0]);
+      F[v0] = sFac * (sK * u[v0] - F[v0]) + pFac;
+      F[v1] = sFac * (sK * u[v1] - F[v1]) + shFac * vJump;
+      F[v2] = sFac * (sK * u[v2] - F[v2]) + shFac * wJump;
+      F[4] = sFac * (sK * u[4] - F[4]) + sS * pFac;
     }
     return sMax;
   }

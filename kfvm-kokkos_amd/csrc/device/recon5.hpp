@@ -219,8 +219,10 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
       // Hydro::eint / pres / weno of the point (Hydro_EOS.H)
       const double ei = t[q] - 0.5 * msq[q] / rho[q];
       okE = okE && ei > pb.minEint;
-      WI[size_t(ax * P + q) * nc + cell] = (gm1 * ei) / pow(rho[q], g.gamma);
-      __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
+      if (WI) {  // the KXRCF indicator's input; not needed by the learned selection
+        WI[size_t(ax * P + q) * nc + cell] = (gm1 * ei) / pow(rho[q], g.gamma);
+        __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
+      }
     }
   }
   // Cells whose states violate a bound go to a list for k_pp_lanes (the

@@ -779,6 +779,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
 
 #include "recon5.hpp"
 #include "mhd.hpp"
+#include "indicator.hpp"
 
 namespace kfvm {
 namespace dev {

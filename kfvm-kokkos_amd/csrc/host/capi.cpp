@@ -5,11 +5,16 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <vector>
 
+#include "indicator.hpp"
 #include "solver.hpp"
 
 struct kfvm_solver {
   kfvm::Solver *s;
+};
+struct kfvm_indicator {
+  kfvm::Indicator *t;
 };
 
 namespace {
@@ -44,6 +49,8 @@ int create(const kfvm::Config &cfg, const std::string &warnings, kfvm_solver **o
 
 #define CHECK_SOLVER(s) \
   if (!(s) || !(s)->s) return fail("null solver")
+#define CHECK_IND(p) \
+  if (!(p) || !(p)->t) return fail("null indicator")
 
 int64_t state_size(const kfvm::Solver &s) { return s.ncell() * s.nv(); }
 }  // namespace
@@ -210,6 +217,140 @@ int kfvm_get_counters(kfvm_solver *s, int64_t *rhs_evals, int64_t *rejects_tol,
   return 0;
 }
 
+int kfvm_set_indicator(kfvm_solver *s, const uint16_t *weights, int64_t n, double threshold) {
+  CHECK_SOLVER(s);
+  if (n < 0 || (n > 0 && !weights)) return fail("null indicator weights");
+  return guarded([&] {
+    std::vector<float> P(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; i++) {
+      const uint32_t u = uint32_t(weights[i]) << 16;
+      std::memcpy(&P[size_t(i)], &u, 4);
+    }
+    s->s->setIndicator(P.data(), int(n), threshold);
+    return 0;
+  });
+}
+
+int kfvm_set_nad(kfvm_solver *s, int on, double range_frac, double value_frac, double max_frac) {
+  CHECK_SOLVER(s);
+  if (!(range_frac >= 0.0) || !(value_frac >= 0.0) || !(max_frac >= 0.0))
+    return fail("NAD tolerances must be >= 0");
+  s->s->setNad(on != 0, range_frac, value_frac, max_frac);
+  return 0;
+}
+
+int kfvm_get_indicator_counters(kfvm_solver *s, int64_t *nad_fallbacks, int64_t *nad_cells, int *learned) {
+  CHECK_SOLVER(s);
+  if (nad_fallbacks) *nad_fallbacks = s->s->nNadFallback;
+  if (nad_cells) *nad_cells = s->s->nNadCells;
+  if (learned) *learned = s->s->learned() ? 1 : 0;
+  return 0;
+}
+
+int kfvm_indicator_create(int dim, kfvm_indicator **out) {
+  if (!out) return fail("null output pointer");
+  *out = nullptr;
+  return guarded([&] {
+    kfvm::Indicator *t = new kfvm::Indicator(dim);
+    *out = new kfvm_indicator{t};
+    return 0;
+  });
+}
+
+int kfvm_indicator_destroy(kfvm_indicator *p) {
+  if (!p) return 0;
+  return guarded([&] {
+    delete p->t;
+    delete p;
+    return 0;
+  });
+}
+
+int kfvm_indicator_collect(kfvm_indicator *p, kfvm_solver *s, double tau, int64_t *nsamples,
+                           int64_t *npositive) {
+  CHECK_IND(p);
+  CHECK_SOLVER(s);
+  return guarded([&] {
+    p->t->collect(*s->s, tau);
+    if (nsamples) *nsamples = p->t->samples();
+    if (npositive) *npositive = p->t->positives();
+    return 0;
+  });
+}
+
+int kfvm_indicator_samples(kfvm_indicator *p, float *X, float *Y, int64_t offset, int64_t n) {
+  CHECK_IND(p);
+  return guarded([&] {
+    p->t->getSamples(X, Y, offset, n);
+    return 0;
+  });
+}
+
+int kfvm_indicator_init(kfvm_indicator *p, uint64_t seed) {
+  CHECK_IND(p);
+  return guarded([&] {
+    p->t->init(seed);
+    return 0;
+  });
+}
+
+int kfvm_indicator_train(kfvm_indicator *p, int iters, int batch, double lr, double pos_weight, uint64_t seed,
+                         double *loss) {
+  CHECK_IND(p);
+  return guarded([&] {
+    const double l = p->t->train(iters, batch, lr, pos_weight, seed);
+    if (loss) *loss = l;
+    return 0;
+  });
+}
+
+int kfvm_indicator_get_params(kfvm_indicator *p, float *params, int64_t n) {
+  CHECK_IND(p);
+  if (!params || n != p->t->params()) return fail("parameter buffer has the wrong size");
+  return guarded([&] {
+    p->t->getParams(params);
+    return 0;
+  });
+}
+
+int kfvm_indicator_set_params(kfvm_indicator *p, const float *params, int64_t n) {
+  CHECK_IND(p);
+  if (!params || n != p->t->params()) return fail("parameter buffer has the wrong size");
+  return guarded([&] {
+    p->t->setParams(params);
+    return 0;
+  });
+}
+
+int kfvm_indicator_get_weights(kfvm_indicator *p, uint16_t *weights, int64_t n) {
+  CHECK_IND(p);
+  if (!weights || n != p->t->params()) return fail("weight buffer has the wrong size");
+  return guarded([&] {
+    p->t->getBf16(weights);
+    return 0;
+  });
+}
+
+int kfvm_indicator_gradient(kfvm_indicator *p, const float *params, const float *X, const float *Y, int64_t n,
+                            double pos_weight, float *grad, double *loss) {
+  CHECK_IND(p);
+  if (!params || !X || !Y || !grad || n <= 0 || n > (int64_t(1) << 24)) return fail("bad gradient arguments");
+  return guarded([&] {
+    const double l = p->t->gradient(params, X, Y, n, pos_weight, grad);
+    if (loss) *loss = l;
+    return 0;
+  });
+}
+
+int kfvm_indicator_logits(kfvm_indicator *p, const float *params, const float *X, int64_t n, float *z) {
+  CHECK_IND(p);
+  if (!params || !X || !z || n <= 0 || n > (int64_t(1) << 28)) return fail("bad logits arguments");
+  return guarded([&] {
+    p->t->logits(params, X, n, z);
+    return 0;
+  });
+}
+
 int kfvm_force_full_weno(kfvm_solver *s) {
   CHECK_SOLVER(s);
   s->s->useSparseWeno = false;
@@ -327,6 +468,14 @@ int kfvm_diff_matrix(int nq, double *dm, int64_t n) {
     std::memcpy(dm, D.data(), D.size() * sizeof(double));
     return 0;
   });
+}
+
+int kfvm_indicator_sizes(int dim, int64_t *ninputs, int64_t *nparams) {
+  if (dim != 2 && dim != 3) return fail("dim must be 2 or 3");
+  const int C = dim == 3 ? 33 : 13, H = 32, nin = 2 * C;
+  if (ninputs) *ninputs = nin;
+  if (nparams) *nparams = H * nin + H + H * H + H + H + 1;
+  return 0;
 }
 
 int kfvm_parse_init(const char *init_text, char *json, int64_t len) {

@@ -149,7 +149,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
 Solver::~Solver() {
   profileEnable(false);
   void *bufs[] = {SRC_, cellW_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -263,15 +263,24 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
   }
   lastRhsFull_ = !useSparseWeno;
   if (useSparseWeno) {
+    // the learned selection does not read the KXRCF indicator
+    double *wi = (learned_ && !forcePlain_ && hydro) ? nullptr : WI_;
     timed("linear", [&] {
-      return launch_linear(dim, nq, nv_, Uin, FS_, WI_, thNew, props_, grid_, faceW_, faceWT_, ppList_, counter_ + 1,
+      return launch_linear(dim, nq, nv_, Uin, FS_, wi, thNew, props_, grid_, faceW_, faceWT_, ppList_, counter_ + 1,
                            stream_);
     });
     if (hydro) {
-      const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
-      timed("select", [&] {
-        return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
-      });
+      if (learned_ && !forcePlain_) {
+        timed("select", [&] {
+          return launch_learned_select(dim, Uin, props_, thPrev, nadFlag_, netP_, netThr_, wflag_, list_, counter_,
+                                       grid_, stream_);
+        });
+      } else {
+        const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
+        timed("select", [&] {
+          return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
+        });
+      }
       timed("weno", [&] {
         return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, list_,
                            counter_, stream_);
@@ -400,8 +409,13 @@ int Solver::takeStep() {
       useSparseWeno = false;
       nRejectUnphys++;
       lastTimeStep = false;
+    } else if (lStat == ACCEPTED && learned_ && !forcePlain_ && nadOn_ && nadFails()) {
+      // a-posteriori check of the learned selection: redo the step with KXRCF
+      forcePlain_ = true;
+      nNadFallback++;
     } else if (lStat == ACCEPTED) {
       accepted = true;
+      forcePlain_ = false;
       time += dt;
       const double dterr = dt * dtfac, dtcfl = cfg_.cfl * dmin / maxVel;
       dt = fixed ? cfg_.fixedDt : std::fmin(dterr, dtcfl);
@@ -425,6 +439,59 @@ int Solver::takeStep() {
   }
   nTS++;
   return status;
+}
+
+bool Solver::nadFails() {
+  check(hipMemsetAsync(nadCount_, 0, sizeof(int), stream_), "clear NAD count");
+  timed("nad", [&] {
+    return launch_nad(cfg_.dim, Uprev_, U_[cur_], nadRange_, nadValue_, nadFlag_, nadCount_, grid_, stream_);
+  });
+  int cnt = 0;
+  check(hipMemcpyAsync(&hostRed_[3], nadCount_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read NAD count");
+  check(hipStreamSynchronize(stream_), "sync");
+  std::memcpy(&cnt, &hostRed_[3], sizeof(int));
+  nNadCells += cnt;
+  return double(cnt) > nadMaxFrac_ * double(ncell_);
+}
+
+void Solver::setIndicator(const float *params, int np, double threshold) {
+  check(hipStreamSynchronize(stream_), "sync");
+  if (np == 0) {
+    learned_ = false;
+    return;
+  }
+  if (nv_ != 5) throw std::runtime_error("the learned indicator is built for the Euler equations");
+  if (np != net_params(cfg_.dim)) throw std::runtime_error("indicator parameter count does not match the dimension");
+  if (!netP_) netP_ = dalloc<float>(size_t(np), "indicator parameters");
+  if (!nadCount_) nadCount_ = dalloc<int>(1, "NAD count");
+  if (!nadFlag_) nadFlag_ = dalloc<double>(size_t(ncell_), "NAD flags");
+  check(hipMemset(nadFlag_, 0, size_t(ncell_) * sizeof(double)), "clear NAD flags");
+  check(hipMemcpy(netP_, params, size_t(np) * sizeof(float), hipMemcpyHostToDevice), "copy indicator");
+  netThr_ = float(threshold);
+  learned_ = true;
+  forcePlain_ = false;
+}
+
+void Solver::collectSamples(float *X, float *Y, double tau) {
+  if (nv_ != 5) throw std::runtime_error("the learned indicator is built for the Euler equations");
+  const int dim = cfg_.dim, nq = cfg_.nQuad;
+  const double *Uin = stateInCur_ ? U_[cur_] : Uprev_;
+  double *thScratch = theta_[th_ ^ 1];
+  const size_t fsBytes = size_t(ncell_) * wts_.F * wts_.Q * nv_ * sizeof(double);
+  if (!FSlin_) FSlin_ = dalloc<double>(fsBytes / sizeof(double), "linear face states");
+  check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
+  check(launch_cell_props(dim, Uin, props_, grid_, stream_), "cell_props");
+  check(launch_linear(dim, nq, nv_, Uin, FS_, WI_, thScratch, props_, grid_, faceW_, faceWT_, ppList_, counter_ + 1,
+                      stream_),
+        "linear");
+  check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
+  check(launch_weno(dim, nq, nv_, Uin, FS_, thScratch, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr, nullptr,
+                    stream_),
+        "weno");
+  check(launch_weno_post(dim, nq, Uin, FS_, thScratch, props_, grid_, nullptr, nullptr, stream_), "weno_post");
+  check(launch_labels(dim, nq, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
+  check(launch_features(dim, Uin, props_, X, grid_, stream_), "features");
+  check(hipStreamSynchronize(stream_), "sync");
 }
 
 void Solver::advanceStages(int64_t n, double dtFixed) {

@@ -51,6 +51,28 @@ class Solver {
   int64_t nTS = 1, nRhsEval = 0, nRejectUnphys = 0, nRejectThresh = 0, lastNWeno = 0;
   bool lastTimeStep = false, useSparseWeno = true;
 
+  // Learned troubled-cell indicator (Euler).  params: np FP32 network
+  // parameters holding bfloat16 values, np == 0 returns to the KXRCF selector.
+  // While it is on, every accepted step is checked for numerical
+  // admissibility: the cells that fail get WENO in every stage of the next
+  // step, and a step in which more than maxFrac of the cells fail is redone
+  // with the KXRCF selector.
+  void setIndicator(const float *params, int np, double threshold);
+  // Admissibility check of learned steps: on / off, tolerance
+  // delta = max(valueFrac |bound|, rangeFrac (max - min)), redo fraction.
+  void setNad(bool on, double rangeFrac, double valueFrac, double maxFrac) {
+    nadOn_ = on;
+    nadRange_ = rangeFrac;
+    nadValue_ = valueFrac;
+    nadMaxFrac_ = maxFrac;
+  }
+  int64_t nNadCells = 0;  // violating cells summed over checked steps
+  bool learned() const { return learned_; }
+  int64_t nNadFallback = 0;
+  // Training samples of the current state: X[ncell][nin] features and Y[ncell]
+  // labels (device pointers), labels with threshold tau.
+  void collectSamples(float *X, float *Y, double tau);
+
   // per-kernel event timing
   void profileEnable(bool on);
   void profileReset();
@@ -70,6 +92,7 @@ class Solver {
   void pipelineMhd(const double *Uin, const RKArgs &a);
   void uploadWeights();
   double readSpeed();
+  bool nadFails();
 
   Config cfg_;
   RKTable rk_;
@@ -101,6 +124,16 @@ class Solver {
   bool lastRhsFull_ = false;  // the last RHS evaluation ran WENO on every cell
   bool stateInCur_ = false;  // last step was not accepted: the solution is in U_[cur_]
   int stagePos_ = 0;         // position inside a step for advanceStages
+
+  // learned indicator: network parameters, logit threshold, NAD counter, and
+  // the linear face states kept while labels are collected
+  float *netP_ = nullptr;
+  float netThr_ = 0.0f;
+  bool learned_ = false, forcePlain_ = false, nadOn_ = true;
+  double nadRange_ = 0.05, nadValue_ = 1e-3, nadMaxFrac_ = 0.02;
+  double *nadFlag_ = nullptr;  // cells that failed the last check
+  int *nadCount_ = nullptr;
+  double *FSlin_ = nullptr;
 
   bool profOn_ = false;
   std::map<std::string, Prof> prof_;

@@ -30,7 +30,11 @@ EXPORTS = [
     "kfvm_force_full_weno", "kfvm_get_weights", "kfvm_set_weights", "kfvm_profile_enable",
     "kfvm_profile_get", "kfvm_profile_reset", "kfvm_sync", "kfvm_weight_sizes",
     "kfvm_build_weights", "kfvm_build_cell_weights", "kfvm_diff_matrix", "kfvm_parse_init",
-    "kfvm_last_error",
+    "kfvm_last_error", "kfvm_set_indicator", "kfvm_get_indicator_counters", "kfvm_indicator_create",
+    "kfvm_indicator_destroy", "kfvm_indicator_collect", "kfvm_indicator_samples", "kfvm_indicator_init",
+    "kfvm_indicator_train", "kfvm_indicator_get_params", "kfvm_indicator_set_params",
+    "kfvm_indicator_get_weights", "kfvm_indicator_gradient", "kfvm_indicator_logits",
+    "kfvm_indicator_sizes", "kfvm_set_nad",
 ]
 
 
@@ -56,6 +60,8 @@ def load_library(path=None):
     P = ctypes.c_void_p
     D = ctypes.POINTER(ctypes.c_double)
     I64 = ctypes.c_int64
+    F32 = ctypes.POINTER(ctypes.c_float)
+    U16 = ctypes.POINTER(ctypes.c_uint16)
     sig = {
         "kfvm_create": [ctypes.c_char_p, ctypes.POINTER(P)],
         "kfvm_create_from_text": [ctypes.c_char_p, ctypes.POINTER(P)],
@@ -85,6 +91,23 @@ def load_library(path=None):
         "kfvm_build_cell_weights": [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [D, I64],
         "kfvm_diff_matrix": [ctypes.c_int, D, I64],
         "kfvm_parse_init": [ctypes.c_char_p, ctypes.c_char_p, I64],
+        "kfvm_set_indicator": [P, U16, I64, ctypes.c_double],
+        "kfvm_get_indicator_counters": [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(ctypes.c_int)],
+        "kfvm_set_nad": [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "kfvm_indicator_create": [ctypes.c_int, ctypes.POINTER(P)],
+        "kfvm_indicator_destroy": [P],
+        "kfvm_indicator_collect": [P, P, ctypes.c_double, ctypes.POINTER(I64), ctypes.POINTER(I64)],
+        "kfvm_indicator_samples": [P, F32, F32, I64, I64],
+        "kfvm_indicator_init": [P, ctypes.c_uint64],
+        "kfvm_indicator_train": [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)],
+        "kfvm_indicator_get_params": [P, F32, I64],
+        "kfvm_indicator_set_params": [P, F32, I64],
+        "kfvm_indicator_get_weights": [P, U16, I64],
+        "kfvm_indicator_gradient": [P, F32, F32, F32, I64, ctypes.c_double, F32,
+                                    ctypes.POINTER(ctypes.c_double)],
+        "kfvm_indicator_logits": [P, F32, F32, I64, F32],
+        "kfvm_indicator_sizes": [ctypes.c_int, ctypes.POINTER(I64), ctypes.POINTER(I64)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -141,6 +164,118 @@ def diff_matrix(nq):
     m = np.empty((nq, nq + 2))
     _check(lib, lib.kfvm_diff_matrix(nq, _dptr(m), m.size), "kfvm_diff_matrix")
     return m
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u16ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+
+
+def indicator_sizes(dim):
+    """(features per cell, network parameters) of the learned indicator."""
+    lib = load_library()
+    ni, npar = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib, lib.kfvm_indicator_sizes(dim, ctypes.byref(ni), ctypes.byref(npar)), "kfvm_indicator_sizes")
+    return ni.value, npar.value
+
+
+def bf16_to_float(w):
+    """bfloat16 bit patterns (uint16) -> float32 values."""
+    return (np.asarray(w, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def float_to_bf16(p):
+    """float32 -> bfloat16 bit patterns, round to nearest even (finite inputs)."""
+    u = np.ascontiguousarray(p, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+class Indicator:
+    """Training set and trainer of the learned troubled-cell indicator (GPU)."""
+
+    def __init__(self, dim):
+        self._lib = load_library()
+        self._h = ctypes.c_void_p()
+        _check(self._lib, self._lib.kfvm_indicator_create(dim, ctypes.byref(self._h)), "kfvm_indicator_create")
+        self.dim = dim
+        self.nin, self.np = indicator_sizes(dim)
+        self.nsamples = self.npositive = 0
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, name)(*args)
+        if rc != 0:
+            raise KfvmError(f"{name}: {self._lib.kfvm_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.kfvm_indicator_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def collect(self, solver, tau):
+        n, npos = ctypes.c_int64(), ctypes.c_int64()
+        self._call("kfvm_indicator_collect", self._h, solver._h, float(tau), ctypes.byref(n), ctypes.byref(npos))
+        self.nsamples, self.npositive = n.value, npos.value
+        return n.value, npos.value
+
+    def samples(self, offset, n):
+        X = np.empty((n, self.nin), dtype=np.float32)
+        Y = np.empty(n, dtype=np.float32)
+        self._call("kfvm_indicator_samples", self._h, _fptr(X), _fptr(Y), int(offset), int(n))
+        return X, Y
+
+    def init(self, seed):
+        self._call("kfvm_indicator_init", self._h, int(seed))
+
+    def train(self, iters, batch, lr, pos_weight=1.0, seed=0):
+        loss = ctypes.c_double()
+        self._call("kfvm_indicator_train", self._h, int(iters), int(batch), float(lr), float(pos_weight),
+                   int(seed), ctypes.byref(loss))
+        return loss.value
+
+    @property
+    def params(self):
+        P = np.empty(self.np, dtype=np.float32)
+        self._call("kfvm_indicator_get_params", self._h, _fptr(P), P.size)
+        return P
+
+    @params.setter
+    def params(self, P):
+        a = np.ascontiguousarray(P, dtype=np.float32)
+        self._call("kfvm_indicator_set_params", self._h, _fptr(a), a.size)
+
+    @property
+    def weights(self):
+        """bfloat16 parameters (uint16 bit patterns) the forward pass uses."""
+        W = np.empty(self.np, dtype=np.uint16)
+        self._call("kfvm_indicator_get_weights", self._h, _u16ptr(W), W.size)
+        return W
+
+    def gradient(self, P, X, Y, pos_weight=1.0):
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.empty(self.np, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._call("kfvm_indicator_gradient", self._h, _fptr(P), _fptr(X), _fptr(Y), len(Y), float(pos_weight),
+                   _fptr(g), ctypes.byref(loss))
+        return g, loss.value
+
+    def logits(self, P, X):
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        z = np.empty(len(X), dtype=np.float32)
+        self._call("kfvm_indicator_logits", self._h, _fptr(P), _fptr(X), len(X), _fptr(z))
+        return z
 
 
 def parse_init(text):
@@ -213,6 +348,26 @@ class Solver:
 
     def force_full_weno(self):
         self._call("kfvm_force_full_weno", self._h)
+
+    def set_indicator(self, weights=None, threshold=0.0):
+        """Learned WENO selection with bfloat16 weights (uint16); None: KXRCF."""
+        if weights is None:
+            self._call("kfvm_set_indicator", self._h, None, 0, 0.0)
+            return
+        w = np.ascontiguousarray(weights, dtype=np.uint16)
+        self._call("kfvm_set_indicator", self._h, _u16ptr(w), w.size, float(threshold))
+
+    def set_nad(self, on=True, range_frac=0.05, value_frac=1e-3, max_frac=0.02):
+        """Admissibility check of learned steps: tolerance max(value_frac |bound|,
+        range_frac range); a step with more than max_frac failing cells is redone."""
+        self._call("kfvm_set_nad", self._h, 1 if on else 0, float(range_frac), float(value_frac),
+                   float(max_frac))
+
+    @property
+    def indicator_counters(self):
+        n, cells, on = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        self._call("kfvm_get_indicator_counters", self._h, ctypes.byref(n), ctypes.byref(cells), ctypes.byref(on))
+        return {"nad_fallbacks": n.value, "nad_cells": cells.value, "learned": bool(on.value)}
 
     # ---- data
     @property

@@ -1,0 +1,129 @@
+"""Train the learned troubled-cell indicator on the GPU.
+
+Labels come from the solver itself: snapshots of plain (KXRCF) runs are
+reconstructed once linearly and once with full WENO (+ limiter), and a cell is
+labelled 1 where the two differ by more than tau of its stencil's range
+(kfvm_indicator_collect).  The network (2C features -> 32 -> 32 -> logit,
+bfloat16 weights) is trained by the library's own forward / backward / Adam
+kernels.  The weights are written as a .npy of bfloat16 bit patterns (uint16)
+plus a .json of the recipe and the training statistics.
+
+    python -m kfvm_amd.train_indicator euler3d [--out DIR]
+    python -m kfvm_amd.train_indicator shear2d [--out DIR]
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+import kfvm_amd
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "weights")
+
+
+def init_text(dim, n, problem, user=(), seed=0, bc="outflow", final=1.0, gamma=1.4, wenothresh=1.0,
+              nq=2, cfl=None):
+    lines = ["mesh:resolution " + " ".join(str(x) for x in n[:dim]),
+             "mesh:extent " + " ".join("0.0 1.0" for _ in range(dim)),
+             "stencil:lfac 5", f"time:final {final!r}", "time:atol 1.e-3", "time:rtol 1.e-3",
+             f"eos:gamma {gamma!r}", f"eos:wenothresh {wenothresh!r}", f"problem:name {problem}",
+             f"problem:dim {dim}", f"problem:nquad {nq}", "problem:equation hydro", "problem:riemann hllc",
+             "problem:rk ssp4_3_2", f"problem:bc {bc}", f"user:seed {seed}"]
+    if cfl is not None:
+        lines.append(f"time:cfl {cfl!r}")
+    for i, u in enumerate(user):
+        lines.append(f"user:p{i} {u!r}")
+    return "\n".join(lines) + "\n"
+
+
+# Training problems: never the evaluation problem itself.
+RECIPES = {
+    # 3D Euler planar Riemann problems (the evaluation is the x+y+z = 1.5 plane at
+    # 256^3): other plane offsets at 64^3, snapshots every 4th step of 24
+    "euler3d": dict(dim=3, tau=0.15, iters=2000, batch=8192, lr=2e-3, seed=1,
+                    runs=[dict(n=(64, 64, 64), problem="riemann3d_planar", user=(off,), bc="outflow",
+                               steps=24, every=4) for off in (1.2, 1.4, 1.65, 1.8)]),
+    # 2D shear layer (the evaluation is 1024^2): 256^2 and 512^2, seeds 0-4,
+    # snapshots every 0.1 up to t = 1
+    "shear2d": dict(dim=2, tau=0.15, iters=2000, batch=8192, lr=2e-3, seed=1,
+                    runs=[dict(n=(m, m), problem="shear2d", user=(), bc="periodic", seed=sd, t_end=1.0,
+                               dt_snap=0.1) for m in (256, 512) for sd in range(5)]),
+}
+
+
+def collect(ind, run, tau, dim):
+    text = init_text(dim, run["n"], run["problem"], run.get("user", ()), run.get("seed", 0), run["bc"],
+                     final=run.get("t_end", 1.0))
+    s = kfvm_amd.Solver(init_text=text)
+    if "steps" in run:
+        for k in range(run["steps"]):
+            s.take_step()
+            if k % run["every"] == run["every"] - 1:
+                ind.collect(s, tau)
+    else:
+        nxt = run["dt_snap"]
+        while True:
+            st = s.take_step()
+            t = s.time[0]
+            if t >= nxt - 1e-12:
+                ind.collect(s, tau)
+                nxt += run["dt_snap"]
+            if t >= run["t_end"] - 1e-12 or st not in (0, 1, 2):
+                break
+    s.close()
+
+
+def train(name, out_dir=WEIGHTS, log=print):
+    r = RECIPES[name]
+    dim = r["dim"]
+    t0 = time.time()
+    ind = kfvm_amd.Indicator(dim)
+    for run in r["runs"]:
+        collect(ind, run, r["tau"], dim)
+    total = ind.nsamples
+    X, Y = ind.samples(0, total)
+    pos = float(Y.mean())
+    log(f"[train] {name}: {total} samples, {pos:.4f} labelled ({time.time() - t0:.1f} s)")
+    ind.init(r["seed"])
+    pw = (1.0 - pos) / max(pos, 1e-6)
+    loss = ind.train(r["iters"], r["batch"], r["lr"], pos_weight=pw, seed=r["seed"])
+    W = ind.weights
+    P = kfvm_amd.bf16_to_float(W)
+    sub = slice(None, None, max(1, total // 500000))
+    z = ind.logits(P, X[sub])
+    y = Y[sub] > 0.5
+    flagged = z > 0.0
+    stats = {"samples": int(total), "positive_fraction": pos, "pos_weight": pw, "last_batch_loss": loss,
+             "recall": float((flagged & y).sum() / max(1, y.sum())), "flagged_fraction": float(flagged.mean()),
+             "precision": float((flagged & y).sum() / max(1, flagged.sum())),
+             "train_seconds": time.time() - t0}
+    log(f"[train] {name}: loss {loss:.4f}, recall {stats['recall']:.4f}, flagged {stats['flagged_fraction']:.4f}")
+    os.makedirs(out_dir, exist_ok=True)
+    np.save(os.path.join(out_dir, f"{name}.npy"), W)
+    meta = {k: v for k, v in r.items() if k != "runs"}
+    meta["runs"] = [{k: (list(v) if isinstance(v, tuple) else v) for k, v in run.items()} for run in r["runs"]]
+    meta["stats"] = stats
+    with open(os.path.join(out_dir, f"{name}.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+    ind.close()
+    return W, stats
+
+
+def load(name):
+    """bfloat16 weights (uint16) of a trained recipe."""
+    return np.load(os.path.join(WEIGHTS, f"{name}.npy"), allow_pickle=False)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("recipe", choices=sorted(RECIPES))
+    ap.add_argument("--out", default=WEIGHTS)
+    a = ap.parse_args()
+    train(a.recipe, a.out)
+
+
+if __name__ == "__main__":
+    main()

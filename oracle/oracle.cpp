@@ -472,6 +472,7 @@ struct Base {
   virtual void prime() = 0;
   virtual int take_step() = 0;
   virtual void set_mhd_tables(const double *, const double *) {}
+  virtual void face_states(int, double *) {}
   orc_config cfg{};
   double time = 0.0, dt = 0.0, errEst = 1.0;
   int64_t nTS = 1, nRhsEval = 0, nWeno = 0;
@@ -1296,6 +1297,32 @@ struct Solver : Base {
   }
   double eval_rhs(int which) override { return eval_rhs_on(which == 0 ? U : Uprev); }
 
+  // Own face states of every cell from U: linear reconstruction (which = 0) or
+  // full WENO followed by the positivity limiter (which = 1); layout
+  // out[((f*Q + q)*NV + v)*ncell + cell].  The limiter's theta is not kept.
+  void face_states(int which, double *out) override {
+    cell_bcs(U);
+    if (which == 0) {
+      recon_linear(U);
+    } else {
+      const std::vector<double> keep = flag;
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int64_t i = 0; i < nx; i++)
+        for (int64_t j = 0; j < ny; j++)
+          for (int64_t k = 0; k < nz; k++) recon_weno_cell(U, i, j, k);
+      pos_pres(U);
+      flag = keep;
+    }
+    const size_t ncl = size_t(nx) * ny * nz;
+    for (int64_t i = 0; i < nx; i++)
+      for (int64_t j = 0; j < ny; j++)
+        for (int64_t k = 0; k < nz; k++)
+          for (int f = 0; f < F; f++)
+            for (int q = 0; q < Q; q++)
+              for (int v = 0; v < NV; v++)
+                out[(size_t(f * Q + q) * NV + v) * ncl + (size_t(k) * ny + j) * nx + i] = own(f, i, j, k, q)[v];
+  }
+
   // ---- state access
   void set_state(const double *S) override {
     for (int64_t i = 0; i < nx; i++)
@@ -1542,6 +1569,7 @@ int64_t orc_rhs_evals(const orc_solver *s) { return s->impl->nRhsEval; }
 void orc_force_full_weno(orc_solver *s) { s->impl->useSparseWeno = false; }
 int64_t orc_num_weno(const orc_solver *s) { return s->impl->nWeno; }
 int orc_last_step(const orc_solver *s) { return s->impl->lastTimeStep ? 1 : 0; }
+void orc_face_states(orc_solver *s, int which, double *out) { s->impl->face_states(which, out); }
 double orc_riemann_point(int eq, int rs, double gamma, double ch, int dir, const double *uL,
                          const double *uR, double *F) {
   if (eq == 2) {

@@ -66,6 +66,9 @@ int orc_last_step(const orc_solver *s);
 /* GLM-MHD tables: interior-point weights [nSub][NQ^D][C] and the
  * differentiation matrix [NQ][NQ+2]; required before the first RHS. */
 void orc_set_mhd_tables(orc_solver *s, const double *cellW, const double *Dm);
+/* Own face states of every cell from the current state: linear (which = 0) or
+ * full WENO + positivity limiter (which = 1); out[((f*Q + q)*NV + v)*ncell + cell]. */
+void orc_face_states(orc_solver *s, int which, double *out);
 /* One face-point Riemann solve in direction dir (0 x, 1 y, 2 z): eq 0 hydro
  * (rs 0 LLF, 1 HLL, 2 HLLC) or 2 GLM-MHD (rs 0 LLF, 1 HLL, 4 KEPES, cleaning
  * speed ch).  Writes the flux to F and returns the maximum wave speed. */

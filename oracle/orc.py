@@ -76,6 +76,8 @@ def lib():
         L.orc_last_step.restype = ctypes.c_int
         L.orc_set_mhd_tables.argtypes = [P, D, D]
         L.orc_set_mhd_tables.restype = None
+        L.orc_face_states.argtypes = [P, ctypes.c_int, D]
+        L.orc_face_states.restype = None
         L.orc_riemann_point.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, D, D, D]
         L.orc_riemann_point.restype = ctypes.c_double
@@ -180,6 +182,16 @@ class Oracle:
     def flags(self, F):
         a = np.ascontiguousarray(F, dtype=np.float64).reshape((2,) + self.shape[1:])
         self.L.orc_set_flags(self.h, _dp(a))
+
+    def face_states(self, weno=False):
+        """Own face states [F*Q, NV, nz, ny, nx]: linear, or full WENO + limiter."""
+        nc = int(np.prod(self.shape[1:]))
+        F = 2 * self.cfg.dim
+        Q = self.cfg.nq ** (self.cfg.dim - 1)
+        a = np.empty((F * Q, self.nv) + self.shape[1:])
+        assert a.size == F * Q * self.nv * nc
+        self.L.orc_face_states(self.h, 1 if weno else 0, _dp(a))
+        return a
 
     def eval_rhs(self, which=0):
         return self.L.orc_eval_rhs(self.h, which)
