@@ -68,9 +68,10 @@ _SUBSUM = 33 + 7 + 6 * 11                      # 106 stencil columns over all su
 FLOP_WENO = 2 * 19 * 5 * _SUBSUM + 2 * 8 * 19 * 5 + 2 * 6 * 4 * 5 * _SUBSUM  # 47,100
 FLOP_LINEAR = 2 * 6 * 4 * 5 * 33               # 7,920
 BYTES_STAGE = 40 * 5                           # read U_s, Uprev, Uhat; write U_s+1, Uhat
-# HBM bytes per launch of a kernel from the FETCH_SIZE / WRITE_SIZE counters at
-# 256^3 (profiles/flux_rk_traffic_riemann3d_256.md); None where not measured.
-TRAFFIC = {}
+# HBM bytes per launch of a kernel at 256^3, NQ = 2, from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes over this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
+# factors a plain copy of known size gives); None where not measured.
+TRAFFIC = {"flux_rk": 25.72e9}
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -143,7 +144,7 @@ def _mode_run(n, weights, a, profile):
             ms, cnt = s.profile_get(name)
             if cnt:
                 kern[name] = {"ms_total": ms, "launches": cnt, "ms_avg": ms / cnt}
-    while s.time[0] < 0.2 - 1e-12:
+    while s.time[0] < 0.2 - 1e-12 and not a.no_l1:
         if s.take_step() not in (0, 1, 2):
             break
     l1 = planar_riemann_l1(s.state[0], s.time[0])
@@ -166,6 +167,8 @@ def main():
     ap.add_argument("--wenothresh", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
+    ap.add_argument("--learned-only", action="store_true")
+    ap.add_argument("--no-l1", action="store_true", help="stop after the timed window (profiling runs)")
     ap.add_argument("--retrain", action="store_true", help="train the indicator on the GPU first")
     ap.add_argument("--profile-steps", type=int, default=1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -199,9 +202,11 @@ def main():
                                                log=lambda m: print(m, flush=True))
         else:
             weights = train_indicator.load("euler3d")
-    plain, kp = _mode_run(n, None, a, a.profile_steps if weights is None else 0)
-    print(f"[bench] plain: {plain['value']:.1f} Mzone-updates/s, WENO {plain['weno_fraction']:.3f}, "
-          f"L1 {plain['l1_rho_t0.2']:.5e}", flush=True)
+    plain, kp = None, {}
+    if not a.learned_only or weights is None:
+        plain, kp = _mode_run(n, None, a, a.profile_steps if weights is None else 0)
+        print(f"[bench] plain: {plain['value']:.1f} Mzone-updates/s, WENO {plain['weno_fraction']:.3f}, "
+              f"L1 {plain['l1_rho_t0.2']:.5e}", flush=True)
     learned, kl = (None, {})
     if weights is not None:
         learned, kl = _mode_run(n, weights, a, a.profile_steps)
@@ -224,7 +229,7 @@ def main():
         roof = {"bound": "hbm", "achieved": BYTES_STAGE * ncell / (d["ms_avg"] * 1e-3) / 1e9,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = TRAFFIC.get(dom)
+    roof["traffic"] = TRAFFIC.get(dom) if n == 256 else None
     roof["kernel"] = dom
 
     cpu = None if a.no_cpu else cpu_baseline()
@@ -247,8 +252,8 @@ def main():
                    "cells": ncell, "step": "one RK step (4 stages); value counts zone-updates per stage"},
         "plain": plain,
         "learned": learned,
-        "speedup_learned_vs_plain": (learned["value"] / plain["value"]) if learned else None,
-        "l1_ratio_learned_vs_plain": (learned["l1_rho_t0.2"] / plain["l1_rho_t0.2"]) if learned else None,
+        "speedup_learned_vs_plain": (learned["value"] / plain["value"]) if learned and plain else None,
+        "l1_ratio_learned_vs_plain": (learned["l1_rho_t0.2"] / plain["l1_rho_t0.2"]) if learned and plain else None,
         "roofline": roof,
         "kernels_ms_avg": {k: v["ms_avg"] for k, v in kern.items()},
         "cpu_baseline": cpu,
