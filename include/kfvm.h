@@ -90,6 +90,15 @@ int kfvm_get_weights(kfvm_solver *s, double *face, int64_t nface, double *deriv,
 int kfvm_set_weights(kfvm_solver *s, const double *face, int64_t nface, const double *deriv,
                      int64_t nderiv);
 
+/* Plot / checkpoint output (WriterPDI::writePlot / writeCkpt,
+ * src/io/WriterPDI.cpp:91-160, datasets of user/pdi_*_xdmf.yml) of the current
+ * state under io:datadir / io:basename; path receives the HDF5 file's name.
+ * kfvm_solve writes them by itself when io:plotfreq / io:ckptfreq are set, and
+ * io:restartfile restarts from a checkpoint (Solver.cpp:60-66).  The HDF5
+ * library (1.10 or newer) is loaded at run time; KFVM_HDF5_LIB names it. */
+int kfvm_write_plot(kfvm_solver *s, char *path, int64_t len);
+int kfvm_write_checkpoint(kfvm_solver *s, char *path, int64_t len);
+
 /* Per-kernel device timing with HIP events on the solver's stream.  While
  * enabled, every launch is bracketed by events; kfvm_profile_get returns the
  * accumulated milliseconds and launch count of one kernel family
@@ -144,6 +153,14 @@ int kfvm_indicator_logits(kfvm_indicator *p, const float *params, const float *X
 
 /* ---- host-only entry points (no device work; usable without a GPU) ---- */
 
+/* The writer without a solver: a plot (checkpoint = 0) or checkpoint file of
+ * host arrays U [NV][nz][ny][nx] (n values) and flags [2][nz][ny][nx] for the
+ * mesh and io: keys of init_text; and the reader of a checkpoint. */
+int kfvm_write_output_file(const char *init_text, int checkpoint, const double *U, const double *flags,
+                           int64_t n, int step, double time, double dt, double err_est, char *path,
+                           int64_t len);
+int kfvm_read_checkpoint_file(const char *init_text, const char *path, double *U, int64_t n, int *step,
+                              double *time, double *dt, double *err_est);
 /* Feature and parameter counts of the learned indicator's network. */
 int kfvm_indicator_sizes(int dim, int64_t *ninputs, int64_t *nparams);
 

@@ -28,7 +28,7 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC"
              "-Wno-pass-failed"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
-HOST_CXX = ["common.cpp", "weights.cpp", "problems.cpp"]
+HOST_CXX = ["common.cpp", "weights.cpp", "problems.cpp", "writer.cpp"]
 HOST_HIP = ["solver.cpp", "indicator.cpp", "capi.cpp"]
 DEVICE = ["dispatch.hip", "inst_d2q2.hip", "inst_d2q3.hip", "inst_d3q2.hip", "inst_d3q3.hip"]
 
@@ -87,7 +87,7 @@ def build(force=False, verbose=False):
     relink = force or jobs or not os.path.exists(LIB)
     if relink:
         tmp = LIB + ".tmp"
-        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lquadmath"])
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lquadmath", "-ldl"])
         os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "host", "main.cpp")
     if relink or _stale(EXE, main_src, deps):

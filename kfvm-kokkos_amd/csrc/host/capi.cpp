@@ -351,6 +351,31 @@ int kfvm_indicator_logits(kfvm_indicator *p, const float *params, const float *X
   });
 }
 
+namespace {
+int copy_path(const std::string &p, char *out, int64_t len) {
+  if (out && len > 0) {
+    const size_t n = std::min<size_t>(p.size(), size_t(len - 1));
+    std::memcpy(out, p.data(), n);
+    out[n] = '\0';
+  }
+  return 0;
+}
+}  // namespace
+
+int kfvm_write_plot(kfvm_solver *s, char *path, int64_t len) {
+  CHECK_SOLVER(s);
+  return guarded([&] {
+    const std::string p = s->s->writePlot();
+    s->s->plotNum++;
+    return copy_path(p, path, len);
+  });
+}
+
+int kfvm_write_checkpoint(kfvm_solver *s, char *path, int64_t len) {
+  CHECK_SOLVER(s);
+  return guarded([&] { return copy_path(s->s->writeCkpt(), path, len); });
+}
+
 int kfvm_force_full_weno(kfvm_solver *s) {
   CHECK_SOLVER(s);
   s->s->useSparseWeno = false;
@@ -466,6 +491,42 @@ int kfvm_diff_matrix(int nq, double *dm, int64_t n) {
   return guarded([&] {
     const std::vector<double> D = kfvm::diff_matrix(nq);
     std::memcpy(dm, D.data(), D.size() * sizeof(double));
+    return 0;
+  });
+}
+
+int kfvm_write_output_file(const char *init_text, int checkpoint, const double *U, const double *flags,
+                           int64_t n, int step, double time, double dt, double err_est, char *path,
+                           int64_t len) {
+  if (!init_text || !U || !flags) return fail("null argument");
+  return guarded([&] {
+    kfvm::Config c;
+    kfvm::parse_init_text(init_text, c, nullptr);
+    if (c.dim == 2) c.nZ = 1;
+    if (n != c.nCells() * kfvm::num_vars(c.eq)) return fail("state buffer has the wrong size");
+    kfvm::Writer w(c);
+    kfvm::OutputFields f;
+    f.U = U;
+    f.flags = flags;
+    const std::string p = checkpoint ? w.writeCkpt(f, step, time, dt, err_est) : w.writePlot(f, step, time);
+    return copy_path(p, path, len);
+  });
+}
+
+int kfvm_read_checkpoint_file(const char *init_text, const char *path, double *U, int64_t n, int *step,
+                              double *time, double *dt, double *err_est) {
+  if (!init_text || !path || !U) return fail("null argument");
+  return guarded([&] {
+    kfvm::Config c;
+    kfvm::parse_init_text(init_text, c, nullptr);
+    if (c.dim == 2) c.nZ = 1;
+    if (n != c.nCells() * kfvm::num_vars(c.eq)) return fail("state buffer has the wrong size");
+    const kfvm::CkptData d = kfvm::Writer(c).readCkpt(path);
+    std::memcpy(U, d.U.data(), d.U.size() * sizeof(double));
+    if (step) *step = d.step;
+    if (time) *time = d.time;
+    if (dt) *dt = d.dt;
+    if (err_est) *err_est = d.errEst;
     return 0;
   });
 }

@@ -136,14 +136,65 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   check(launch_fill(wflag_, 0.0, nc, stream_), "fill wflag");
   check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
 
-  std::vector<double> U0;
-  fill_initial_condition(cfg_, U0);
-  setState(U0.data());
+  const bool restart = !cfg_.restartFile.empty();
+  if (cfg_.plotFreq > 0.0 || cfg_.ckptFreq > 0 || restart) writer_.reset(new Writer(cfg_));
+  if (!restart) {
+    std::vector<double> U0;
+    fill_initial_condition(cfg_, U0);
+    setState(U0.data());
+  } else {
+    // checkpoint restart (Solver.cpp:60-66); errEst and the limiter's theta
+    // are restored too when the file has them
+    const CkptData d = writer_->readCkpt(cfg_.restartFile);
+    setState(d.U.data());
+    time = d.time;
+    nTS = d.step + 1;
+    dt = d.dt;
+    if (d.haveErrEst) errEst = d.errEst;
+    if (!d.theta.empty())
+      check(hipMemcpy(theta_[th_], d.theta.data(), nc * sizeof(double), hipMemcpyHostToDevice), "restore theta");
+    plotNum = cfg_.plotFreq > 0.0 ? int(std::ceil(time / cfg_.plotFreq)) : 0;
+  }
   useSparseWeno = cfg_.wenoThresh > 0.0;
+  if (!restart) {
+    // initial outputs (Solver.cpp:50-59)
+    if (cfg_.plotFreq > 0.0) {
+      writePlot();
+      plotNum++;
+    }
+    if (cfg_.ckptFreq > 0) {
+      const int64_t keep = nTS;
+      nTS = 0;
+      writeCkpt();
+      nTS = keep;
+    }
+  }
 
   // FSAL prime and the small first step (Solver.cpp:73-80)
   const double maxVel = evalRHS(0);
-  dt = 0.01 * cfg_.cfl * cfg_.dmin() / maxVel;
+  if (!restart) dt = 0.01 * cfg_.cfl * cfg_.dmin() / maxVel;
+}
+
+std::string Solver::writePlot() {
+  if (!writer_) writer_.reset(new Writer(cfg_));
+  std::vector<double> U(static_cast<size_t>(ncell_) * nv_), F(2 * static_cast<size_t>(ncell_));
+  getState(U.data());
+  getFlags(F.data());
+  OutputFields f;
+  f.U = U.data();
+  f.flags = F.data();
+  return writer_->writePlot(f, plotNum, time);
+}
+
+std::string Solver::writeCkpt() {
+  if (!writer_) writer_.reset(new Writer(cfg_));
+  std::vector<double> U(static_cast<size_t>(ncell_) * nv_), F(2 * static_cast<size_t>(ncell_));
+  getState(U.data());
+  getFlags(F.data());
+  OutputFields f;
+  f.U = U.data();
+  f.flags = F.data();
+  return writer_->writeCkpt(f, int(nTS), time, dt, errEst);
 }
 
 Solver::~Solver() {
@@ -330,9 +381,14 @@ double Solver::evalRHS(int which) {
 int Solver::takeStep() {
   const bool fixed = cfg_.fixedDt > 0.0;
   if (fixed) dt = cfg_.fixedDt;
+  // limit dt to hit plot times or the final time (Solver.cpp:197-205)
   if (time + dt > cfg_.finalTime) {
     dt = cfg_.finalTime - time;
     lastTimeStep = true;
+    writePlot_ = true;
+  } else if (cfg_.plotFreq > 0.0 && time + dt > plotNum * cfg_.plotFreq) {
+    dt = plotNum * cfg_.plotFreq - time;
+    writePlot_ = true;
   }
   const double dmin = cfg_.dmin();
   const double nDofs = double(ncell_) * nv_;
@@ -408,7 +464,10 @@ int Solver::takeStep() {
       firstUnphys = false;
       useSparseWeno = false;
       nRejectUnphys++;
-      lastTimeStep = false;
+      if (lastTimeStep) {
+        lastTimeStep = false;
+        writePlot_ = false;
+      }
     } else if (lStat == ACCEPTED && learned_ && !forcePlain_ && nadOn_ && nadFails()) {
       // a-posteriori check of the learned selection: redo the step with KXRCF
       forcePlain_ = true;
@@ -425,7 +484,10 @@ int Solver::takeStep() {
     } else {
       dt = dt * dtfac;
       nRejectThresh++;
-      lastTimeStep = false;
+      if (lastTimeStep) {
+        lastTimeStep = false;
+        writePlot_ = false;
+      }
     }
   }
   if (!accepted) {
@@ -543,8 +605,24 @@ void Solver::advanceStages(int64_t n, double dtFixed) {
 }
 
 int64_t Solver::solve() {
+  // Solver::Solve (Solver.cpp:84-140): step, then the outputs that are due
   int64_t n = 0;
-  for (; nTS < cfg_.maxSteps && !lastTimeStep; n++) takeStep();
+  for (; nTS < cfg_.maxSteps && !lastTimeStep; n++) {
+    const int64_t step = nTS;  // the reference advances its counter after the outputs
+    takeStep();
+    const bool last = lastTimeStep || step == cfg_.maxSteps - 1;
+    if (cfg_.plotFreq > 0.0 && (writePlot_ || last)) {
+      writePlot_ = false;
+      writePlot();
+      plotNum++;
+    }
+    if (cfg_.ckptFreq > 0 && (step % cfg_.ckptFreq == 0 || last)) {
+      const int64_t keep = nTS;
+      nTS = step;
+      writeCkpt();
+      nTS = keep;
+    }
+  }
   return n;
 }
 

@@ -11,9 +11,12 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "../device/device_api.hpp"
 #include "common.hpp"
 #include "weights.hpp"
+#include "writer.hpp"
 
 namespace kfvm {
 
@@ -40,6 +43,12 @@ class Solver {
   void setFlags(const double *host);
   void setWeights(const double *face, const double *deriv);
   void sync() const;
+
+  // Plot / checkpoint files of the current state (WriterPDI::writePlot /
+  // writeCkpt); return the path of the HDF5 file.
+  std::string writePlot();
+  std::string writeCkpt();
+  int plotNum = 0;  // index of the next plot (Solver.H plotNum)
 
   const Config &config() const { return cfg_; }
   const WeightTable &weights() const { return wts_; }
@@ -134,6 +143,9 @@ class Solver {
   double *nadFlag_ = nullptr;  // cells that failed the last check
   int *nadCount_ = nullptr;
   double *FSlin_ = nullptr;
+
+  std::unique_ptr<Writer> writer_;
+  bool writePlot_ = false;  // the last step ended on a plot time
 
   bool profOn_ = false;
   std::map<std::string, Prof> prof_;

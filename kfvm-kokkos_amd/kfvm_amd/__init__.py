@@ -34,7 +34,8 @@ EXPORTS = [
     "kfvm_indicator_destroy", "kfvm_indicator_collect", "kfvm_indicator_samples", "kfvm_indicator_init",
     "kfvm_indicator_train", "kfvm_indicator_get_params", "kfvm_indicator_set_params",
     "kfvm_indicator_get_weights", "kfvm_indicator_gradient", "kfvm_indicator_logits",
-    "kfvm_indicator_sizes", "kfvm_set_nad",
+    "kfvm_indicator_sizes", "kfvm_set_nad", "kfvm_write_plot", "kfvm_write_checkpoint",
+    "kfvm_write_output_file", "kfvm_read_checkpoint_file",
 ]
 
 
@@ -94,6 +95,12 @@ def load_library(path=None):
         "kfvm_set_indicator": [P, U16, I64, ctypes.c_double],
         "kfvm_get_indicator_counters": [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(ctypes.c_int)],
         "kfvm_set_nad": [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "kfvm_write_plot": [P, ctypes.c_char_p, I64],
+        "kfvm_write_checkpoint": [P, ctypes.c_char_p, I64],
+        "kfvm_write_output_file": [ctypes.c_char_p, ctypes.c_int, D, D, I64, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_char_p, I64],
+        "kfvm_read_checkpoint_file": [ctypes.c_char_p, ctypes.c_char_p, D, I64, ctypes.POINTER(ctypes.c_int),
+                                      D, D, D],
         "kfvm_indicator_create": [ctypes.c_int, ctypes.POINTER(P)],
         "kfvm_indicator_destroy": [P],
         "kfvm_indicator_collect": [P, P, ctypes.c_double, ctypes.POINTER(I64), ctypes.POINTER(I64)],
@@ -278,6 +285,30 @@ class Indicator:
         return z
 
 
+def write_output_file(init_text, U, flags, step=0, time=0.0, dt=0.0, err_est=1.0, checkpoint=False):
+    """Plot or checkpoint file of host arrays (no solver); returns its path."""
+    lib = load_library()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    F = np.ascontiguousarray(flags, dtype=np.float64)
+    buf = ctypes.create_string_buffer(4096)
+    _check(lib, lib.kfvm_write_output_file(init_text.encode(), 1 if checkpoint else 0, _dptr(U), _dptr(F),
+                                           U.size, int(step), float(time), float(dt), float(err_est), buf,
+                                           len(buf)), "kfvm_write_output_file")
+    return buf.value.decode()
+
+
+def read_checkpoint_file(init_text, path, shape):
+    """(U, step, time, dt, errEst) of a checkpoint; shape = (NV, nz, ny, nx)."""
+    lib = load_library()
+    U = np.empty(shape)
+    st = ctypes.c_int()
+    t, dt, e = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _check(lib, lib.kfvm_read_checkpoint_file(init_text.encode(), path.encode(), _dptr(U), U.size,
+                                              ctypes.byref(st), ctypes.byref(t), ctypes.byref(dt),
+                                              ctypes.byref(e)), "kfvm_read_checkpoint_file")
+    return U, st.value, t.value, dt.value, e.value
+
+
 def parse_init(text):
     """The library's reading of an init file, as a dict."""
     import json
@@ -356,6 +387,16 @@ class Solver:
             return
         w = np.ascontiguousarray(weights, dtype=np.uint16)
         self._call("kfvm_set_indicator", self._h, _u16ptr(w), w.size, float(threshold))
+
+    def write_plot(self):
+        buf = ctypes.create_string_buffer(4096)
+        self._call("kfvm_write_plot", self._h, buf, len(buf))
+        return buf.value.decode()
+
+    def write_checkpoint(self):
+        buf = ctypes.create_string_buffer(4096)
+        self._call("kfvm_write_checkpoint", self._h, buf, len(buf))
+        return buf.value.decode()
 
     def set_nad(self, on=True, range_frac=0.05, value_frac=1e-3, max_frac=0.02):
         """Admissibility check of learned steps: tolerance max(value_frac |bound|,
