@@ -112,6 +112,56 @@ def train(name, out_dir=WEIGHTS, log=print):
     return W, stats
 
 
+EVALUATIONS = {
+    # configuration 3: 2D shear layer at 1024^2, seed 0, periodic, to t = 1
+    "shear2d": dict(dim=2, n=(1024, 1024), problem="shear2d", user=(), bc="periodic", seed=0, t_end=1.0),
+}
+
+
+def evaluate(name, weights=None, n=None, log=print):
+    """Full WENO (the reference's plain WENO), KXRCF-sparse and learned runs of
+    the recipe's evaluation problem to its end time: wall time, zone-updates
+    per second, mean WENO fraction, and each run's L1 distance (mean over
+    cells and variables) from the full-WENO solution."""
+    import kfvm_amd as k
+    e = EVALUATIONS[name]
+    W = load(name) if weights is None else weights
+    size = n or e["n"]
+    res = {}
+    states = {}
+    for mode in ("full", "kxrcf", "learned"):
+        text = init_text(e["dim"], size, e["problem"], e["user"], e["seed"], e["bc"], final=e["t_end"],
+                         wenothresh=-1.0 if mode == "full" else 1.0)
+        s = k.Solver(init_text=text)
+        if mode == "learned":
+            s.set_indicator(W, 0.0)
+        ncell = int(np.prod(size))
+        fr = []
+        s.sync()
+        t0 = time.time()
+        while s.time[0] < e["t_end"] - 1e-12:
+            st = s.take_step()
+            if st not in (0, 1, 2):
+                break
+            fr.append(s.counters["last_nweno"] / ncell)
+        s.sync()
+        el = time.time() - t0
+        c = s.counters
+        states[mode] = s.state
+        res[mode] = {"seconds": el, "rhs_evals": c["rhs_evals"], "steps": s.time[2] - 1,
+                     "mzone_updates_per_s": ncell * c["rhs_evals"] / el / 1e6,
+                     "weno_fraction": float(np.mean(fr)) if fr else 1.0, "t_final": s.time[0]}
+        if mode == "learned":
+            res[mode].update(s.indicator_counters)
+        s.close()
+        log(f"[evaluate] {name} {mode}: {res[mode]}")
+    for mode in ("kxrcf", "learned"):
+        res[mode]["l1_vs_full_weno"] = float(np.abs(states[mode] - states["full"]).mean())
+    res["l1_ratio_learned_vs_kxrcf"] = res["learned"]["l1_vs_full_weno"] / res["kxrcf"]["l1_vs_full_weno"]
+    res["speedup_learned_vs_kxrcf"] = res["learned"]["mzone_updates_per_s"] / res["kxrcf"]["mzone_updates_per_s"]
+    return res
+
+
 def load(name):
     """bfloat16 weights (uint16) of a trained recipe."""
     return np.load(os.path.join(WEIGHTS, f"{name}.npy"), allow_pickle=False)
@@ -121,8 +171,16 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("recipe", choices=sorted(RECIPES))
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--evaluate", action="store_true", help="also run the recipe's evaluation problem")
+    ap.add_argument("--evaluate-only", action="store_true")
     a = ap.parse_args()
-    train(a.recipe, a.out)
+    W = None
+    if not a.evaluate_only:
+        W, _ = train(a.recipe, a.out)
+    if a.evaluate or a.evaluate_only:
+        res = evaluate(a.recipe, W)
+        with open(os.path.join(a.out, f"{a.recipe}_evaluation.json"), "w") as f:
+            json.dump(res, f, indent=1)
 
 
 if __name__ == "__main__":
