@@ -49,6 +49,14 @@ int kfvm_create(const char *init_path, kfvm_solver **out);
 /* Same, from the text of an init file. */
 int kfvm_create_from_text(const char *init_text, kfvm_solver **out);
 int kfvm_destroy(kfvm_solver *s);
+/* A batch of n independent 2D problems of one configuration (same mesh and
+ * scheme), problem k with seed seeds[k], advanced together: every kernel of a
+ * stage covers all of them in one launch, each problem keeps its own step size
+ * and controller (Solver::TakeStep per problem).  State arrays are
+ * [NV][n][ny][nx].  No counterpart in the reference (one problem per run). */
+int kfvm_create_batch(const char *init_text, const uint64_t *seeds, int64_t n, kfvm_solver **out);
+/* Time and next step size of every problem of a batch (n = its size). */
+int kfvm_get_slice_times(kfvm_solver *s, double *t, double *dt, int64_t n);
 
 /* Solver::Solve (src/Solver.cpp:84-140): step until final time / maxsteps. */
 int kfvm_solve(kfvm_solver *s, int64_t *steps_taken);

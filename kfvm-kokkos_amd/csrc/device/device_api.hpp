@@ -41,6 +41,14 @@ struct RKArgs {
   unsigned long long *vmaxBits;  // running max wave speed (bit pattern of a non-negative double)
   const unsigned long long *chBits;  // GLM-MHD: {max(c_f + |u|), max |u|} of this evaluation
   const double *src;                 // GLM-MHD: source terms at the interior points
+  // 2D batch of independent problems (slices): per-slice step sizes; the
+  // epilogue then uses beta * sliceDt[s] and bhat * sliceDt[s] and keeps one
+  // running max wave speed per slice (vmaxBits[s])
+  const double *sliceDt;
+  double beta, bhat;
+  // PRIME / FSAL modes of a batch: write K and Uhat only for slices with
+  // sliceWrite[s] != 0 (the others keep the values of their accepted step)
+  const int *sliceWrite;
 };
 
 // Euler: per-cell internal energy and sound speed, props [2][ncell], read by
@@ -97,6 +105,16 @@ hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
                           const Grid &g, const RKArgs &a, hipStream_t s);
 int flux_rk_partials(int dim, const Grid &g);
+// Batch of slices: per-slice finalize (out[2 s], out[2 s + 1]) over the flux
+// kernel's blocks, which are slice-major (blocksPerSlice each), and the copy of
+// the accepted slices' states (mask[s] != 0) from U to Uprev.
+hipError_t launch_finalize_slices(const double *errPartial, const double *posPartial, int blocksPerSlice,
+                                  int nslice, double *out, hipStream_t s);
+hipError_t launch_commit_slices(const double *U, double *Uprev, const int *mask, int nv, int ncell,
+                                int cellsPerSlice, hipStream_t s);
+hipError_t launch_stage_pre_slices(int nv, const double *Uprev, const double *K, double *U, double *Uhat,
+                                   double *Utmp, const double *sliceDt, double beta, double bhat, int ncell,
+                                   int cellsPerSlice, hipStream_t s);
 hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double *U, double *Uhat,
                             double *Utmp, double betaDt, double bhatDt, int ncell, hipStream_t s);
 hipError_t launch_finalize(const double *errPartial, const double *posPartial, int nPart,

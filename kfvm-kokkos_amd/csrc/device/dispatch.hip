@@ -104,6 +104,34 @@ hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double
   return hipGetLastError();
 }
 
+hipError_t launch_stage_pre_slices(int nv, const double *Uprev, const double *K, double *U, double *Uhat,
+                                   double *Utmp, const double *sliceDt, double beta, double bhat, int ncell,
+                                   int cellsPerSlice, hipStream_t s) {
+  if (nv == 5)
+    hipLaunchKernelGGL((dev::k_stage_pre_slices<5>), dim3(stage_blocks(ncell)), dim3(dev::BLK), 0, s, Uprev, K, U,
+                       Uhat, Utmp, sliceDt, beta, bhat, ncell, cellsPerSlice);
+  else if (nv == 1)
+    hipLaunchKernelGGL((dev::k_stage_pre_slices<1>), dim3(stage_blocks(ncell)), dim3(dev::BLK), 0, s, Uprev, K, U,
+                       Uhat, Utmp, sliceDt, beta, bhat, ncell, cellsPerSlice);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_commit_slices(const double *U, double *Uprev, const int *mask, int nv, int ncell,
+                                int cellsPerSlice, hipStream_t s) {
+  hipLaunchKernelGGL(dev::k_commit_slices, dim3(stage_blocks(ncell)), dim3(dev::BLK), 0, s, U, Uprev, mask, nv,
+                     ncell, cellsPerSlice);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_slices(const double *errPartial, const double *posPartial, int blocksPerSlice,
+                                  int nslice, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(dev::k_finalize_slices, dim3(nslice), dim3(dev::BLK), 0, s, errPartial, posPartial,
+                     blocksPerSlice, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_finalize(const double *errPartial, const double *posPartial, int nPart,
                            double *out, hipStream_t s) {
   hipLaunchKernelGGL(dev::k_finalize, dim3(1), dim3(1024), 0, s, errPartial, posPartial, nPart, out);

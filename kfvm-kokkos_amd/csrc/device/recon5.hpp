@@ -88,7 +88,7 @@ __device__ inline PPBounds pp_bounds_lds(const double *__restrict__ lds, int c0,
 #pragma unroll
       for (int zo = zl; zo <= zh; zo++) {
         int sz;
-        const int kk = D == 3 ? map_axis(k + zo, g.nz, g.bc[4], g.bc[5], sz) : 0;
+        const int kk = D == 3 ? map_axis(k + zo, g.nz, g.bc[4], g.bc[5], sz) : k;
         const size_t c = (size_t(kk) * g.ny + jj) * g.nx + ii;
         const double rho = lds[c0 + xo * SX + yo * SY + zo * SZ];
         maxDens = fmax(maxDens, rho);
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
     const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, c = r / Bx::LY;
     const int ex = box_out<Bx::BX, H>(a), ey = box_out<Bx::BY, H>(bb);
     const int ez = D == 3 ? box_out<Bx::BZ, H>(c) : 0;
-    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + c - H : 0;
+    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + c - H : k0;
     // cells no stencil of a live cell reads (box corners, beyond the halo)
     const bool need = ex * ex + ey * ey + ez * ez <= H * H && gi < g.nx + H && gj < g.ny + H &&
                       (D == 2 || gk < g.nz + H);
@@ -276,7 +276,7 @@ __device__ inline PPBounds pp_bounds_p(const double *__restrict__ U,
 #pragma unroll
       for (int zo = zl; zo <= zh; zo++) {
         int sz;
-        const int kk = D == 3 ? map_axis(k + zo, g.nz, g.bc[4], g.bc[5], sz) : 0;
+        const int kk = D == 3 ? map_axis(k + zo, g.nz, g.bc[4], g.bc[5], sz) : k;
         const size_t c = (size_t(kk) * g.ny + jj) * g.nx + ii;
         const double rho = U[c];
         maxDens = fmax(maxDens, rho);

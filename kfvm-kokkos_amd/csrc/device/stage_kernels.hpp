@@ -69,7 +69,8 @@ __device__ inline CellRef map_cell(const Grid &g, int i, int j, int k) {
   int sx, sy, sz = -1;
   const int ii = map_axis(i, g.nx, g.bc[0], g.bc[1], sx);
   const int jj = map_axis(j, g.ny, g.bc[2], g.bc[3], sy);
-  int kk = 0;
+  // 2D: k is the slice of a batch of independent problems (0 for one problem)
+  int kk = k;
   if (D == 3) kk = map_axis(k, g.nz, g.bc[4], g.bc[5], sz);
   CellRef r;
   r.idx = (size_t(kk) * g.ny + jj) * g.nx + ii;
@@ -98,7 +99,7 @@ __device__ inline void cell_ijk(int c, const Grid &g, int &i, int &j, int &k) {
   i = c % g.nx;
   const int r = c / g.nx;
   j = r % g.ny;
-  k = D == 3 ? r / g.ny : 0;
+  k = r / g.ny;  // 2D: the slice of a batch (nz = number of problems)
 }
 
 template <int D, int NQ>
@@ -622,6 +623,10 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   const size_t nc = g.ncell;
   const size_t c = live ? (size_t(ck) * g.ny + cj) * g.nx + ci : 0;
   double vmax = 0.0, errLoc = 0.0, posLoc = 1.0;
+  // 2D batches: the brick's slice (blockIdx.z) and its step size
+  const int slice = D == 2 ? int(blockIdx.z) : 0;
+  const double betaDt = a.sliceDt ? a.beta * a.sliceDt[slice] : a.betaDt;
+  const double bhatDt = a.sliceDt ? a.bhat * a.sliceDt[slice] : a.bhatDt;
   // GLM cleaning speed of this evaluation (NV == 9)
   const double ch = NV == 9 ? __longlong_as_double((long long)a.chBits[0]) -
                                   __longlong_as_double((long long)a.chBits[1])
@@ -725,10 +730,13 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
           for (int v = 0; v < NV; v++) rhs[v] += w * a.src[(size_t(p) * NV + v) * nc + c];
         }
   }
+  const bool write = !a.sliceWrite || a.sliceWrite[slice] != 0;
   if (live) {
     if (a.mode == RK_PRIME) {
+      if (write) {
 #pragma unroll
-      for (int v = 0; v < NV; v++) a.K[size_t(v) * nc + c] = rhs[v];
+        for (int v = 0; v < NV; v++) a.K[size_t(v) * nc + c] = rhs[v];
+      }
     } else if (a.mode == RK_STAGE) {
 #pragma unroll
       for (int v = 0; v < NV; v++) {
@@ -739,17 +747,17 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
           tmp = a.Utmp[o] + a.delta * u;
           a.Utmp[o] = tmp;
         }
-        a.Unext[o] = a.g1 * u + a.g2 * tmp + a.g3 * a.Uprev[o] + a.betaDt * rhs[v];
-        a.Uhat[o] = a.Uhat[o] + a.bhatDt * rhs[v];
+        a.Unext[o] = a.g1 * u + a.g2 * tmp + a.g3 * a.Uprev[o] + betaDt * rhs[v];
+        a.Uhat[o] = a.Uhat[o] + bhatDt * rhs[v];
       }
-    } else {  // RK_FSAL: finish the embedded solution, keep K, error norm
+    } else if (write) {  // RK_FSAL: finish the embedded solution, keep K, error norm
       bool hasNan = false;
       double st[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) {
         const size_t o = size_t(v) * nc + c;
         const double u = U[o];
-        const double uh = a.Uhat[o] + a.bhatDt * rhs[v];
+        const double uh = a.Uhat[o] + bhatDt * rhs[v];
         a.Uhat[o] = uh;
         a.K[o] = rhs[v];
         st[v] = u;
@@ -763,7 +771,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   }
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   const double bmax = block_max(vmax, sh, tid);
-  if (tid == 0 && bmax > 0.0) atomicMax(a.vmaxBits, (unsigned long long)__double_as_longlong(bmax));
+  if (tid == 0 && bmax > 0.0) atomicMax(a.vmaxBits + slice, (unsigned long long)__double_as_longlong(bmax));
   if (a.mode == RK_FSAL) {
     const double es = block_sum(errLoc, sh, tid);
     const double pm = block_min(posLoc, sh, tid);
@@ -801,6 +809,66 @@ __global__ __launch_bounds__(BLK) void k_stage_pre(const double *__restrict__ Up
     U[o] = up + betaDt * kk;
     Uhat[o] = up + bhatDt * kk;
     if (Utmp) Utmp[o] = up;
+  }
+}
+
+// Stage start of a batch of slices, each with its own step size.
+template <int NV>
+__global__ __launch_bounds__(BLK) void k_stage_pre_slices(const double *__restrict__ Uprev,
+                                                          const double *__restrict__ K,
+                                                          double *__restrict__ U, double *__restrict__ Uhat,
+                                                          double *__restrict__ Utmp,
+                                                          const double *__restrict__ sliceDt, double beta,
+                                                          double bhat, int ncell, int cellsPerSlice) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= ncell) return;
+  const double dts = sliceDt[c / cellsPerSlice];
+  const double betaDt = beta * dts, bhatDt = bhat * dts;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const size_t o = size_t(v) * ncell + c;
+    const double up = Uprev[o], kk = K[o];
+    U[o] = up + betaDt * kk;
+    Uhat[o] = up + bhatDt * kk;
+    if (Utmp) Utmp[o] = up;
+  }
+}
+
+// Accepted slices: U -> Uprev.
+__global__ __launch_bounds__(BLK) void k_commit_slices(const double *__restrict__ U, double *__restrict__ Uprev,
+                                                       const int *__restrict__ mask, int nv, int ncell,
+                                                       int cellsPerSlice) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= ncell || !mask[c / cellsPerSlice]) return;
+  for (int v = 0; v < nv; v++) Uprev[size_t(v) * ncell + c] = U[size_t(v) * ncell + c];
+}
+
+// Per-slice final reduction: one block per slice over its blocks' partials.
+__global__ __launch_bounds__(BLK) void k_finalize_slices(const double *__restrict__ errPartial,
+                                                         const double *__restrict__ posPartial,
+                                                         int blocksPerSlice, double *__restrict__ out) {
+  __shared__ double se[BLK], sp[BLK];
+  const size_t b0 = size_t(blockIdx.x) * blocksPerSlice;
+  double e = 0.0, p = 1.0;
+  for (int t = threadIdx.x; t < blocksPerSlice; t += BLK) {
+    e += errPartial[b0 + t];
+    const double o = posPartial[b0 + t];
+    p = p > o ? o : p;
+  }
+  se[threadIdx.x] = e;
+  sp[threadIdx.x] = p;
+  __syncthreads();
+  for (int s = BLK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      se[threadIdx.x] += se[threadIdx.x + s];
+      const double o = sp[threadIdx.x + s];
+      sp[threadIdx.x] = sp[threadIdx.x] > o ? o : sp[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = se[0];
+    out[2 * blockIdx.x + 1] = sp[0];
   }
 }
 

@@ -76,6 +76,23 @@ int kfvm_create_from_text(const char *init_text, kfvm_solver **out) {
   return create(cfg, warn, out);
 }
 
+int kfvm_create_batch(const char *init_text, const uint64_t *seeds, int64_t n, kfvm_solver **out) {
+  if (!init_text || !seeds || n < 1) return fail("null init text or no seeds");
+  kfvm::Config cfg;
+  std::string warn;
+  kfvm::parse_init_text(init_text, cfg, &warn);
+  cfg.slices = int(n);
+  cfg.sliceSeeds.assign(seeds, seeds + n);
+  return create(cfg, warn, out);
+}
+
+int kfvm_get_slice_times(kfvm_solver *s, double *t, double *dt, int64_t n) {
+  CHECK_SOLVER(s);
+  if (n != s->s->slices()) return fail("one value per slice");
+  s->s->sliceTimes(t, dt);
+  return 0;
+}
+
 int kfvm_destroy(kfvm_solver *s) {
   if (!s) return 0;
   return guarded([&] {
@@ -192,7 +209,7 @@ int kfvm_get_info(kfvm_solver *s, kfvm_info *info) {
   info->nv = s->s->nv();
   info->nx = c.nX;
   info->ny = c.nY;
-  info->nz = c.dim == 3 ? c.nZ : 1;
+  info->nz = c.dim == 3 ? c.nZ : s->s->slices();
   info->ncell = s->s->ncell();
   info->nsub = w.nSub;
   info->nfaces = w.F;

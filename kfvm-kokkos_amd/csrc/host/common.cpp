@@ -316,6 +316,7 @@ bool apply_line(const std::string &category, const std::string &setting, Config 
       else if (key == "dim") c.dim = std::stoi(val(1));
       else if (key == "nquad") c.nQuad = std::stoi(val(1));
       else if (key == "radius") c.rad = std::stoi(val(1));
+      else if (key == "slices") c.slices = std::stoi(val(1));
       else if (key == "equation") {
         if (val(1) == "hydro") c.eq = Eq::Hydro;
         else if (val(1) == "linadv") c.eq = Eq::LinAdv;

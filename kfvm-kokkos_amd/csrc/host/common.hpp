@@ -85,6 +85,10 @@ struct Config {
   // user parameters of the problem registry
   double user[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t seed = 0;
+  // 2D batch: `slices` independent problems of this configuration advanced
+  // together (problem:slices), each with its own seed (seed + s unless given)
+  int slices = 1;
+  std::vector<uint64_t> sliceSeeds;
   // io
   std::string baseName = "default", dataDir = "data";
   double plotFreq = -1.0;

@@ -54,14 +54,25 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   if (cfg_.cfl <= 0.0) cfg_.cfl = cfg_.cflValue();
 
   nv_ = num_vars(cfg_.eq);
-  ncell_ = cfg_.nCells();
+  slices_ = cfg_.slices;
+  if (slices_ < 1) throw std::runtime_error("problem:slices must be >= 1");
+  if (slices_ == 1 && !cfg_.sliceSeeds.empty()) cfg_.seed = cfg_.sliceSeeds[0];  // a batch of one
+  if (slices_ > 1) {
+    if (cfg_.dim != 2 || cfg_.eq == Eq::MhdGlm)
+      throw std::runtime_error("batches of slices are built for 2D hydro and linear advection");
+    if (cfg_.plotFreq > 0.0 || cfg_.ckptFreq > 0 || !cfg_.restartFile.empty())
+      throw std::runtime_error("batches of slices have no plot / checkpoint output");
+    if (!cfg_.sliceSeeds.empty() && int(cfg_.sliceSeeds.size()) != slices_)
+      throw std::runtime_error("one seed per slice required");
+  }
+  ncell_ = cfg_.nCells() * (cfg_.dim == 2 ? slices_ : 1);
   if (ncell_ > (int64_t(1) << 31) - 1) throw std::runtime_error("grid too large for one block");
   rk_ = rk_table(cfg_.rk);
   wts_ = build_weights(cfg_.dim, cfg_.nQuad, cfg_.lfac, cfg_.dx(), cfg_.dy(), cfg_.dz());
 
   grid_.nx = int(cfg_.nX);
   grid_.ny = int(cfg_.nY);
-  grid_.nz = int(cfg_.dim == 3 ? cfg_.nZ : 1);
+  grid_.nz = int(cfg_.dim == 3 ? cfg_.nZ : slices_);  // 2D: one z-plane per slice
   grid_.ncell = int(ncell_);
   for (int f = 0; f < 6; f++) {
     grid_.bc[f] = int(cfg_.bc[f]);
@@ -127,7 +138,16 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   errPart_ = dalloc<double>(nPart_, "err partials");
   posPart_ = dalloc<double>(nPart_, "pos partials");
   redOut_ = dalloc<double>(2, "reduction");
-  vmax_ = dalloc<unsigned long long>(1, "vmax");
+  vmax_ = dalloc<unsigned long long>(size_t(slices_), "vmax");
+  if (slices_ > 1) {
+    sDtDev_ = dalloc<double>(size_t(slices_), "slice dt");
+    sRed_ = dalloc<double>(2 * size_t(slices_), "slice reductions");
+    sMask_ = dalloc<int>(size_t(slices_), "slice mask");
+    check(hipHostMalloc((void **)&sHost_, 4 * size_t(slices_) * sizeof(double), hipHostMallocDefault),
+          "hipHostMalloc");
+    check(hipHostMalloc((void **)&sMaskHost_, size_t(slices_) * sizeof(int), hipHostMallocDefault),
+          "hipHostMalloc");
+  }
   check(hipHostMalloc((void **)&hostRed_, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
 
   uploadWeights();
@@ -138,7 +158,20 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
 
   const bool restart = !cfg_.restartFile.empty();
   if (cfg_.plotFreq > 0.0 || cfg_.ckptFreq > 0 || restart) writer_.reset(new Writer(cfg_));
-  if (!restart) {
+  if (!restart && slices_ > 1) {
+    // one initial condition per slice, each with its own seed
+    std::vector<double> U0(size_t(ncell_) * nv_), one;
+    const size_t cps = size_t(ncell_ / slices_);
+    for (int sl = 0; sl < slices_; sl++) {
+      Config c1 = cfg_;
+      c1.slices = 1;
+      c1.seed = cfg_.sliceSeeds.empty() ? cfg_.seed + uint64_t(sl) : cfg_.sliceSeeds[size_t(sl)];
+      fill_initial_condition(c1, one);
+      for (int v = 0; v < nv_; v++)
+        std::copy(one.begin() + v * cps, one.begin() + (v + 1) * cps, U0.begin() + v * ncell_ + sl * cps);
+    }
+    setState(U0.data());
+  } else if (!restart) {
     std::vector<double> U0;
     fill_initial_condition(cfg_, U0);
     setState(U0.data());
@@ -173,6 +206,167 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   // FSAL prime and the small first step (Solver.cpp:73-80)
   const double maxVel = evalRHS(0);
   if (!restart) dt = 0.01 * cfg_.cfl * cfg_.dmin() / maxVel;
+  if (slices_ > 1) {
+    std::vector<unsigned long long> vm(static_cast<size_t>(slices_));
+    check(hipMemcpy(vm.data(), vmax_, vm.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost), "read vmax");
+    sT_.assign(size_t(slices_), 0.0);
+    sErr_.assign(size_t(slices_), 1.0);
+    sLast_.assign(size_t(slices_), 0);
+    sDone_.assign(size_t(slices_), 0);
+    sDt_.resize(size_t(slices_));
+    for (int sl = 0; sl < slices_; sl++) {
+      double v;
+      std::memcpy(&v, &vm[size_t(sl)], sizeof(double));
+      sDt_[size_t(sl)] = 0.01 * cfg_.cfl * cfg_.dmin() / v;
+    }
+    dt = *std::min_element(sDt_.begin(), sDt_.end());
+  }
+}
+
+void Solver::sliceTimes(double *t, double *dtOut) const {
+  for (int sl = 0; sl < slices_; sl++) {
+    if (t) t[sl] = slices_ > 1 ? sT_[size_t(sl)] : time;
+    if (dtOut) dtOut[sl] = slices_ > 1 ? sDt_[size_t(sl)] : dt;
+  }
+}
+
+// One lockstep step of every unfinished slice, each with its own step size
+// and controller (the reference's TakeStep, Solver.cpp:188-343, per slice):
+// the stages of all slices run in the same launches; slices whose attempt is
+// rejected retry in further attempts while the accepted ones are committed.
+int Solver::takeStepSlices() {
+  const int S = slices_;
+  const int cps = int(ncell_ / S);
+  const bool fixed = cfg_.fixedDt > 0.0;
+  const double dmin = cfg_.dmin(), nDofs = double(cps) * nv_;
+  std::vector<char> pending(static_cast<size_t>(S)), firstUnphys(static_cast<size_t>(S), 1);
+  for (int sl = 0; sl < S; sl++) {
+    pending[size_t(sl)] = !sDone_[size_t(sl)];
+    if (!pending[size_t(sl)]) continue;
+    double &d = sDt_[size_t(sl)];
+    if (fixed) d = cfg_.fixedDt;
+    if (sT_[size_t(sl)] + d > cfg_.finalTime) {
+      d = cfg_.finalTime - sT_[size_t(sl)];
+      sLast_[size_t(sl)] = 1;
+    }
+  }
+  int status = ACCEPTED;
+  stateInCur_ = false;
+  stagePos_ = 0;
+  const int blocksPerSlice = nPart_ / S;
+  double *hDt = sHost_, *hRed = sHost_ + S, *hVm = sHost_ + 3 * S;
+  for (int nT = 0; nT < cfg_.rejectionLimit; nT++) {
+    if (std::none_of(pending.begin(), pending.end(), [](char p) { return p != 0; })) break;
+    for (int sl = 0; sl < S; sl++) {
+      hDt[sl] = pending[size_t(sl)] ? sDt_[size_t(sl)] : 0.0;
+      sMaskHost_[sl] = pending[size_t(sl)] ? 1 : 0;
+    }
+    check(hipMemcpyAsync(sDtDev_, hDt, size_t(S) * sizeof(double), hipMemcpyHostToDevice, stream_), "slice dt");
+    check(hipMemcpyAsync(sMask_, sMaskHost_, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_), "mask");
+    check(hipMemsetAsync(vmax_, 0, size_t(S) * sizeof(unsigned long long), stream_), "clear vmax");
+    if (nT > 0) {  // the retrying slices' FSAL register, from their step start state
+      RKArgs a{};
+      a.mode = RK_PRIME;
+      a.K = K_;
+      a.vmaxBits = vmax_;
+      a.sliceWrite = sMask_;
+      pipeline(Uprev_, a);
+    }
+    timed("stage_pre", [&] {
+      return launch_stage_pre_slices(nv_, Uprev_, K_, U_[cur_], Uhat_, Utmp_, sDtDev_, rk_.beta[0], rk_.bhat[0],
+                                     int(ncell_), cps, stream_);
+    });
+    for (int st = 1; st < rk_.nStages; st++) {
+      RKArgs a{};
+      a.mode = RK_STAGE;
+      a.g1 = rk_.gam1[st];
+      a.g2 = rk_.gam2[st];
+      a.g3 = rk_.gam3[st];
+      a.delta = rk_.delta[st];
+      a.sliceDt = sDtDev_;
+      a.beta = rk_.beta[st];
+      a.bhat = rk_.bhat[st];
+      a.Uprev = Uprev_;
+      a.Uhat = Uhat_;
+      a.Utmp = Utmp_;
+      a.Unext = U_[cur_ ^ 1];
+      a.vmaxBits = vmax_;
+      pipeline(U_[cur_], a);
+      cur_ ^= 1;
+    }
+    {
+      RKArgs a{};
+      a.mode = RK_FSAL;
+      a.sliceDt = sDtDev_;
+      a.bhat = rk_.bhatfsal;
+      a.atol = cfg_.atol;
+      a.rtol = cfg_.rtol;
+      a.K = K_;
+      a.Uhat = Uhat_;
+      a.errPartial = errPart_;
+      a.posPartial = posPart_;
+      a.vmaxBits = vmax_;
+      a.sliceWrite = sMask_;
+      pipeline(U_[cur_], a);
+    }
+    check(launch_finalize_slices(errPart_, posPart_, blocksPerSlice, S, sRed_, stream_), "finalize");
+    check(hipMemcpyAsync(hRed, sRed_, 2 * size_t(S) * sizeof(double), hipMemcpyDeviceToHost, stream_), "read");
+    check(hipMemcpyAsync(hVm, vmax_, size_t(S) * sizeof(double), hipMemcpyDeviceToHost, stream_), "read vmax");
+    check(hipMemcpyAsync(&hostRed_[3], counter_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read count");
+    check(hipStreamSynchronize(stream_), "sync");
+    int cnt;
+    std::memcpy(&cnt, &hostRed_[3], sizeof(int));
+    lastNWeno = lastRhsFull_ ? ncell_ : cnt;
+    for (int sl = 0; sl < S; sl++) {
+      sMaskHost_[sl] = 0;
+      if (!pending[size_t(sl)]) continue;
+      const double errSum = hRed[2 * sl], posFlag = hRed[2 * sl + 1], maxVel = hVm[sl];
+      double &d = sDt_[size_t(sl)];
+      const double errNew = 1.0 / std::sqrt(errSum / nDofs);
+      double dtfac = std::pow(errNew, rk_.ep1) * std::pow(sErr_[size_t(sl)], rk_.ep2);
+      dtfac = std::isnan(dtfac) ? 1.2 : 1.0 + std::atan(dtfac - 1.0);
+      if (posFlag < 0.0) {
+        d = firstUnphys[size_t(sl)] ? std::fmin(0.25 * cfg_.cfl * dmin / maxVel, d / 4.0) : d / 2.0;
+        firstUnphys[size_t(sl)] = 0;
+        useSparseWeno = false;
+        nRejectUnphys++;
+        sLast_[size_t(sl)] = 0;
+      } else if (dtfac >= cfg_.rejectionThresh || fixed) {
+        sT_[size_t(sl)] += d;
+        d = fixed ? cfg_.fixedDt : std::fmin(d * dtfac, cfg_.cfl * dmin / maxVel);
+        sErr_[size_t(sl)] = errNew;
+        sMaskHost_[sl] = 1;
+        pending[size_t(sl)] = 0;
+        if (sLast_[size_t(sl)]) sDone_[size_t(sl)] = 1;
+      } else {
+        d = d * dtfac;
+        nRejectThresh++;
+        sLast_[size_t(sl)] = 0;
+      }
+    }
+    check(hipMemcpyAsync(sMask_, sMaskHost_, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_), "mask");
+    check(launch_commit_slices(U_[cur_], Uprev_, sMask_, nv_, int(ncell_), cps, stream_), "commit");
+  }
+  // slices still rejected after the limit stop where they are
+  for (int sl = 0; sl < S; sl++)
+    if (pending[size_t(sl)]) {
+      sDone_[size_t(sl)] = 1;
+      status = REJECT_LIMIT;
+    }
+  nTS++;
+  lastTimeStep = std::all_of(sDone_.begin(), sDone_.end(), [](char x) { return x != 0; });
+  time = 0.0;
+  dt = 0.0;
+  bool first = true;
+  for (int sl = 0; sl < S; sl++)
+    if (!sDone_[size_t(sl)] || first) {
+      time = first ? sT_[size_t(sl)] : std::fmin(time, sT_[size_t(sl)]);
+      dt = first ? sDt_[size_t(sl)] : std::fmin(dt, sDt_[size_t(sl)]);
+      first = false;
+    }
+  if (lastTimeStep) time = *std::min_element(sT_.begin(), sT_.end());
+  check(hipStreamSynchronize(stream_), "sync");
+  return status;
 }
 
 std::string Solver::writePlot() {
@@ -200,11 +394,13 @@ std::string Solver::writeCkpt() {
 Solver::~Solver() {
   profileEnable(false);
   void *bufs[] = {SRC_, cellW_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (hostRed_) (void)hipHostFree(hostRed_);
+  if (sHost_) (void)hipHostFree(sHost_);
+  if (sMaskHost_) (void)hipHostFree(sMaskHost_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -368,7 +564,7 @@ double Solver::readSpeed() {
 }
 
 double Solver::evalRHS(int which) {
-  check(hipMemsetAsync(vmax_, 0, sizeof(unsigned long long), stream_), "clear vmax");
+  check(hipMemsetAsync(vmax_, 0, size_t(slices_) * sizeof(unsigned long long), stream_), "clear vmax");
   RKArgs a{};
   a.mode = RK_PRIME;
   a.K = K_;
@@ -379,6 +575,7 @@ double Solver::evalRHS(int which) {
 }
 
 int Solver::takeStep() {
+  if (slices_ > 1) return takeStepSlices();
   const bool fixed = cfg_.fixedDt > 0.0;
   if (fixed) dt = cfg_.fixedDt;
   // limit dt to hit plot times or the final time (Solver.cpp:197-205)
@@ -518,6 +715,7 @@ bool Solver::nadFails() {
 
 void Solver::setIndicator(const float *params, int np, double threshold) {
   check(hipStreamSynchronize(stream_), "sync");
+  if (slices_ > 1 && np != 0) throw std::runtime_error("the learned indicator runs on single problems");
   if (np == 0) {
     learned_ = false;
     return;
@@ -557,6 +755,7 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
 }
 
 void Solver::advanceStages(int64_t n, double dtFixed) {
+  if (slices_ > 1) throw std::runtime_error("advance_stages is built for single problems");
   stateInCur_ = false;
   for (int64_t it = 0; it < n; it++) {
     if (stagePos_ == 0) {

@@ -31,6 +31,9 @@ class Solver {
   // stores the FSAL register and returns the max wave speed.
   double evalRHS(int which);
   int takeStep();
+  // 2D batches: number of slices and their times / step sizes
+  int slices() const { return slices_; }
+  void sliceTimes(double *t, double *dt) const;
   int64_t solve();
   // Advance exactly n RK stages of fixed-dt steps, queued on the device with
   // no host synchronisation (a step completes every rkStages() stages).
@@ -102,6 +105,7 @@ class Solver {
   void uploadWeights();
   double readSpeed();
   bool nadFails();
+  int takeStepSlices();
 
   Config cfg_;
   RKTable rk_;
@@ -143,6 +147,14 @@ class Solver {
   double *nadFlag_ = nullptr;  // cells that failed the last check
   int *nadCount_ = nullptr;
   double *FSlin_ = nullptr;
+
+  // 2D batch of slices: per-slice controller state (host) and step sizes,
+  // reductions and commit mask (device, with pinned host mirrors)
+  int slices_ = 1;
+  std::vector<double> sT_, sDt_, sErr_;
+  std::vector<char> sLast_, sDone_;
+  double *sDtDev_ = nullptr, *sRed_ = nullptr, *sHost_ = nullptr;  // host: [dt S][red 2S][vmax S]
+  int *sMask_ = nullptr, *sMaskHost_ = nullptr;
 
   std::unique_ptr<Writer> writer_;
   bool writePlot_ = false;  // the last step ended on a plot time

@@ -35,7 +35,7 @@ EXPORTS = [
     "kfvm_indicator_train", "kfvm_indicator_get_params", "kfvm_indicator_set_params",
     "kfvm_indicator_get_weights", "kfvm_indicator_gradient", "kfvm_indicator_logits",
     "kfvm_indicator_sizes", "kfvm_set_nad", "kfvm_write_plot", "kfvm_write_checkpoint",
-    "kfvm_write_output_file", "kfvm_read_checkpoint_file",
+    "kfvm_write_output_file", "kfvm_read_checkpoint_file", "kfvm_create_batch", "kfvm_get_slice_times",
 ]
 
 
@@ -95,6 +95,8 @@ def load_library(path=None):
         "kfvm_set_indicator": [P, U16, I64, ctypes.c_double],
         "kfvm_get_indicator_counters": [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(ctypes.c_int)],
         "kfvm_set_nad": [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "kfvm_create_batch": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64), I64, ctypes.POINTER(P)],
+        "kfvm_get_slice_times": [P, D, D, I64],
         "kfvm_write_plot": [P, ctypes.c_char_p, I64],
         "kfvm_write_checkpoint": [P, ctypes.c_char_p, I64],
         "kfvm_write_output_file": [ctypes.c_char_p, ctypes.c_int, D, D, I64, ctypes.c_int, ctypes.c_double,
@@ -321,12 +323,18 @@ def parse_init(text):
 class Solver:
     """One problem on one GPU (KFVM::Solver)."""
 
-    def __init__(self, init_path=None, init_text=None):
+    def __init__(self, init_path=None, init_text=None, seeds=None):
+        """seeds: a batch of len(seeds) independent 2D problems (init_text only)."""
         self._lib = load_library()
         self._h = ctypes.c_void_p()
         if (init_path is None) == (init_text is None):
             raise ValueError("give exactly one of init_path / init_text")
-        if init_path is not None:
+        if seeds is not None:
+            if init_text is None:
+                raise ValueError("batches take init_text")
+            sd = (ctypes.c_uint64 * len(seeds))(*[int(x) for x in seeds])
+            self._call("kfvm_create_batch", init_text.encode(), sd, len(seeds), ctypes.byref(self._h))
+        elif init_path is not None:
             self._call("kfvm_create", init_path.encode(), ctypes.byref(self._h))
         else:
             self._call("kfvm_create_from_text", init_text.encode(), ctypes.byref(self._h))
@@ -444,6 +452,14 @@ class Solver:
         t, dt, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
         self._call("kfvm_get_time", self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(n))
         return t.value, dt.value, n.value
+
+    @property
+    def slice_times(self):
+        """(times, step sizes) of the problems of a batch."""
+        n = self.shape[1] if self.info.dim == 2 else 1
+        t, dt = np.empty(n), np.empty(n)
+        self._call("kfvm_get_slice_times", self._h, _dptr(t), _dptr(dt), n)
+        return t, dt
 
     def set_time(self, t, dt):
         self._call("kfvm_set_time", self._h, float(t), float(dt))
