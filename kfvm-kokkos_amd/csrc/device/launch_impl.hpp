@@ -139,9 +139,9 @@ struct Launch {
                        nadFlag, count, g);
     return hipGetLastError();
   }
-  static dim3 flux_grid(const Grid &g) {
+  static dim3 flux_grid(const Grid &g) {  // one-dimensional: the kernel renumbers the bricks
     using Bk = dev::Brick<D>;
-    return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X), unsigned((g.ny + Bk::Y - 1) / Bk::Y),
+    return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X) * unsigned((g.ny + Bk::Y - 1) / Bk::Y) *
                 unsigned((g.nz + Bk::Z - 1) / Bk::Z));
   }
   static hipError_t flux_rk(int nv, const double *U, const double *FS, const Grid &g,

@@ -49,14 +49,6 @@ struct RBox {
   static constexpr int NT = BX * BY * BZ;  // threads per block
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; handing each XCD a
-// contiguous range of bricks keeps neighbouring bricks (and their shared halo
-// cells) in the same L2.
-__device__ inline int xcd_block(int b, int nb) {
-  if (nb % 8) return b;
-  return (b % 8) * (nb / 8) + b / 8;
-}
-
 // distance of a box coordinate outside the brick interior [H, H+B)
 template <int B, int H>
 __device__ inline int box_out(int t) {

@@ -565,6 +565,14 @@ __device__ inline double block_max(double x, double *sh, int tid) {
   return r;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Renumbering block b of nb so that every XCD works on one contiguous range of
+// bricks keeps neighbouring bricks (and their shared halo cells) in the same L2.
+__device__ inline int xcd_block(int b, int nb) {
+  if (nb % 8) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
 // Brick of cells handled by one workgroup of the flux kernel.  16 cells in x
 // make every row of a face-state plane one full 128-byte line, so no line is
 // fetched twice by the two bricks that share it.
@@ -617,14 +625,18 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   __shared__ double fl[NV][Bk::NF];
   __shared__ double sh[BLK];
   const int tid = threadIdx.x + Bk::X * (threadIdx.y + Bk::Y * threadIdx.z);
-  const int i0 = blockIdx.x * Bk::X, j0 = blockIdx.y * Bk::Y, k0 = blockIdx.z * Bk::Z;
+  // one-dimensional grid of bricks, x fastest, renumbered per XCD
+  const int nbx = (g.nx + Bk::X - 1) / Bk::X, nby = (g.ny + Bk::Y - 1) / Bk::Y;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int bzi = b / (nbx * nby);
+  const int i0 = (b % nbx) * Bk::X, j0 = ((b / nbx) % nby) * Bk::Y, k0 = bzi * Bk::Z;
   const int ci = i0 + threadIdx.x, cj = j0 + threadIdx.y, ck = k0 + threadIdx.z;
   const bool live = ci < g.nx && cj < g.ny && (D == 2 || ck < g.nz);
   const size_t nc = g.ncell;
   const size_t c = live ? (size_t(ck) * g.ny + cj) * g.nx + ci : 0;
   double vmax = 0.0, errLoc = 0.0, posLoc = 1.0;
   // 2D batches: the brick's slice (blockIdx.z) and its step size
-  const int slice = D == 2 ? int(blockIdx.z) : 0;
+  const int slice = D == 2 ? bzi : 0;
   const double betaDt = a.sliceDt ? a.beta * a.sliceDt[slice] : a.betaDt;
   const double bhatDt = a.sliceDt ? a.bhat * a.sliceDt[slice] : a.bhatDt;
   // GLM cleaning speed of this evaluation (NV == 9)
@@ -769,7 +781,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
                       : (NV == 5 ? Hydro::positivity(st) : (NV == 9 ? fmin(st[0], Mhd::eint(st)) : 1.0));
     }
   }
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int bid = b;  // slice-major: the bricks of a 2D slice are one contiguous range
   const double bmax = block_max(vmax, sh, tid);
   if (tid == 0 && bmax > 0.0) atomicMax(a.vmaxBits + slice, (unsigned long long)__double_as_longlong(bmax));
   if (a.mode == RK_FSAL) {
