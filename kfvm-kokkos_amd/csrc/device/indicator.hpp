@@ -125,23 +125,69 @@ __device__ inline void wave_append(bool flagged, int c, int *__restrict__ list, 
 
 // Learned selection: WENO where the network's logit exceeds thr, where the
 // previous RHS evaluation limited the cell (theta < 1, as the KXRCF selector),
-// or where the last step's admissibility check failed (nadFlag > 0).
+// or where the last step's admissibility check failed (nadFlag > 0).  One
+// block per brick of the linear reconstruction's geometry: rho and p of the
+// brick and its radius-2 halo are staged in LDS once, and each cell's
+// features (the same arithmetic as cell_features) read them from there.
 template <int D>
-__global__ __launch_bounds__(BLK) void k_learned_select(const double *__restrict__ U,
-                                                        const double *__restrict__ eint,
-                                                        const double *__restrict__ theta,
-                                                        const double *__restrict__ nadFlag,
-                                                        const float *__restrict__ P, float thr,
-                                                        double *__restrict__ wflag,
-                                                        int *__restrict__ list,
-                                                        int *__restrict__ counter, Grid g) {
-  const int c = blockIdx.x * BLK + threadIdx.x;
+__global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__restrict__ U,
+                                                                const double *__restrict__ eint,
+                                                                const double *__restrict__ theta,
+                                                                const double *__restrict__ nadFlag,
+                                                                const float *__restrict__ P, float thr,
+                                                                double *__restrict__ wflag,
+                                                                int *__restrict__ list,
+                                                                int *__restrict__ counter, Grid g) {
+  using Bx = RBox<D>;
+  constexpr auto &ST = StTab<D>::ST;
+  constexpr int C = SInfo<D>::C, O = centre_index<D>(), H = Bx::H, NP = Bx::N;
+  __shared__ double sr[NP], sp[NP];
+  const int nbx = (g.nx + Bx::BX - 1) / Bx::BX, nby = (g.ny + Bx::BY - 1) / Bx::BY;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (b % nbx) * Bx::BX, j0 = ((b / nbx) % nby) * Bx::BY, k0 = (b / (nbx * nby)) * Bx::BZ;
+  const int tid = threadIdx.x;
+  const double gm1 = g.gamma - 1.0;
+  for (int idx = tid; idx < NP; idx += Bx::NT) {
+    const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, cz = r / Bx::LY;
+    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + cz - H : k0;
+    // box cells beyond the grid's far edges by more than the halo are never read
+    if (gi < g.nx + H && gj < g.ny + H && (D == 2 || gk < g.nz + H)) {
+      const size_t q = map_cell<D, 5>(g, gi, gj, gk).idx;
+      sr[idx] = U[q];
+      sp[idx] = gm1 * eint[q];
+    }
+  }
+  __syncthreads();
+  const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
+  const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+  const bool live = i < g.nx && j < g.ny && (D == 2 || k < g.nz);
+  const int c = live ? (k * g.ny + j) * g.nx + i : 0;
   bool flagged = false;
-  if (c < g.ncell) {
-    int i, j, k;
-    cell_ijk<D>(c, g, i, j, k);
+  if (live) {
+    const int corner = (tz * Bx::LY + ty) * Bx::LX + tx;
+    auto at = [&](int n) {
+      return corner + (ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
+             (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0);
+    };
+    const double r0 = sr[at(O)], p0 = sp[at(O)];
+    double smr = 0.0, smp = 0.0;
+#pragma unroll
+    for (int n = 0; n < C; n++) {
+      if (n == O) continue;
+      smr = fmax(smr, fabs(sr[at(n)] - r0));
+      smp = fmax(smp, fabs(sp[at(n)] - p0));
+    }
+    const double ir = 1.0 / (smr + 1e-300), ip = 1.0 / (smp + 1e-300);
     float x[Net<D>::NIN];
-    cell_features<D>(U, eint, g, i, j, k, x);
+#pragma unroll
+    for (int n = 0; n < C; n++) {
+      if (n == O) continue;
+      const int m = n < O ? n : n - 1;
+      x[m] = float((sr[at(n)] - r0) * ir);
+      x[C - 1 + m] = float((sp[at(n)] - p0) * ip);
+    }
+    x[2 * C - 2] = float(fmin(smr / fabs(r0), 4.0));
+    x[2 * C - 1] = float(fmin(smp / fabs(p0), 4.0));
     flagged = net_logit<D>(P, x) > thr || theta[c] < 1.0 || nadFlag[c] > 0.0;
     wflag[c] = flagged ? 1.0 : 0.0;
   }

@@ -108,8 +108,8 @@ struct Launch {
   static hipError_t learned_select(const double *U, const double *eint, const double *theta, const double *nadFlag,
                                    const float *P, float thr, double *wflag, int *list, int *counter,
                                    const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_learned_select<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint,
-                       theta, nadFlag, P, thr, wflag, list, counter, g);
+    hipLaunchKernelGGL((dev::k_learned_select<D>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint, theta,
+                       nadFlag, P, thr, wflag, list, counter, g);
     return hipGetLastError();
   }
   static hipError_t features(const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {

@@ -168,3 +168,30 @@ def test_trained_indicator_gains_over_kxrcf_at_equal_l1():
           f"learned L1 {res['learned'][0]:.4e} WENO {res['learned'][1]:.3f}", flush=True)
     assert res["learned"][0] <= 1.05 * res["plain"][0]
     assert res["learned"][1] < 0.5 * res["plain"][1]
+
+
+@pytest.mark.parametrize("name", ["riemann3d", "quadrant2d"])
+def test_learned_selection_flags_the_cells_the_network_picks(name):
+    """The selection kernel (features from an LDS-staged brick, network in FP32)
+    flags exactly the cells whose float64 logit is positive or that the
+    previous evaluation limited, ties aside."""
+    import kfvm_amd
+    c = CASES[name]
+    s, _, p = _solver_and_oracle(c)
+    dim = c["dim"]
+    rng = np.random.default_rng(4)
+    nin, npar = ir.net_sizes(dim)
+    P = rng.normal(0.0, 0.4, npar).astype(np.float32)
+    P[-1] = 0.3
+    W = kfvm_amd.float_to_bf16(P)
+    s.set_indicator(W, 0.0)
+    U = s.state
+    theta_prev = s.flags[1].ravel()
+    s.eval_rhs(0)
+    flags = s.flags[0].ravel()
+    off = (stencil3d() if dim == 3 else stencil2d())[0]
+    z = ir.forward(kfvm_amd.bf16_to_float(W), ir.features(U, off, p["gamma"], p["bc"]), dim)
+    clear = np.abs(z) > 1e-4
+    mism = int(((flags > 0.5) != ((z > 0) | (theta_prev < 1.0)))[clear].sum())
+    print(f"[indicator] {name}: {int(flags.sum())} of {flags.size} flagged, {mism} mismatches", flush=True)
+    assert 0 < flags.sum() < flags.size and mism == 0
