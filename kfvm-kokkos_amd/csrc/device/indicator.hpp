@@ -416,36 +416,55 @@ __global__ __launch_bounds__(BLK) void k_adam(float *__restrict__ M, float *__re
 // Numerical admissibility of the new cell averages: rho and p of every cell
 // within [min - delta, max + delta] of its 3^D neighbourhood at the previous
 // step, delta = max(fv max(|min|, |max|), fr (max - min)).  Marks the
-// violating cells in nadFlag (1 / 0) and counts them.
+// violating cells in nadFlag (1 / 0) and counts them.  One block per brick:
+// rho and p of the old state over the brick and a one-cell halo are staged in
+// LDS.
 template <int D>
-__global__ __launch_bounds__(BLK) void k_nad(const double *__restrict__ Uold, const double *__restrict__ Unew,
-                                             double fr, double fv, double *__restrict__ nadFlag,
-                                             int *__restrict__ count, Grid g) {
-  const int c = blockIdx.x * BLK + threadIdx.x;
+__global__ __launch_bounds__(RBox<D>::NT) void k_nad(const double *__restrict__ Uold,
+                                                     const double *__restrict__ Unew, double fr, double fv,
+                                                     double *__restrict__ nadFlag, int *__restrict__ count,
+                                                     Grid g) {
+  using Bx = RBox<D>;
+  constexpr int LX = Bx::BX + 2, LY = Bx::BY + 2, LZ = D == 3 ? Bx::BZ + 2 : 1, NP = LX * LY * LZ;
+  __shared__ double sr[NP], sp[NP];
+  const int nbx = (g.nx + Bx::BX - 1) / Bx::BX, nby = (g.ny + Bx::BY - 1) / Bx::BY;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (b % nbx) * Bx::BX, j0 = ((b / nbx) % nby) * Bx::BY, k0 = (b / (nbx * nby)) * Bx::BZ;
+  const int tid = threadIdx.x;
+  const size_t nc = g.ncell;
+  const double gm1 = g.gamma - 1.0;
+  for (int idx = tid; idx < NP; idx += Bx::NT) {
+    const int a = idx % LX, r = idx / LX, bb = r % LY, cz = r / LY;
+    const int gi = i0 + a - 1, gj = j0 + bb - 1, gk = D == 3 ? k0 + cz - 1 : k0;
+    if (gi < g.nx + 1 && gj < g.ny + 1 && (D == 2 || gk < g.nz + 1)) {
+      const CellRef cr = map_cell<D, 5>(g, gi, gj, gk);
+      double s[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) s[v] = ld(Uold, nc, cr, v);
+      sr[idx] = s[0];
+      sp[idx] = gm1 * (s[4] - 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0]);
+    }
+  }
+  __syncthreads();
+  const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
+  const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
   bool bad = false;
-  if (c < g.ncell) {
-    int i, j, k;
-    cell_ijk<D>(c, g, i, j, k);
-    const size_t nc = g.ncell;
-    const double gm1 = g.gamma - 1.0;
+  if (i < g.nx && j < g.ny && (D == 2 || k < g.nz)) {
+    const size_t c = (size_t(k) * g.ny + j) * g.nx + i;
     double lo[2] = {1.7976931348623157e308, 1.7976931348623157e308};
     double hi[2] = {-1.7976931348623157e308, -1.7976931348623157e308};
     constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
-#pragma unroll 1
-    for (int xo = -1; xo <= 1; xo++)
-#pragma unroll 1
-      for (int yo = -1; yo <= 1; yo++)
-#pragma unroll 1
-        for (int zo = zl; zo <= zh; zo++) {
-          const CellRef r = map_cell<D, 5>(g, i + xo, j + yo, k + zo);
-          double s[5];
 #pragma unroll
-          for (int v = 0; v < 5; v++) s[v] = ld(Uold, nc, r, v);
-          const double p = gm1 * (s[4] - 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0]);
-          lo[0] = fmin(lo[0], s[0]);
-          hi[0] = fmax(hi[0], s[0]);
-          lo[1] = fmin(lo[1], p);
-          hi[1] = fmax(hi[1], p);
+    for (int xo = -1; xo <= 1; xo++)
+#pragma unroll
+      for (int yo = -1; yo <= 1; yo++)
+#pragma unroll
+        for (int zo = zl; zo <= zh; zo++) {
+          const int q = (tx + 1 + xo) + LX * ((ty + 1 + yo) + LY * (D == 3 ? tz + 1 + zo : 0));
+          lo[0] = fmin(lo[0], sr[q]);
+          hi[0] = fmax(hi[0], sr[q]);
+          lo[1] = fmin(lo[1], sp[q]);
+          hi[1] = fmax(hi[1], sp[q]);
         }
     double s[5];
 #pragma unroll

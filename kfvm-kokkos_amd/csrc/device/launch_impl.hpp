@@ -135,7 +135,7 @@ struct Launch {
   }
   static hipError_t nad(const double *Uold, const double *Unew, double fr, double fv, double *nadFlag, int *count,
                         const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_nad<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, Uold, Unew, fr, fv,
+    hipLaunchKernelGGL((dev::k_nad<D>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
                        nadFlag, count, g);
     return hipGetLastError();
   }

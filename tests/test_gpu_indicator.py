@@ -137,12 +137,30 @@ def test_failed_cells_get_weno_in_the_next_step():
     W[-1] = kfvm_amd.float_to_bf16(np.float32([-1.0]))[0]
     s.set_indicator(W, 0.0)
     s.set_nad(True, 1e-3, 1e-4, 1.0)
+    U0 = s.state
     assert s.take_step() == 0
     failed = s.indicator_counters["nad_cells"]
     assert failed > 0
+    assert failed == _nad_count(U0, s.state, kfvm_amd.parse_init(init_text(c))["gamma"],
+                                1e-3, 1e-4)
     assert s.take_step() == 0
     assert s.indicator_counters["nad_fallbacks"] == 0
     assert s.flags[0].sum() >= failed
+
+
+def _nad_count(Uo, Un, g, fr, fv):
+    """Cells of a 2D outflow grid whose rho or p leave the widened 3x3 range."""
+    def prs(U):
+        return (g - 1) * (U[4] - 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0])
+    bad = np.zeros(Uo.shape[1:], bool)
+    for qo, qn in ((Uo[0], Un[0]), (prs(Uo), prs(Un))):
+        P = np.pad(qo[0], 1, mode="symmetric")
+        ny, nx = qo.shape[1:]
+        win = np.stack([P[a:a + ny, b:b + nx] for a in range(3) for b in range(3)])
+        lo, hi = win.min(0), win.max(0)
+        d = np.maximum(fv * np.maximum(np.abs(lo), np.abs(hi)), fr * (hi - lo))
+        bad[0] |= (qn[0] < lo - d) | (qn[0] > hi + d)
+    return int(bad.sum())
 
 
 def test_trained_indicator_gains_over_kxrcf_at_equal_l1():
