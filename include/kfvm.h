@@ -116,11 +116,13 @@ int kfvm_profile_get(kfvm_solver *s, const char *name, double *ms, int64_t *coun
 int kfvm_profile_reset(kfvm_solver *s);
 int kfvm_sync(kfvm_solver *s);
 
-/* ---- learned troubled-cell indicator (Euler) ----
+/* ---- learned troubled-cell indicator (Euler and GLM-MHD) ----
  * None of these has a counterpart in the reference: they add a small network
  * (features -> 32 -> 32 -> logit, bfloat16 weights, FP32 sums) that replaces
  * the KXRCF selection (WenoSelect_K, src/numeric/Numeric_K.H:292-398) of which
- * cells get WENO, trained on the GPU on labels the solver generates. */
+ * cells get WENO, trained on the GPU on labels the solver generates.  The
+ * features are the stencil's density and gas pressure (for GLM-MHD the
+ * pressure without the magnetic and cleaning energies). */
 typedef struct kfvm_indicator kfvm_indicator;
 
 /* Switch the solver to the learned selection with n bfloat16 parameters

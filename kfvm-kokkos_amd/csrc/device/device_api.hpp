@@ -121,16 +121,18 @@ hipError_t launch_finalize(const double *errPartial, const double *posPartial, i
                            double *out, hipStream_t s);
 hipError_t launch_fill(double *p, double v, size_t n, hipStream_t s);
 
-// ---- learned troubled-cell indicator (Euler); P = network parameters
+// ---- learned troubled-cell indicator (Euler nv 5, GLM-MHD nv 9); P = network parameters
 int net_inputs(int dim);  // features per cell
 int net_params(int dim);  // floats in the parameter vector
 int net_train_block();    // samples per training block
-// eint: per-cell internal energy (the first plane of the limiter's cell properties)
-hipError_t launch_learned_select(int dim, const double *U, const double *eint, const double *theta,
+// eint: per-cell internal energy (the first plane of the limiter's cell
+// properties) for nv 5; unused for nv 9
+hipError_t launch_learned_select(int dim, int nv, const double *U, const double *eint, const double *theta,
                                  const double *nadFlag, const float *P, float thr, double *wflag, int *list,
                                  int *counter, const Grid &g, hipStream_t s);
-hipError_t launch_features(int dim, const double *U, const double *eint, float *X, const Grid &g, hipStream_t s);
-hipError_t launch_labels(int dim, int nq, const double *U, const double *FSL, const double *FSW, float tau,
+hipError_t launch_features(int dim, int nv, const double *U, const double *eint, float *X, const Grid &g,
+                           hipStream_t s);
+hipError_t launch_labels(int dim, int nq, int nv, const double *U, const double *FSL, const double *FSW, float tau,
                          float *Y, const Grid &g, hipStream_t s);
 hipError_t launch_logits(int dim, const float *X, const float *P, float *Z, int n, hipStream_t s);
 hipError_t launch_train_grad(int dim, const float *X, const float *Y, const int *idx, int n, int nb,
@@ -140,7 +142,7 @@ hipError_t launch_train_reduce(const float *part, const float *lossPart, int nbl
                                float *loss, hipStream_t s);
 hipError_t launch_adam(float *M, float *Pbf, const float *grad, float *m1, float *m2, int np, float lr,
                        float b1, float b2, float eps, float c1, float c2, hipStream_t s);
-hipError_t launch_nad(int dim, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
-                      int *count, const Grid &g, hipStream_t s);
+hipError_t launch_nad(int dim, int nv, const double *Uold, const double *Unew, double fr, double fv,
+                      double *nadFlag, int *count, const Grid &g, hipStream_t s);
 
 }  // namespace kfvm

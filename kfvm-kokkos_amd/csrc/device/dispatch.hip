@@ -142,17 +142,21 @@ int net_inputs(int dim) { return dim == 3 ? dev::Net<3>::NIN : dev::Net<2>::NIN;
 int net_params(int dim) { return dim == 3 ? dev::Net<3>::NP : dev::Net<2>::NP; }
 int net_train_block() { return dev::TB; }
 
-hipError_t launch_learned_select(int dim, const double *U, const double *eint, const double *theta,
+hipError_t launch_learned_select(int dim, int nv, const double *U, const double *eint, const double *theta,
                                  const double *nadFlag, const float *P, float thr, double *wflag, int *list,
                                  int *counter, const Grid &g, hipStream_t s) {
-  KFVM_SELECT(dim, 2, L::learned_select(U, eint, theta, nadFlag, P, thr, wflag, list, counter, g, s));
+  if (nv != 5 && nv != 9) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, 2, L::learned_select(nv, U, eint, theta, nadFlag, P, thr, wflag, list, counter, g, s));
 }
-hipError_t launch_features(int dim, const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {
-  KFVM_SELECT(dim, 2, L::features(U, eint, X, g, s));
+hipError_t launch_features(int dim, int nv, const double *U, const double *eint, float *X, const Grid &g,
+                           hipStream_t s) {
+  if (nv != 5 && nv != 9) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, 2, L::features(nv, U, eint, X, g, s));
 }
-hipError_t launch_labels(int dim, int nq, const double *U, const double *FSL, const double *FSW, float tau,
+hipError_t launch_labels(int dim, int nq, int nv, const double *U, const double *FSL, const double *FSW, float tau,
                          float *Y, const Grid &g, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::labels(U, FSL, FSW, tau, Y, g, s));
+  if (nv != 5 && nv != 9) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::labels(nv, U, FSL, FSW, tau, Y, g, s));
 }
 hipError_t launch_logits(int dim, const float *X, const float *P, float *Z, int n, hipStream_t s) {
   KFVM_SELECT(dim, 2, L::logits(X, P, Z, n, s));
@@ -174,9 +178,10 @@ hipError_t launch_adam(float *M, float *Pbf, const float *grad, float *m1, float
                      b2, eps, c1, c2);
   return hipGetLastError();
 }
-hipError_t launch_nad(int dim, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
-                      int *count, const Grid &g, hipStream_t s) {
-  KFVM_SELECT(dim, 2, L::nad(Uold, Unew, fr, fv, nadFlag, count, g, s));
+hipError_t launch_nad(int dim, int nv, const double *Uold, const double *Unew, double fr, double fv,
+                      double *nadFlag, int *count, const Grid &g, hipStream_t s) {
+  if (nv != 5 && nv != 9) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, 2, L::nad(nv, Uold, Unew, fr, fv, nadFlag, count, g, s));
 }
 
 hipError_t launch_fill(double *p, double v, size_t n, hipStream_t s) {

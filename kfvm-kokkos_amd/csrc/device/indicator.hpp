@@ -34,6 +34,21 @@ struct Net {
                        oB3 = oW3 + H, NP = oB3 + 1;
 };
 
+// Internal energy of cell idx: the limiter's per-cell plane (Euler) or from
+// the nine GLM-MHD variables (E - |B|^2/2 - |m|^2/(2 rho) - psi^2/2, the
+// order of MHD_GLM_EOS.H); both are even in the momenta and the field.
+template <int NV>
+__device__ inline double cell_eint(const double *__restrict__ U, const double *__restrict__ eint, size_t idx,
+                                   size_t nc) {
+  if (NV == 9) {
+    double s[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) s[v] = U[size_t(v) * nc + idx];
+    return Mhd::eint(s);
+  }
+  return eint[idx];
+}
+
 // Index of the centre cell in the stencil order.
 template <int D>
 constexpr int centre_index() {
@@ -46,7 +61,7 @@ constexpr int centre_index() {
 // energy E - |m|^2 / (2 rho) (the limiter's cell properties), so p = (g-1) eint
 // is the pressure of Hydro_EOS.H bit for bit; both are even in the momenta, so
 // mirrored ghost cells need no sign.
-template <int D>
+template <int D, int NV>
 __device__ inline void cell_features(const double *__restrict__ U, const double *__restrict__ eint,
                                      const Grid &g, int i, int j, int k, float *x) {
   constexpr auto &ST = StTab<D>::ST;
@@ -65,7 +80,7 @@ __device__ inline void cell_features(const double *__restrict__ U, const double 
     else
       idx = map_cell<D, 5>(g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2]).idx;
     rho[n] = U[idx];
-    pr[n] = gm1 * eint[idx];
+    pr[n] = gm1 * cell_eint<NV>(U, eint, idx, g.ncell);
   }
   double sr = 0.0, sp = 0.0;
 #pragma unroll
@@ -129,7 +144,7 @@ __device__ inline void wave_append(bool flagged, int c, int *__restrict__ list, 
 // block per brick of the linear reconstruction's geometry: rho and p of the
 // brick and its radius-2 halo are staged in LDS once, and each cell's
 // features (the same arithmetic as cell_features) read them from there.
-template <int D>
+template <int D, int NV>
 __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__restrict__ U,
                                                                 const double *__restrict__ eint,
                                                                 const double *__restrict__ theta,
@@ -152,9 +167,9 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__
     const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + cz - H : k0;
     // box cells beyond the grid's far edges by more than the halo are never read
     if (gi < g.nx + H && gj < g.ny + H && (D == 2 || gk < g.nz + H)) {
-      const size_t q = map_cell<D, 5>(g, gi, gj, gk).idx;
+      const size_t q = map_cell<D, NV>(g, gi, gj, gk).idx;
       sr[idx] = U[q];
-      sp[idx] = gm1 * eint[q];
+      sp[idx] = gm1 * cell_eint<NV>(U, eint, q, g.ncell);
     }
   }
   __syncthreads();
@@ -195,7 +210,7 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__
 }
 
 // Features of every cell, feature-major rows: X[c][NIN] (training data).
-template <int D>
+template <int D, int NV>
 __global__ __launch_bounds__(BLK) void k_features(const double *__restrict__ U, const double *__restrict__ eint,
                                                   float *__restrict__ X, Grid g) {
   const int c = blockIdx.x * BLK + threadIdx.x;
@@ -203,7 +218,7 @@ __global__ __launch_bounds__(BLK) void k_features(const double *__restrict__ U, 
   int i, j, k;
   cell_ijk<D>(c, g, i, j, k);
   float x[Net<D>::NIN];
-  cell_features<D>(U, eint, g, i, j, k, x);
+  cell_features<D, NV>(U, eint, g, i, j, k, x);
 #pragma unroll
   for (int n = 0; n < Net<D>::NIN; n++) X[size_t(c) * Net<D>::NIN + n] = x[n];
 }
@@ -221,7 +236,7 @@ __global__ __launch_bounds__(BLK) void k_logits(const float *__restrict__ X, con
 }
 
 // Labels from the linear (FSL) and full-WENO (FSW) face states of every cell.
-template <int D, int NQ>
+template <int D, int NQ, int NV>
 __global__ __launch_bounds__(BLK) void k_labels(const double *__restrict__ U, const double *__restrict__ FSL,
                                                 const double *__restrict__ FSW, float tau,
                                                 float *__restrict__ Y, Grid g) {
@@ -236,7 +251,7 @@ __global__ __launch_bounds__(BLK) void k_labels(const double *__restrict__ U, co
                                                                             -1.7976931348623157e308};
 #pragma unroll
   for (int n = 0; n < C; n++) {
-    const CellRef r = map_cell<D, 5>(g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2]);
+    const CellRef r = map_cell<D, NV>(g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2]);
     const double d = ld(U, nc, r, 0), e = ld(U, nc, r, 4);
     lo[0] = fmin(lo[0], d);
     hi[0] = fmax(hi[0], d);
@@ -248,7 +263,7 @@ __global__ __launch_bounds__(BLK) void k_labels(const double *__restrict__ U, co
   double d = 0.0;
 #pragma unroll 1
   for (int fq = 0; fq < FQ; fq++) {
-    const size_t o0 = (size_t(fq) * 5 + 0) * nc + c, o4 = (size_t(fq) * 5 + 4) * nc + c;
+    const size_t o0 = (size_t(fq) * NV + 0) * nc + c, o4 = (size_t(fq) * NV + 4) * nc + c;
     d = fmax(d, fabs(FSW[o0] - FSL[o0]) / sc[0]);
     d = fmax(d, fabs(FSW[o4] - FSL[o4]) / sc[1]);
   }
@@ -419,7 +434,7 @@ __global__ __launch_bounds__(BLK) void k_adam(float *__restrict__ M, float *__re
 // violating cells in nadFlag (1 / 0) and counts them.  One block per brick:
 // rho and p of the old state over the brick and a one-cell halo are staged in
 // LDS.
-template <int D>
+template <int D, int NV>
 __global__ __launch_bounds__(RBox<D>::NT) void k_nad(const double *__restrict__ Uold,
                                                      const double *__restrict__ Unew, double fr, double fv,
                                                      double *__restrict__ nadFlag, int *__restrict__ count,
@@ -437,12 +452,12 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_nad(const double *__restrict__ 
     const int a = idx % LX, r = idx / LX, bb = r % LY, cz = r / LY;
     const int gi = i0 + a - 1, gj = j0 + bb - 1, gk = D == 3 ? k0 + cz - 1 : k0;
     if (gi < g.nx + 1 && gj < g.ny + 1 && (D == 2 || gk < g.nz + 1)) {
-      const CellRef cr = map_cell<D, 5>(g, gi, gj, gk);
-      double s[5];
+      const CellRef cr = map_cell<D, NV>(g, gi, gj, gk);
+      double s[NV];
 #pragma unroll
-      for (int v = 0; v < 5; v++) s[v] = ld(Uold, nc, cr, v);
+      for (int v = 0; v < NV; v++) s[v] = ld(Uold, nc, cr, v);
       sr[idx] = s[0];
-      sp[idx] = gm1 * (s[4] - 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0]);
+      sp[idx] = gm1 * (NV == 9 ? Mhd::eint(s) : Hydro::eint(s));
     }
   }
   __syncthreads();
@@ -466,10 +481,10 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_nad(const double *__restrict__ 
           lo[1] = fmin(lo[1], sp[q]);
           hi[1] = fmax(hi[1], sp[q]);
         }
-    double s[5];
+    double s[NV];
 #pragma unroll
-    for (int v = 0; v < 5; v++) s[v] = Unew[size_t(v) * nc + c];
-    const double val[2] = {s[0], gm1 * (s[4] - 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0])};
+    for (int v = 0; v < NV; v++) s[v] = Unew[size_t(v) * nc + c];
+    const double val[2] = {s[0], gm1 * (NV == 9 ? Mhd::eint(s) : Hydro::eint(s))};
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const double del = fmax(fv * fmax(fabs(lo[q]), fabs(hi[q])), fr * (hi[q] - lo[q]));

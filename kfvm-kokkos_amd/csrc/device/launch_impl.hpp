@@ -69,8 +69,9 @@ struct Launch {
                                const double *W0, const double *CW, hipStream_t s) {
     hipLaunchKernelGGL((dev::k_mhd_linear<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
                        g, W0, CW);
-    hipLaunchKernelGGL((dev::k_mhd_indicator<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, FS, WI,
-                       g);
+    if (WI)  // the learned selection does not read the KXRCF indicator
+      hipLaunchKernelGGL((dev::k_mhd_indicator<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, FS, WI,
+                         g);
     return hipGetLastError();
   }
   static hipError_t mhd_select(const double *U, const double *WI, const double *theta, double *wflag,
@@ -105,21 +106,34 @@ struct Launch {
     return hipGetLastError();
   }
   // ---- learned troubled-cell indicator
-  static hipError_t learned_select(const double *U, const double *eint, const double *theta, const double *nadFlag,
-                                   const float *P, float thr, double *wflag, int *list, int *counter,
-                                   const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_learned_select<D>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint, theta,
-                       nadFlag, P, thr, wflag, list, counter, g);
+  // nv 5 (Euler; eint = the limiter's internal-energy plane) or 9 (GLM-MHD;
+  // the internal energy comes from the conserved variables)
+  static hipError_t learned_select(int nv, const double *U, const double *eint, const double *theta,
+                                   const double *nadFlag, const float *P, float thr, double *wflag, int *list,
+                                   int *counter, const Grid &g, hipStream_t s) {
+    if (nv == 9)
+      hipLaunchKernelGGL((dev::k_learned_select<D, 9>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
+                         theta, nadFlag, P, thr, wflag, list, counter, g);
+    else
+      hipLaunchKernelGGL((dev::k_learned_select<D, 5>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
+                         theta, nadFlag, P, thr, wflag, list, counter, g);
     return hipGetLastError();
   }
-  static hipError_t features(const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_features<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
+  static hipError_t features(int nv, const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {
+    if (nv == 9)
+      hipLaunchKernelGGL((dev::k_features<D, 9>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
+    else
+      hipLaunchKernelGGL((dev::k_features<D, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
     return hipGetLastError();
   }
-  static hipError_t labels(const double *U, const double *FSL, const double *FSW, float tau, float *Y,
+  static hipError_t labels(int nv, const double *U, const double *FSL, const double *FSW, float tau, float *Y,
                            const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_labels<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW, tau,
-                       Y, g);
+    if (nv == 9)
+      hipLaunchKernelGGL((dev::k_labels<D, NQ, 9>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW,
+                         tau, Y, g);
+    else
+      hipLaunchKernelGGL((dev::k_labels<D, NQ, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW,
+                         tau, Y, g);
     return hipGetLastError();
   }
   static hipError_t logits(const float *X, const float *P, float *Z, int n, hipStream_t s) {
@@ -133,10 +147,14 @@ struct Launch {
                        part, lossPart);
     return hipGetLastError();
   }
-  static hipError_t nad(const double *Uold, const double *Unew, double fr, double fv, double *nadFlag, int *count,
-                        const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_nad<D>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
-                       nadFlag, count, g);
+  static hipError_t nad(int nv, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
+                        int *count, const Grid &g, hipStream_t s) {
+    if (nv == 9)
+      hipLaunchKernelGGL((dev::k_nad<D, 9>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
+                         nadFlag, count, g);
+    else
+      hipLaunchKernelGGL((dev::k_nad<D, 5>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
+                         nadFlag, count, g);
     return hipGetLastError();
   }
   static dim3 flux_grid(const Grid &g) {  // one-dimensional: the kernel renumbers the bricks

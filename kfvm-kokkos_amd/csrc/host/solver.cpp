@@ -465,13 +465,21 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
   lastRhsFull_ = !useSparseWeno;
   check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
   if (useSparseWeno) {
+    const bool lrn = learned_ && !forcePlain_;
     timed("linear", [&] {
-      return launch_mhd_linear(dim, nq, Uin, FS_, SRC_, WI_, grid_, faceW_, cellW_, stream_);
+      return launch_mhd_linear(dim, nq, Uin, FS_, SRC_, lrn ? nullptr : WI_, grid_, faceW_, cellW_, stream_);
     });
-    const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
-    timed("select", [&] {
-      return launch_mhd_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
-    });
+    if (lrn) {
+      timed("select", [&] {
+        return launch_learned_select(dim, 9, Uin, nullptr, thPrev, nadFlag_, netP_, netThr_, wflag_, list_,
+                                     counter_, grid_, stream_);
+      });
+    } else {
+      const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
+      timed("select", [&] {
+        return launch_mhd_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
+      });
+    }
     timed("weno", [&] {
       return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceW_, derivW_, cellW_, list_, counter_,
                              stream_);
@@ -519,7 +527,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     if (hydro) {
       if (learned_ && !forcePlain_) {
         timed("select", [&] {
-          return launch_learned_select(dim, Uin, props_, thPrev, nadFlag_, netP_, netThr_, wflag_, list_, counter_,
+          return launch_learned_select(dim, 5, Uin, props_, thPrev, nadFlag_, netP_, netThr_, wflag_, list_, counter_,
                                        grid_, stream_);
         });
       } else {
@@ -703,7 +711,7 @@ int Solver::takeStep() {
 bool Solver::nadFails() {
   check(hipMemsetAsync(nadCount_, 0, sizeof(int), stream_), "clear NAD count");
   timed("nad", [&] {
-    return launch_nad(cfg_.dim, Uprev_, U_[cur_], nadRange_, nadValue_, nadFlag_, nadCount_, grid_, stream_);
+    return launch_nad(cfg_.dim, nv_, Uprev_, U_[cur_], nadRange_, nadValue_, nadFlag_, nadCount_, grid_, stream_);
   });
   int cnt = 0;
   check(hipMemcpyAsync(&hostRed_[3], nadCount_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read NAD count");
@@ -720,7 +728,8 @@ void Solver::setIndicator(const float *params, int np, double threshold) {
     learned_ = false;
     return;
   }
-  if (nv_ != 5) throw std::runtime_error("the learned indicator is built for the Euler equations");
+  if (nv_ != 5 && nv_ != 9)
+    throw std::runtime_error("the learned indicator is built for the Euler and GLM-MHD equations");
   if (np != net_params(cfg_.dim)) throw std::runtime_error("indicator parameter count does not match the dimension");
   if (!netP_) netP_ = dalloc<float>(size_t(np), "indicator parameters");
   if (!nadCount_) nadCount_ = dalloc<int>(1, "NAD count");
@@ -733,12 +742,25 @@ void Solver::setIndicator(const float *params, int np, double threshold) {
 }
 
 void Solver::collectSamples(float *X, float *Y, double tau) {
-  if (nv_ != 5) throw std::runtime_error("the learned indicator is built for the Euler equations");
+  if (nv_ != 5 && nv_ != 9)
+    throw std::runtime_error("the learned indicator is built for the Euler and GLM-MHD equations");
   const int dim = cfg_.dim, nq = cfg_.nQuad;
   const double *Uin = stateInCur_ ? U_[cur_] : Uprev_;
   double *thScratch = theta_[th_ ^ 1];
   const size_t fsBytes = size_t(ncell_) * wts_.F * wts_.Q * nv_ * sizeof(double);
   if (!FSlin_) FSlin_ = dalloc<double>(fsBytes / sizeof(double), "linear face states");
+  if (nv_ == 9) {
+    // labels compare the linear face states with the limited WENO ones
+    check(launch_mhd_linear(dim, nq, Uin, FS_, SRC_, nullptr, grid_, faceW_, cellW_, stream_), "mhd_linear");
+    check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
+    check(launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceW_, derivW_, cellW_, nullptr, nullptr, stream_),
+          "mhd_weno");
+    check(launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thScratch, grid_, stream_), "mhd_pp");
+    check(launch_labels(dim, nq, 9, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
+    check(launch_features(dim, 9, Uin, nullptr, X, grid_, stream_), "features");
+    check(hipStreamSynchronize(stream_), "sync");
+    return;
+  }
   check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
   check(launch_cell_props(dim, Uin, props_, grid_, stream_), "cell_props");
   check(launch_linear(dim, nq, nv_, Uin, FS_, WI_, thScratch, props_, grid_, faceW_, faceWT_, ppList_, counter_ + 1,
@@ -749,8 +771,8 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
                     stream_),
         "weno");
   check(launch_weno_post(dim, nq, Uin, FS_, thScratch, props_, grid_, nullptr, nullptr, stream_), "weno_post");
-  check(launch_labels(dim, nq, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
-  check(launch_features(dim, Uin, props_, X, grid_, stream_), "features");
+  check(launch_labels(dim, nq, 5, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
+  check(launch_features(dim, 5, Uin, props_, X, grid_, stream_), "features");
   check(hipStreamSynchronize(stream_), "sync");
 }
 

@@ -63,7 +63,7 @@ class Solver {
   int64_t nTS = 1, nRhsEval = 0, nRejectUnphys = 0, nRejectThresh = 0, lastNWeno = 0;
   bool lastTimeStep = false, useSparseWeno = true;
 
-  // Learned troubled-cell indicator (Euler).  params: np FP32 network
+  // Learned troubled-cell indicator (Euler, GLM-MHD).  params: np FP32 network
   // parameters holding bfloat16 values, np == 0 returns to the KXRCF selector.
   // While it is on, every accepted step is checked for numerical
   // admissibility: the cells that fail get WENO in every stage of the next

@@ -98,6 +98,17 @@ def _stencil_values(V, off, shape, dim, r=2):
     return out
 
 
+def _eint(s):
+    """Internal energy per volume of the Euler (5 variables, Hydro_EOS.H) or
+    GLM-MHD (9 variables, MHD_GLM_EOS.H) state s, in the device's operation
+    order."""
+    ek = 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0]
+    if len(s) == 9:
+        pb = 0.5 * (s[5] * s[5] + s[6] * s[6] + s[7] * s[7])
+        return s[4] - pb - ek - 0.5 * s[8] * s[8]
+    return s[4] - ek
+
+
 def features(U, off, gamma, bc):
     """Features [ncell, 2C] (float32) of every cell, cells in x-fastest order."""
     dim = 3 if U.shape[1] > 1 else 2
@@ -106,7 +117,7 @@ def features(U, off, gamma, bc):
     o = off.index((0, 0, 0))
     gm1 = gamma - 1.0
     rho = [s[0] for s in S]
-    pr = [gm1 * (s[4] - 0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]) / s[0]) for s in S]
+    pr = [gm1 * _eint(s) for s in S]
     others = [n for n in range(len(off)) if n != o]
     sr = np.zeros_like(rho[o])
     sp = np.zeros_like(rho[o])

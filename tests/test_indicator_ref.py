@@ -57,6 +57,28 @@ def test_features_of_a_density_ramp(dim):
     assert abs(cell[2 * C - 2] - 0.2 / 1.4) < 1e-6 and abs(cell[2 * C - 1]) < 1e-6
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mhd_features_see_the_gas_pressure(dim):
+    """GLM-MHD states whose field and cleaning scalar carry energy
+    (E = eint + |m|^2/(2 rho) + |B|^2/2 + psi^2/2) give the features of the
+    Euler state with the same density, momenta and internal energy."""
+    off = (stencil3d() if dim == 3 else stencil2d())[0]
+    nx, ny, nz = 8, 6, (5 if dim == 3 else 1)
+    g = 5.0 / 3.0
+    rng = np.random.default_rng(7 + dim)
+    rho = rng.uniform(0.5, 2.0, (nz, ny, nx))
+    m = rng.uniform(-1.0, 1.0, (3, nz, ny, nx))
+    B = rng.uniform(-1.0, 1.0, (3, nz, ny, nx))
+    psi = rng.uniform(-0.1, 0.1, (nz, ny, nx))
+    p = rng.uniform(0.2, 3.0, (nz, ny, nx))
+    ek = 0.5 * (m ** 2).sum(0) / rho
+    H = np.concatenate([rho[None], m, (p / (g - 1) + ek)[None]])
+    M = np.concatenate([rho[None], m, (p / (g - 1) + ek + 0.5 * (B ** 2).sum(0) + 0.5 * psi ** 2)[None],
+                        B, psi[None]])
+    bc = ["periodic"] * 6
+    np.testing.assert_allclose(ir.features(M, off, g, bc), ir.features(H, off, g, bc), rtol=0, atol=2e-5)
+
+
 def test_labels_of_hand_made_face_states():
     """2D, two cells differ: a WENO face state moved by 0.3 of the stencil range
     labels its cell at tau = 0.25 and not at tau = 0.35."""
