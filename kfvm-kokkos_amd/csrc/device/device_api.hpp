@@ -87,7 +87,8 @@ hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double
 // GLM-MHD (9 variables).  SRC [(p * 9 + v)][ncell]: states, then source
 // terms, at the NQ^D interior points; CW: interior weights [nSub][NQ^D][C];
 // Dm: differentiation matrix [NQ][NQ + 2]; chBits: the two maxima of the
-// cleaning speed (zeroed by the caller before launch_mhd_speed).
+// cleaning speed over the limited face states (zeroed by the caller before
+// launch_mhd_pp, which limits the states and then takes the maxima).
 hipError_t launch_mhd_linear(int dim, int nq, const double *U, double *FS, double *SRC, double *WI,
                              const Grid &g, const double *W0, const double *CW, hipStream_t s);
 hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI, const double *theta,
@@ -97,9 +98,7 @@ hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double 
                            const double *FW, const double *DW, const double *CW, const int *list,
                            const int *counter, hipStream_t s);
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
-                         const Grid &g, hipStream_t s);
-hipError_t launch_mhd_speed(int dim, int nq, const double *FS, const Grid &g, unsigned long long *chBits,
-                            hipStream_t s);
+                         const Grid &g, unsigned long long *chBits, hipStream_t s);
 hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
                               const double *theta, const Grid &g, const double *Dm, hipStream_t s);
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,

@@ -89,14 +89,9 @@ struct Launch {
     return hipGetLastError();
   }
   static hipError_t mhd_pp(const double *U, double *FS, double *SRC, double *thetaNew, const Grid &g,
-                           hipStream_t s) {
+                           unsigned long long *chBits, hipStream_t s) {
     hipLaunchKernelGGL((dev::k_mhd_pp<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
-                       thetaNew, g);
-    return hipGetLastError();
-  }
-  static hipError_t mhd_speed(const double *FS, const Grid &g, unsigned long long *chBits, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_mhd_speed<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, FS, g,
-                       chBits);
+                       thetaNew, g, chBits);
     return hipGetLastError();
   }
   static hipError_t mhd_sources(const double *U, const double *FS, double *SRC, const double *theta,

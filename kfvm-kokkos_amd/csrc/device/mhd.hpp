@@ -10,9 +10,10 @@
 //   k_mhd_weno       KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504)
 //                    with VarConvert<MHD_GLM> (MHD_GLM_K.H:24-69)
 //   k_mhd_pp         PositivityPreserve_K with interior points
-//                    (Common_K.H:299-643)
-//   k_mhd_speed      SpeedEstimate_K (MHD_GLM_K.H:213-310): the cleaning speed
-//                    c_h = max(c_f + |u|) - max(|u|), x-components on every face
+//                    (Common_K.H:299-643), then SpeedEstimate_K
+//                    (MHD_GLM_K.H:213-310) on the limited face states: the
+//                    cleaning speed c_h = max(c_f + |u|) - max(|u|),
+//                    x-components on every face
 //   k_mhd_sources    SourceTerms_K (MHD_GLM_K.H:316-547): div B and psi-gradient
 //                    terms at the interior points
 #pragma once
@@ -247,106 +248,111 @@ __device__ inline void mhd_scale(double *__restrict__ FS, double *__restrict__ S
   }
 }
 
+// Fastest magnetosonic speed + |u| and |u| of one face state, x-components on
+// every face (SpeedEstimate_K, MHD_GLM_K.H:213-310).
+__device__ inline void mhd_face_speed(double gamma, const double *s, double &maxLam, double &maxVel) {
+  const double vel = fabs(s[1] / s[0]);
+  const double lam = Mhd::fast(gamma, s, s[5]) + vel;
+  maxVel = vel > maxVel ? vel : maxVel;
+  maxLam = lam > maxLam ? lam : maxLam;
+}
+
+// Limiter of every cell, then the cleaning-speed maxima of its limited face
+// states: the energy pass already holds each face state, so the speeds come
+// from those registers unless that pass rescales the states again.  A block
+// folds its maxima into chBits (zeroed by the caller) with one atomic each.
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
                                                 double *__restrict__ FS,
                                                 double *__restrict__ SRC,
-                                                double *__restrict__ thetaNew, Grid g) {
+                                                double *__restrict__ thetaNew, Grid g,
+                                                unsigned long long *__restrict__ chBits) {
   constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
-  int i, j, k;
-  cell_ijk<D>(c, g, i, j, k);
-  const size_t nc = g.ncell;
-  double ref[9];
-#pragma unroll
-  for (int v = 0; v < 9; v++) ref[v] = U[size_t(v) * nc + c];
-  // ranges over the 3^D neighbourhood (Common_K.H:329-387)
-  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
-  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
-  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
-#pragma unroll 1
-  for (int xo = -1; xo <= 1; xo++)
-#pragma unroll 1
-    for (int yo = -1; yo <= 1; yo++)
-#pragma unroll 1
-      for (int zo = zl; zo <= zh; zo++) {
-        const CellRef r = map_cell<D, 9>(g, i + xo, j + yo, k + zo);
-        double s[9];
-#pragma unroll
-        for (int v = 0; v < 9; v++) s[v] = ld(U, nc, r, v);
-        maxDens = fmax(maxDens, s[0]);
-        minDens = fmin(minDens, s[0]);
-        minEint = fmin(minEint, Mhd::eint(s));
-        minSdSp = fmin(minSdSp, Mhd::sound(g.gamma, s));
-      }
-  const double dxU = load_u<D, 9>(U, g, i + 1, j, k, 1) / load_u<D, 9>(U, g, i + 1, j, k, 0) -
-                     load_u<D, 9>(U, g, i - 1, j, k, 1) / load_u<D, 9>(U, g, i - 1, j, k, 0);
-  const double dyV = load_u<D, 9>(U, g, i, j + 1, k, 2) / load_u<D, 9>(U, g, i, j + 1, k, 0) -
-                     load_u<D, 9>(U, g, i, j - 1, k, 2) / load_u<D, 9>(U, g, i, j - 1, k, 0);
-  double dzW;
-  if (D == 3)
-    dzW = load_u<D, 9>(U, g, i, j, k + 1, 3) / load_u<D, 9>(U, g, i, j, k + 1, 0) -
-          load_u<D, 9>(U, g, i, j, k - 1, 3) / load_u<D, 9>(U, g, i, j, k - 1, 0);
-  else
-    dzW = ref[3] / ref[0] - ref[3] / ref[0];
-  const double divV = (dxU + dyV + dzW) / 2.0;
-  double kappa = 0.2;
-  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
-  kappa *= (1.0 - eta);
-  maxDens *= (1.0 + kappa);
-  minDens *= (1.0 - kappa);
-  minEint *= (1.0 - kappa);
-  PPBounds b;
-  b.rDens = ref[0];
-  b.maxDens = maxDens;
-  b.minDens = minDens;
-  b.minEint = minEint;
-  // density: faces then interior points
-  double thD = 1.0;
-#pragma unroll 1
-  for (int p = 0; p < FQ + NI; p++) {
-    const double rho = p < FQ ? FS[size_t(p) * 9 * nc + c] : SRC[size_t(p - FQ) * 9 * nc + c];
-    thD = dens_theta(thD, rho, b);
-  }
-  if (thD > 0.0 && thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, false);
-  else if (thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, true);
-  // internal energy
-  double thP = 1.0;
-#pragma unroll 1
-  for (int p = 0; p < FQ + NI; p++) {
-    const double *q = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
-    double s[9];
-#pragma unroll
-    for (int v = 0; v < 9; v++) s[v] = q[v * nc];
-    const double ei = Mhd::eint(s);
-    thP = ei > minEint ? thP : fmin(thP, mhd_eint_pos(minEint, ref, s));
-  }
-  if (thP > 0.0 && thP < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thP, false);
-  else if (thP < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thP, true);
-  thetaNew[c] = fmin(thD, thP);
-}
-
-// ------------------------------------------------------------ cleaning speed
-template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_speed(const double *__restrict__ FS, Grid g,
-                                                   unsigned long long *__restrict__ chBits) {
-  constexpr int FQ = MShape<D, NQ>::FQ;
   __shared__ double sh[BLK];
   const int c = blockIdx.x * BLK + threadIdx.x;
   double maxLam = 0.0, maxVel = 0.0;
   if (c < g.ncell) {
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
     const size_t nc = g.ncell;
+    double ref[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) ref[v] = U[size_t(v) * nc + c];
+    // ranges over the 3^D neighbourhood (Common_K.H:329-387)
+    double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+    double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+    constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
 #pragma unroll 1
-    for (int fq = 0; fq < FQ; fq++) {
+    for (int xo = -1; xo <= 1; xo++)
+#pragma unroll 1
+      for (int yo = -1; yo <= 1; yo++)
+#pragma unroll 1
+        for (int zo = zl; zo <= zh; zo++) {
+          const CellRef r = map_cell<D, 9>(g, i + xo, j + yo, k + zo);
+          double s[9];
+#pragma unroll
+          for (int v = 0; v < 9; v++) s[v] = ld(U, nc, r, v);
+          maxDens = fmax(maxDens, s[0]);
+          minDens = fmin(minDens, s[0]);
+          minEint = fmin(minEint, Mhd::eint(s));
+          minSdSp = fmin(minSdSp, Mhd::sound(g.gamma, s));
+        }
+    const double dxU = load_u<D, 9>(U, g, i + 1, j, k, 1) / load_u<D, 9>(U, g, i + 1, j, k, 0) -
+                       load_u<D, 9>(U, g, i - 1, j, k, 1) / load_u<D, 9>(U, g, i - 1, j, k, 0);
+    const double dyV = load_u<D, 9>(U, g, i, j + 1, k, 2) / load_u<D, 9>(U, g, i, j + 1, k, 0) -
+                       load_u<D, 9>(U, g, i, j - 1, k, 2) / load_u<D, 9>(U, g, i, j - 1, k, 0);
+    double dzW;
+    if (D == 3)
+      dzW = load_u<D, 9>(U, g, i, j, k + 1, 3) / load_u<D, 9>(U, g, i, j, k + 1, 0) -
+            load_u<D, 9>(U, g, i, j, k - 1, 3) / load_u<D, 9>(U, g, i, j, k - 1, 0);
+    else
+      dzW = ref[3] / ref[0] - ref[3] / ref[0];
+    const double divV = (dxU + dyV + dzW) / 2.0;
+    double kappa = 0.2;
+    const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+    kappa *= (1.0 - eta);
+    maxDens *= (1.0 + kappa);
+    minDens *= (1.0 - kappa);
+    minEint *= (1.0 - kappa);
+    PPBounds b;
+    b.rDens = ref[0];
+    b.maxDens = maxDens;
+    b.minDens = minDens;
+    b.minEint = minEint;
+    // density: faces then interior points
+    double thD = 1.0;
+#pragma unroll 1
+    for (int p = 0; p < FQ + NI; p++) {
+      const double rho = p < FQ ? FS[size_t(p) * 9 * nc + c] : SRC[size_t(p - FQ) * 9 * nc + c];
+      thD = dens_theta(thD, rho, b);
+    }
+    if (thD > 0.0 && thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, false);
+    else if (thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, true);
+    // internal energy (and the speeds of the face states as they stand)
+    double thP = 1.0;
+#pragma unroll 1
+    for (int p = 0; p < FQ + NI; p++) {
+      const double *q = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
       double s[9];
 #pragma unroll
-      for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
-      const double vel = fabs(s[1] / s[0]);
-      const double lam = Mhd::fast(g.gamma, s, s[5]) + vel;
-      maxVel = vel > maxVel ? vel : maxVel;
-      maxLam = lam > maxLam ? lam : maxLam;
+      for (int v = 0; v < 9; v++) s[v] = q[v * nc];
+      const double ei = Mhd::eint(s);
+      thP = ei > minEint ? thP : fmin(thP, mhd_eint_pos(minEint, ref, s));
+      if (p < FQ) mhd_face_speed(g.gamma, s, maxLam, maxVel);
     }
+    if (thP < 1.0) {
+      mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thP, !(thP > 0.0));
+      maxLam = 0.0;
+      maxVel = 0.0;
+#pragma unroll 1
+      for (int fq = 0; fq < FQ; fq++) {
+        double s[9];
+#pragma unroll
+        for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
+        mhd_face_speed(g.gamma, s, maxLam, maxVel);
+      }
+    }
+    thetaNew[c] = fmin(thD, thP);
   }
   const double bl = block_max(maxLam, sh, threadIdx.x);
   const double bv = block_max(maxVel, sh, threadIdx.x);
@@ -356,6 +362,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_speed(const double *__restrict__ FS
   }
 }
 
+// ------------------------------------------------------------ cleaning speed
 __device__ inline double glm_ch(const unsigned long long *chBits) {
   return __longlong_as_double((long long)chBits[0]) - __longlong_as_double((long long)chBits[1]);
 }
