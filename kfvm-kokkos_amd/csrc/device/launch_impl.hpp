@@ -67,11 +67,9 @@ struct Launch {
   // ---- GLM-MHD
   static hipError_t mhd_linear(const double *U, double *FS, double *SRC, double *WI, const Grid &g,
                                const double *W0, const double *CW, hipStream_t s) {
+    // WI null: the learned selection does not read the KXRCF indicator
     hipLaunchKernelGGL((dev::k_mhd_linear<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
-                       g, W0, CW);
-    if (WI)  // the learned selection does not read the KXRCF indicator
-      hipLaunchKernelGGL((dev::k_mhd_indicator<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, FS, WI,
-                         g);
+                       g, W0, CW, WI);
     return hipGetLastError();
   }
   static hipError_t mhd_select(const double *U, const double *WI, const double *theta, double *wflag,

@@ -4,9 +4,9 @@
 // planes [(point * 9 + var)][cell].
 //
 //   k_mhd_linear     KernelLinearRecon_K incl. the source branch
-//                    (Stencil_K.H:32-118)
-//   k_mhd_indicator  p/rho^gamma of every unlimited face state (the KXRCF
-//                    input, Numeric_K.H:315-379)
+//                    (Stencil_K.H:32-118), then p/rho^gamma of every
+//                    unlimited face state (the KXRCF input,
+//                    Numeric_K.H:315-379)
 //   k_mhd_weno       KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504)
 //                    with VarConvert<MHD_GLM> (MHD_GLM_K.H:24-69)
 //   k_mhd_pp         PositivityPreserve_K with interior points
@@ -38,12 +38,16 @@ struct MhdLin {
   }
 };
 
+// Linear face and interior states of every cell; with WI, then the KXRCF
+// indicator p/rho^gamma of each face state, read back from the lines this
+// thread has just written (still in cache).
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U,
                                                     double *__restrict__ FS,
                                                     double *__restrict__ SRC, Grid g,
                                                     const double *__restrict__ W,
-                                                    const double *__restrict__ CW) {
+                                                    const double *__restrict__ CW,
+                                                    double *__restrict__ WI) {
   constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = SInfo<D>::C, FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
   const int c = blockIdx.x * BLK + threadIdx.x;
@@ -72,21 +76,14 @@ __global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U
       SRC[(size_t(p) * 9 + v) * nc + c] = acc;
     }
   }
-}
-
-template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_indicator(const double *__restrict__ FS,
-                                                       double *__restrict__ WI, Grid g) {
-  constexpr int FQ = MShape<D, NQ>::FQ;
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
-  const size_t nc = g.ncell;
+  if (WI) {
 #pragma unroll 1
-  for (int fq = 0; fq < FQ; fq++) {
-    double s[9];
+    for (int fq = 0; fq < FQ; fq++) {
+      double s[9];
 #pragma unroll
-    for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
-    WI[size_t(fq) * nc + c] = Mhd::weno(g.gamma, s);
+      for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
+      WI[size_t(fq) * nc + c] = Mhd::weno(g.gamma, s);
+    }
   }
 }
 
