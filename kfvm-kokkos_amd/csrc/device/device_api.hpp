@@ -75,15 +75,17 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
 // leaves the face states in linearised primitive variables for
 // launch_weno_post; linear advection completes here.
 // FW, DW: reference weight layouts; FWS [nSub][C][F*Q], DWS [nSub][C][I]:
-// stencil-major copies (Euler).
+// stencil-major copies (Euler).  WS (Euler, may be null): compact staging
+// [(F*Q*5)][cap] for the first cap listed cells, shared with launch_weno_post.
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
                        const Grid &g, const double *FW, const double *DW, const double *FWS,
-                       const double *DWS, const int *list, const int *counter, hipStream_t s);
+                       const double *DWS, const int *list, const int *counter, double *WS, int cap,
+                       hipStream_t s);
 // Euler: back to conserved variables and the positivity limiter on the
-// listed cells.
+// listed cells (in WS for the first cap of them, then copied to FS).
 hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,
                             const double *props, const Grid &g, const int *list, const int *counter,
-                            hipStream_t s);
+                            double *WS, int cap, hipStream_t s);
 // GLM-MHD (9 variables).  SRC [(p * 9 + v)][ncell]: states, then source
 // terms, at the NQ^D interior points; CW: interior weights [nSub][NQ^D][C];
 // Dm: differentiation matrix [NQ][NQ + 2]; chBits: the two maxima of the

@@ -39,15 +39,19 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
 
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
                        const Grid &g, const double *FW, const double *DW, const double *FWS,
-                       const double *DWS, const int *list, const int *counter, hipStream_t s) {
+                       const double *DWS, const int *list, const int *counter, double *WS, int cap,
+                       hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, g, nv == 5 ? FWS : FW, nv == 5 ? DWS : DW, list, counter, s));
+  if (WS && cap < 0) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, g, nv == 5 ? FWS : FW, nv == 5 ? DWS : DW, list, counter, WS,
+                               cap, s));
 }
 
 hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,
                             const double *props, const Grid &g, const int *list, const int *counter,
-                            hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::weno_post(U, FS, thetaNew, props, g, list, counter, s));
+                            double *WS, int cap, hipStream_t s) {
+  if (WS && cap < 0) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::weno_post(U, FS, thetaNew, props, g, list, counter, WS, cap, s));
 }
 
 hipError_t launch_mhd_linear(int dim, int nq, const double *U, double *FS, double *SRC, double *WI,

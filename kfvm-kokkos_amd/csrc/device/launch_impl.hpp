@@ -39,11 +39,11 @@ struct Launch {
   }
   static hipError_t weno(int nv, const double *U, double *FS, double *thetaNew, const Grid &g,
                          const double *FW, const double *DW, const int *list, const int *counter,
-                         hipStream_t s) {
+                         double *WS, int cap, hipStream_t s) {
     const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 1u << 16);
     if (nv == 5)
       hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, FW, DW, list,
-                         counter);
+                         counter, WS, WS ? cap : 0);
     else
       hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
                          dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
@@ -58,10 +58,11 @@ struct Launch {
     return hipGetLastError();
   }
   static hipError_t weno_post(const double *U, double *FS, double *thetaNew, const double *props,
-                              const Grid &g, const int *list, const int *counter, hipStream_t s) {
+                              const Grid &g, const int *list, const int *counter, double *WS, int cap,
+                              hipStream_t s) {
     const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
     hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
-                       counter);
+                       counter, WS, WS ? cap : 0);
     return hipGetLastError();
   }
   // ---- GLM-MHD

@@ -396,13 +396,17 @@ struct WenoChunk {  // face points per accumulation chunk
   static constexpr int H = D == 3 ? (NQ == 2 ? 12 : 9) : FQ;
 };
 
+// Listed cells at list positions t < cap write their states compactly to
+// WS[(point * 5 + var) * cap + t] (whole cache lines, read back coalesced by
+// k_weno_post); the others, and the unlisted full pass, write FS in place.
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
                                                   double *__restrict__ FS, Grid g,
                                                   const double *__restrict__ FT,
                                                   const double *__restrict__ DT,
                                                   const int *__restrict__ list,
-                                                  const int *__restrict__ counter) {
+                                                  const int *__restrict__ counter,
+                                                  double *__restrict__ WS, int cap) {
   constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb;
   constexpr int nSub = SInfo<D>::nSub, I = SInfo<D>::I;
@@ -418,6 +422,10 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
 #pragma unroll 1
   for (int t = waveId * CPW + G.slot; t < n; t += nWaves * CPW) {
     const int c = list ? list[t] : t;
+    const bool compact = list && t < cap;
+    double *const P = compact ? WS : FS;
+    const size_t ps = compact ? size_t(cap) : nc;
+    const int pc = compact ? t : c;
     int i, j, k;
     cell_ijk<D>(c, g, i, j, k);
     double ref[5];
@@ -469,7 +477,7 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
       for (int h = 0; h < HF; h++) out[h] = 0.0;
       WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, fc, om, out);
 #pragma unroll
-      for (int h = 0; h < HF; h++) FS[(size_t(h0 + h) * 5 + G.v) * nc + c] = out[h];
+      for (int h = 0; h < HF; h++) P[(size_t(h0 + h) * 5 + G.v) * ps + pc] = out[h];
     }
   }
 }
@@ -587,18 +595,25 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
 // ------------------------------------------------- post-WENO pass
 // After WENO: back to conserved variables about the cell's own average
 // (Hydro_K.H:53-64), then the positivity limiter.
+// States of list positions t < cap come from WS (k_wenorec5), are converted
+// and limited there, and then go to their cell's slots in FS.
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
                                                    double *__restrict__ FS,
                                                    double *__restrict__ thetaNew,
                                                    const double *__restrict__ props, Grid g,
                                                    const int *__restrict__ list,
-                                                   const int *__restrict__ counter) {
+                                                   const int *__restrict__ counter,
+                                                   double *__restrict__ WS, int cap) {
   constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
   const int n = list ? *counter : g.ncell;
   const size_t nc = g.ncell;
   for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
     const int c = list ? list[t] : t;
+    const bool compact = list && t < cap;
+    double *const P = compact ? WS : FS;
+    const size_t ps = compact ? size_t(cap) : nc;
+    const int pc = compact ? t : c;
     int i, j, k;
     cell_ijk<D>(c, g, i, j, k);
     double ref[5];
@@ -611,8 +626,8 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
     const PPBounds pb = pp_bounds_p<D>(U, props, g, i, j, k, ref);
 #pragma unroll 2
     for (int fq = 0; fq < FQ; fq++) {
-      double *p = FS + (size_t(fq) * 5) * nc + c;
-      const double d = p[0], mx = p[nc], my = p[2 * nc], mz = p[3 * nc], e = p[4 * nc];
+      double *p = P + (size_t(fq) * 5) * ps + pc;
+      const double d = p[0], mx = p[ps], my = p[2 * ps], mz = p[3 * ps], e = p[4 * ps];
       double s[5];
       s[4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + e / lp.gamm);
       s[1] = (lp.rho * mx + lp.u * d);
@@ -620,15 +635,19 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
       s[3] = (lp.rho * mz + lp.w * d);
       s[0] = d;
 #pragma unroll
-      for (int w = 1; w < 5; w++) p[w * nc] = s[w];
+      for (int w = 1; w < 5; w++) p[w * ps] = s[w];
       thD = dens_theta(thD, d, pb);
       okE = okE && Hydro::eint(s) > pb.minEint;
     }
     // most WENO cells need the exact sequence: kept inline (a separate
     // lane-parallel pass costs more than the divergence it avoids)
     double th = 1.0;
-    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, c, ref, pb);
+    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(P, ps, pc, ref, pb);
     thetaNew[c] = th;
+    if (compact) {
+#pragma unroll 4
+      for (int q = 0; q < FQ * 5; q++) FS[size_t(q) * nc + c] = WS[size_t(q) * cap + t];
+    }
   }
 }
 

@@ -111,6 +111,10 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   K_ = dalloc<double>(nc * nv_, "K");
   if (rk_.needsTmp()) Utmp_ = dalloc<double>(nc * nv_, "Utmp");
   FS_ = dalloc<double>(nc * FQ * nv_, "face states");
+  if (nv_ == 5) {  // compact WENO staging for up to a quarter of the cells
+    wsCap_ = int(std::max<size_t>(1, (nc + 3) / 4));
+    WS_ = dalloc<double>(size_t(wsCap_) * FQ * nv_, "compact WENO states");
+  }
   WI_ = dalloc<double>(nc * FQ, "indicator");
   props_ = dalloc<double>(nc * 2, "cell properties");
   if (cfg_.eq == Eq::MhdGlm) {
@@ -394,7 +398,7 @@ std::string Solver::writeCkpt() {
 Solver::~Solver() {
   profileEnable(false);
   void *bufs[] = {SRC_, cellW_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -537,10 +541,10 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
       }
       timed("weno", [&] {
         return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, list_,
-                           counter_, stream_);
+                           counter_, WS_, wsCap_, stream_);
       });
       timed("weno_post", [&] {
-        return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, list_, counter_, stream_);
+        return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, list_, counter_, WS_, wsCap_, stream_);
       });
       timed("pp", [&] {
         return launch_pp_lanes(dim, nq, Uin, FS_, thNew, props_, wflag_, grid_, ppList_, counter_ + 1, stream_);
@@ -549,11 +553,11 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
   } else {
     timed("weno", [&] {
       return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr,
-                         nullptr, stream_);
+                         nullptr, nullptr, 0, stream_);
     });
     if (hydro)
       timed("weno_post", [&] {
-        return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, nullptr, nullptr, stream_);
+        return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, nullptr, nullptr, nullptr, 0, stream_);
       });
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
@@ -767,9 +771,10 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
         "linear");
   check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
   check(launch_weno(dim, nq, nv_, Uin, FS_, thScratch, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr, nullptr,
-                    stream_),
+                    nullptr, 0, stream_),
         "weno");
-  check(launch_weno_post(dim, nq, Uin, FS_, thScratch, props_, grid_, nullptr, nullptr, stream_), "weno_post");
+  check(launch_weno_post(dim, nq, Uin, FS_, thScratch, props_, grid_, nullptr, nullptr, nullptr, 0, stream_),
+        "weno_post");
   check(launch_labels(dim, nq, 5, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
   check(launch_features(dim, 5, Uin, props_, X, grid_, stream_), "features");
   check(hipStreamSynchronize(stream_), "sync");

@@ -120,6 +120,8 @@ class Solver {
   int cur_ = 0;
   double *Uprev_ = nullptr, *Uhat_ = nullptr, *Utmp_ = nullptr, *K_ = nullptr;
   double *FS_ = nullptr, *WI_ = nullptr, *wflag_ = nullptr;
+  double *WS_ = nullptr;  // compact WENO states of the first wsCap_ listed cells (Euler)
+  int wsCap_ = 0;
   double *props_ = nullptr;  // per-cell internal energy and sound speed [2][ncell]
   double *theta_[2] = {nullptr, nullptr};  // positivity theta: previous RHS / this RHS
   int th_ = 0;                             // theta_[th_] holds the latest values

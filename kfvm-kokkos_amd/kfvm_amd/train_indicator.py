@@ -137,11 +137,12 @@ EVALUATIONS = {
 }
 
 
-def evaluate(name, weights=None, n=None, log=print):
+def evaluate(name, weights=None, n=None, log=print, modes=("full", "kxrcf", "learned")):
     """Full WENO (the reference's plain WENO), KXRCF-sparse and learned runs of
     the recipe's evaluation problem to its end time: wall time, zone-updates
     per second, mean WENO fraction, and each run's L1 distance (mean over
-    cells and variables) from the full-WENO solution."""
+    cells and variables) from the full-WENO solution (from the KXRCF one when
+    the full-WENO run is left out)."""
     import kfvm_amd as k
     e = EVALUATIONS[name]
     r = RECIPES[name]
@@ -149,7 +150,7 @@ def evaluate(name, weights=None, n=None, log=print):
     size = n or e["n"]
     res = {}
     states = {}
-    for mode in ("full", "kxrcf", "learned"):
+    for mode in modes:
         text = init_text(e["dim"], size, e["problem"], e["user"], e["seed"], e["bc"], final=e["t_end"],
                          wenothresh=-1.0 if mode == "full" else 1.0, gamma=e.get("gamma", 1.4),
                          equation=e.get("equation", "hydro"), riemann=e.get("riemann", "hllc"))
@@ -181,12 +182,19 @@ def evaluate(name, weights=None, n=None, log=print):
             res[mode].update(s.indicator_counters)
         s.close()
         log(f"[evaluate] {name} {mode}: {res[mode]}")
-    for mode in ("kxrcf", "learned"):
-        res[mode]["l1_vs_full_weno"] = float(np.abs(states[mode] - states["full"]).mean())
-    res["l1_ratio_learned_vs_kxrcf"] = res["learned"]["l1_vs_full_weno"] / res["kxrcf"]["l1_vs_full_weno"]
+    if "full" in states:
+        for mode in ("kxrcf", "learned"):
+            if mode in states:
+                res[mode]["l1_vs_full_weno"] = float(np.abs(states[mode] - states["full"]).mean())
+        if "kxrcf" in states and "learned" in states:
+            res["l1_ratio_learned_vs_kxrcf"] = (res["learned"]["l1_vs_full_weno"] /
+                                                res["kxrcf"]["l1_vs_full_weno"])
+    elif "kxrcf" in states and "learned" in states:
+        res["learned"]["l1_vs_kxrcf"] = float(np.abs(states["learned"] - states["kxrcf"]).mean())
     # wall time to the same end time (redone steps count against the learned run)
-    res["speedup_learned_vs_kxrcf"] = res["kxrcf"]["seconds"] / res["learned"]["seconds"]
-    res["speedup_learned_vs_full"] = res["full"]["seconds"] / res["learned"]["seconds"]
+    for ref in ("kxrcf", "full"):
+        if ref in res and "learned" in res:
+            res[f"speedup_learned_vs_{ref}"] = res[ref]["seconds"] / res["learned"]["seconds"]
     return res
 
 
@@ -202,14 +210,16 @@ def main():
     ap.add_argument("--evaluate", action="store_true", help="also run the recipe's evaluation problem")
     ap.add_argument("--evaluate-only", action="store_true")
     ap.add_argument("--n", type=int, default=0, help="evaluation resolution per axis (default: the recipe's)")
+    ap.add_argument("--modes", default="full,kxrcf,learned", help="runs of the evaluation, comma-separated")
     a = ap.parse_args()
     W = None
     if not a.evaluate_only:
         W, _ = train(a.recipe, a.out)
     if a.evaluate or a.evaluate_only:
         dim = EVALUATIONS[a.recipe]["dim"]
-        res = evaluate(a.recipe, W, n=(a.n,) * dim if a.n else None)
-        tag = f"_n{a.n}" if a.n else ""
+        modes = tuple(a.modes.split(","))
+        res = evaluate(a.recipe, W, n=(a.n,) * dim if a.n else None, modes=modes)
+        tag = (f"_n{a.n}" if a.n else "") + ("" if "full" in modes else "_sparse")
         with open(os.path.join(a.out, f"{a.recipe}_evaluation{tag}.json"), "w") as f:
             json.dump(res, f, indent=1)
 
