@@ -96,8 +96,10 @@ hipError_t launch_mhd_linear(int dim, int nq, const double *U, double *FS, doubl
 hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                              double *wflag, int *list, int *counter, double wThresh, const Grid &g,
                              hipStream_t s);
+// FT, DT, CT: stencil-major face [nSub][C][F*Q], smoothness [nSub][C][I] and
+// interior [nSub][C][NQ^D] weights.
 hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
-                           const double *FW, const double *DW, const double *CW, const int *list,
+                           const double *FT, const double *DT, const double *CT, const int *list,
                            const int *counter, hipStream_t s);
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
                          const Grid &g, unsigned long long *chBits, hipStream_t s);

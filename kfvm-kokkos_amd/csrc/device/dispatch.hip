@@ -64,9 +64,9 @@ hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI,
   KFVM_SELECT(dim, nq, L::mhd_select(U, WI, theta, wflag, list, counter, wThresh, g, s));
 }
 hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
-                           const double *FW, const double *DW, const double *CW, const int *list,
+                           const double *FT, const double *DT, const double *CT, const int *list,
                            const int *counter, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FW, DW, CW, list, counter, s));
+  KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FT, DT, CT, list, counter, s));
 }
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
                          const Grid &g, unsigned long long *chBits, hipStream_t s) {

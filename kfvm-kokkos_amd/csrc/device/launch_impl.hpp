@@ -79,11 +79,14 @@ struct Launch {
                        theta, wflag, list, counter, wThresh, g);
     return hipGetLastError();
   }
-  static hipError_t mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FW,
-                             const double *DW, const double *CW, const int *list, const int *counter,
+  static hipError_t mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FT,
+                             const double *DT, const double *CT, const int *list, const int *counter,
                              hipStream_t s) {
-    const unsigned nb = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
-    hipLaunchKernelGGL((dev::k_mhd_weno<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, SRC, g, FW, DW, CW, list,
+    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::MCPB - 1) / dev::MCPB), 1u << 16);
+    hipLaunchKernelGGL((dev::k_mhd_wenorec<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, SRC, g, FT, DT, CT,
+                       list, counter);
+    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
+    hipLaunchKernelGGL((dev::k_mhd_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, g, list,
                        counter);
     return hipGetLastError();
   }

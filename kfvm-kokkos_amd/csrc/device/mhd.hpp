@@ -1,14 +1,15 @@
 // Stage kernels of the GLM-MHD equations (9 variables: rho, mx, my, mz, E,
-// Bx, By, Bz, psi).  One thread per cell; the face states (FS) and the
-// interior quadrature-point states (SRC, needed by the GLM source terms) are
-// planes [(point * 9 + var)][cell].
+// Bx, By, Bz, psi).  One thread per cell (the WENO pass: one lane per cell
+// and variable); the face states (FS) and the interior quadrature-point
+// states (SRC, needed by the GLM source terms) are planes
+// [(point * 9 + var)][cell].
 //
 //   k_mhd_linear     KernelLinearRecon_K incl. the source branch
 //                    (Stencil_K.H:32-118), then p/rho^gamma of every
 //                    unlimited face state (the KXRCF input,
 //                    Numeric_K.H:315-379)
-//   k_mhd_weno       KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504)
-//                    with VarConvert<MHD_GLM> (MHD_GLM_K.H:24-69)
+//   k_mhd_wenorec    KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504)
+//   k_mhd_weno_post  with VarConvert<MHD_GLM> (MHD_GLM_K.H:24-69)
 //   k_mhd_pp         PositivityPreserve_K with interior points
 //                    (Common_K.H:299-643), then SpeedEstimate_K
 //                    (MHD_GLM_K.H:213-310) on the limited face states: the
@@ -87,133 +88,164 @@ __global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U
   }
 }
 
-// One output point of the WENO combination: sum_s omega_s (W_s . V_s).
-template <int D>
-__device__ inline double weno_point(const double (&vs)[SInfo<D>::C], const double *om,
-                                    const double *__restrict__ Wt, int stride, int row) {
-  constexpr auto &ST = StTab<D>::ST;
-  constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb, nSub = SInfo<D>::nSub;
-  double acc = 0.0;
-#pragma unroll
-  for (int s = 0; s < nSub; s++) {
-    const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
-    const double *w = Wt + (size_t(s) * stride + row) * C;
-    double t = 0.0;
-#pragma unroll
-    for (int m = 0; m < n; m++) t = fma(w[m], vs[s < 2 ? m : ST.sub[s - 2][m]], t);
-    acc = fma(om[s], t, acc);
-  }
-  return acc;
-}
-
-// Back to conserved variables (linPrim2Cons) accumulated in the output planes:
-// the variables are processed in the order 0,1,2,3,5,6,7,4,8, which is the
-// operand order of the reference's energy sum
-//   E = vsq d + rho u mx + rho v my + rho w mz + bx Bx + by By + bz Bz
-//       + e / (gamma - 1) + psi0 psi.
-__device__ inline void mhd_store(double *__restrict__ P, size_t nc, int v, double acc,
-                                 const MhdLin &lp) {
-  double *e = P + 4 * nc;
-  if (v == 0) {
-    P[0] = acc;
-    *e = lp.vsq * acc;
-  } else if (v < 4) {
-    const double uv = v == 1 ? lp.u : (v == 2 ? lp.v : lp.w);
-    P[v * nc] = lp.rho * acc + uv * P[0];
-    *e = *e + lp.rho * uv * acc;
-  } else if (v < 8 && v > 4) {
-    const double bv = v == 5 ? lp.bx : (v == 6 ? lp.by : lp.bz);
-    P[v * nc] = acc;
-    *e = *e + bv * acc;
-  } else if (v == 4) {
-    *e = *e + acc / lp.gamm;
-  } else {
-    P[8 * nc] = acc;
-    *e = *e + lp.rPsi * acc;
-  }
-}
+// KFVM-WENO of the listed cells (list == nullptr: all), one lane per (cell,
+// variable): 7 cells of 9 lanes per wave.  The lane gathers its variable's
+// linearised-primitive stencil values, forms the 8 smoothness indicators and
+// the WENO-AO weights, and writes the weighted face and interior values, still
+// linearised primitive (k_mhd_weno_post converts them back).  The products
+// walk the stencil in their outer loop with one accumulator per output (the
+// Euler kernel's pieces: WenoSubs, stencil-major weights FT [nSub][C][F*Q],
+// DT [nSub][C][I], CT [nSub][C][NQ^D]), so each output still sums the stencil
+// cells in ascending order.
+constexpr int MLPC = 9;                  // lanes per cell
+constexpr int MCPW = 7;                  // cells per wave
+constexpr int MCPB = MCPW * (BLK / 64);  // cells per block
 
 template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_weno(const double *__restrict__ U,
-                                                  double *__restrict__ FS,
-                                                  double *__restrict__ SRC, Grid g,
-                                                  const double *__restrict__ FW,
-                                                  const double *__restrict__ DW,
-                                                  const double *__restrict__ CW,
-                                                  const int *__restrict__ list,
-                                                  const int *__restrict__ counter) {
+struct MhdChunk {  // interior points per accumulation chunk
+  static constexpr int NI = MShape<D, NQ>::NI;
+  static constexpr int H = NI > 12 ? 9 : NI;
+};
+
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ U,
+                                                     double *__restrict__ FS,
+                                                     double *__restrict__ SRC, Grid g,
+                                                     const double *__restrict__ FT,
+                                                     const double *__restrict__ DT,
+                                                     const double *__restrict__ CT,
+                                                     const int *__restrict__ list,
+                                                     const int *__restrict__ counter) {
   constexpr auto ST = make_stencil_tables<D>();
-  constexpr int C = SInfo<D>::C, Cc = SInfo<D>::Cc, Cb = SInfo<D>::Cb;
-  constexpr int nSub = SInfo<D>::nSub, I = SInfo<D>::I;
+  constexpr int C = SInfo<D>::C, nSub = SInfo<D>::nSub;
   constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+  constexpr int HF = WenoChunk<D, NQ>::H, HI = MhdChunk<D, NQ>::H;
+  static_assert(FQ % HF == 0 && NI % HI == 0, "chunks must tile the points");
   constexpr double gHi = 0.7, gLo = 0.7;
   constexpr double g0 = gHi, g1 = (1.0 - gHi) * gLo, gb = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
-  const int nt = list ? *counter : g.ncell;
+  const int lane = threadIdx.x & 63;
+  const int slot = lane / MLPC;
+  if (slot >= MCPW) return;  // lane 63 idles
+  const int v = lane - MLPC * slot;
+  const int n = list ? *counter : g.ncell;
+  const int waveId = blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
+  const int nWaves = gridDim.x * (BLK / 64);
   const size_t nc = g.ncell;
-  for (int t = blockIdx.x * BLK + threadIdx.x; t < nt; t += gridDim.x * BLK) {
+#pragma unroll 1
+  for (int t = waveId * MCPW + slot; t < n; t += nWaves * MCPW) {
     const int c = list ? list[t] : t;
     int i, j, k;
     cell_ijk<D>(c, g, i, j, k);
     double ref[9];
 #pragma unroll
-    for (int v = 0; v < 9; v++) ref[v] = U[size_t(v) * nc + c];
+    for (int w = 0; w < 9; w++) ref[w] = U[size_t(w) * nc + c];
     const MhdLin lp(ref, g.gamma);
-#pragma unroll 1
-    for (int vi = 0; vi < 9; vi++) {
-      const int v = vi < 4 ? vi : (vi < 7 ? vi + 1 : (vi == 7 ? 4 : 8));
-      const double uv = v == 1 ? lp.u : (v == 2 ? lp.v : lp.w);
-      // stencil values of variable v, linearised primitive (cons2LinPrim)
-      double vs[C];
+    const double uv = v == 1 ? lp.u : (v == 2 ? lp.v : lp.w);
+    // stencil values of variable v, linearised primitive (cons2LinPrim)
+    double vs[C];
 #pragma unroll
-      for (int n = 0; n < C; n++) {
-        const CellRef r = map_cell<D, 9>(g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2]);
-        if (v == 4) {
-          double s[9];
+    for (int m = 0; m < C; m++) {
+      const CellRef r = map_cell<D, 9>(g, i + ST.off[m][0], j + ST.off[m][1], k + ST.off[m][2]);
+      if (v == 4) {
+        double q[9];
 #pragma unroll
-          for (int w = 0; w < 9; w++) s[w] = ld(U, nc, r, w);
-          vs[n] = lp.gamm * (lp.vsq * s[0] + s[4] - lp.u * s[1] - lp.v * s[2] - lp.w * s[3] -
-                             lp.bx * s[5] - lp.by * s[6] - lp.bz * s[7] - lp.rPsi * s[8]);
-        } else if (v >= 1 && v <= 3) {
-          vs[n] = (ld(U, nc, r, v) - uv * ld(U, nc, r, 0)) / lp.rho;
-        } else {
-          vs[n] = ld(U, nc, r, v);
-        }
+        for (int w = 0; w < 9; w++) q[w] = ld(U, nc, r, w);
+        vs[m] = lp.gamm * (lp.vsq * q[0] + q[4] - lp.u * q[1] - lp.v * q[2] - lp.w * q[3] - lp.bx * q[5] -
+                           lp.by * q[6] - lp.bz * q[7] - lp.rPsi * q[8]);
+      } else if (v >= 1 && v <= 3) {
+        vs[m] = (ld(U, nc, r, v) - uv * ld(U, nc, r, 0)) / lp.rho;
+      } else {
+        vs[m] = ld(U, nc, r, v);
       }
-      // smoothness indicators and WENO-AO weights
-      double om[nSub];
+    }
+    // weight tables opaque to loop-invariant code motion (see k_linear_fused)
+    const double *dt0 = DT, *ft0 = FT, *ct0 = CT;
+    asm volatile("" : "+s"(dt0), "+s"(ft0), "+s"(ct0));
+    const cdouble *dt = (const cdouble *)dt0;
+    // smoothness indicators and WENO-AO weights
+    double om[nSub];
+    WenoSubs<D, 0, nSub>::beta(vs, dt, om);
+    double tot = 0.0;
 #pragma unroll
-      for (int s = 0; s < nSub; s++) {
-        const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
-        double beta = 0.0;
+    for (int s = 0; s < nSub; s++) {
+      const double gs = s == 0 ? g0 : (s == 1 ? g1 : gb);
+      om[s] = gs / (om[s] * om[s] + 1.e-40);
+      tot += om[s];
+    }
+#pragma unroll
+    for (int s = 0; s < nSub; s++) om[s] /= tot;
+    om[0] /= g0;
+#pragma unroll
+    for (int s = 1; s < nSub; s++) om[s] -= om[0] * (s == 1 ? g1 : gb);
+    // face points, then interior points, a chunk at a time
 #pragma unroll 1
-        for (int ii = 0; ii < I; ii++) {
-          const double *dd = DW + (s * I + ii) * C;
-          double tt = 0.0;
+    for (int h0 = 0; h0 < FQ; h0 += HF) {
+      const double *fc0 = ft0 + h0;
+      asm volatile("" : "+s"(fc0));
+      double out[HF];
 #pragma unroll
-          for (int m = 0; m < n; m++) tt = fma(dd[m], vs[s < 2 ? m : ST.sub[s - 2][m]], tt);
-          beta = fma(tt, tt, beta);
-        }
-        om[s] = beta;
-      }
-      double tot = 0.0;
+      for (int h = 0; h < HF; h++) out[h] = 0.0;
+      WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, (const cdouble *)fc0, om, out);
 #pragma unroll
-      for (int s = 0; s < nSub; s++) {
-        const double gs = s == 0 ? g0 : (s == 1 ? g1 : gb);
-        om[s] = gs / (om[s] * om[s] + 1.e-40);
-        tot += om[s];
-      }
-#pragma unroll
-      for (int s = 0; s < nSub; s++) om[s] /= tot;
-      om[0] /= g0;
-#pragma unroll
-      for (int s = 1; s < nSub; s++) om[s] -= om[0] * (s == 1 ? g1 : gb);
-      // face points, then interior points
+      for (int h = 0; h < HF; h++) FS[(size_t(h0 + h) * 9 + v) * nc + c] = out[h];
+    }
 #pragma unroll 1
-      for (int fq = 0; fq < FQ; fq++)
-        mhd_store(FS + size_t(fq) * 9 * nc + c, nc, v, weno_point<D>(vs, om, FW, FQ, fq), lp);
-#pragma unroll 1
-      for (int p = 0; p < NI; p++)
-        mhd_store(SRC + size_t(p) * 9 * nc + c, nc, v, weno_point<D>(vs, om, CW, NI, p), lp);
+    for (int p0 = 0; p0 < NI; p0 += HI) {
+      const double *cc0 = ct0 + p0;
+      asm volatile("" : "+s"(cc0));
+      double out[HI];
+#pragma unroll
+      for (int h = 0; h < HI; h++) out[h] = 0.0;
+      WenoSubs<D, 0, nSub>::template faces<HI, NI>(vs, (const cdouble *)cc0, om, out);
+#pragma unroll
+      for (int h = 0; h < HI; h++) SRC[(size_t(p0 + h) * 9 + v) * nc + c] = out[h];
+    }
+  }
+}
+
+// Back to conserved variables (linPrim2Cons, MHD_GLM_K.H:24-69) at every face
+// and interior point of the listed cells.  The energy sums its terms in the
+// reference's operand order
+//   E = vsq d + rho u mx + rho v my + rho w mz + bx Bx + by By + bz Bz
+//       + e / (gamma - 1) + psi0 psi.
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_weno_post(const double *__restrict__ U,
+                                                       double *__restrict__ FS,
+                                                       double *__restrict__ SRC, Grid g,
+                                                       const int *__restrict__ list,
+                                                       const int *__restrict__ counter) {
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+  const int n = list ? *counter : g.ncell;
+  const size_t nc = g.ncell;
+  for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
+    const int c = list ? list[t] : t;
+    double ref[9];
+#pragma unroll
+    for (int w = 0; w < 9; w++) ref[w] = U[size_t(w) * nc + c];
+    const MhdLin lp(ref, g.gamma);
+#pragma unroll 2
+    for (int p = 0; p < FQ + NI; p++) {
+      double *P = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
+      double a[9];
+#pragma unroll
+      for (int w = 0; w < 9; w++) a[w] = P[w * nc];
+      const double d = a[0];
+      double e = lp.vsq * d;
+      const double mx = lp.rho * a[1] + lp.u * d;
+      e = e + lp.rho * lp.u * a[1];
+      const double my = lp.rho * a[2] + lp.v * d;
+      e = e + lp.rho * lp.v * a[2];
+      const double mz = lp.rho * a[3] + lp.w * d;
+      e = e + lp.rho * lp.w * a[3];
+      e = e + lp.bx * a[5];
+      e = e + lp.by * a[6];
+      e = e + lp.bz * a[7];
+      e = e + a[4] / lp.gamm;
+      e = e + lp.rPsi * a[8];
+      P[nc] = mx;
+      P[2 * nc] = my;
+      P[3 * nc] = mz;
+      P[4 * nc] = e;
     }
   }
 }

@@ -122,6 +122,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
     const int NI = wts_.nqCell;
     SRC_ = dalloc<double>(nc * NI * nv_, "interior point states");
     cellW_ = dalloc<double>(wts_.cell.size(), "interior weights");
+    cellWS_ = dalloc<double>(wts_.cell.size(), "interior weights (stencil-major)");
     diffM_ = dalloc<double>(size_t(qr2.n) * (qr2.n + 2), "differentiation matrix");
     chBits_ = dalloc<unsigned long long>(2, "cleaning speed");
     const std::vector<double> Dm = diff_matrix(cfg_.nQuad);
@@ -397,7 +398,7 @@ std::string Solver::writeCkpt() {
 
 Solver::~Solver() {
   profileEnable(false);
-  void *bufs[] = {SRC_, cellW_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
+  void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
                   faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
@@ -485,12 +486,12 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
       });
     }
     timed("weno", [&] {
-      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceW_, derivW_, cellW_, list_, counter_,
+      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, list_, counter_,
                              stream_);
     });
   } else {
     timed("weno", [&] {
-      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceW_, derivW_, cellW_, nullptr, nullptr,
+      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr,
                              stream_);
     });
     useSparseWeno = cfg_.wenoThresh > 0.0;
@@ -756,7 +757,7 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
     // labels compare the linear face states with the limited WENO ones
     check(launch_mhd_linear(dim, nq, Uin, FS_, SRC_, nullptr, grid_, faceW_, cellW_, stream_), "mhd_linear");
     check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
-    check(launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceW_, derivW_, cellW_, nullptr, nullptr, stream_),
+    check(launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr, stream_),
           "mhd_weno");
     check(launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thScratch, grid_, chBits_, stream_), "mhd_pp");
     check(launch_labels(dim, nq, 9, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
@@ -887,7 +888,8 @@ void Solver::setWeights(const double *face, const double *deriv) {
 // faceWeights[nSub][F][Q][C] and derivWeights[nSub][I][C]; the full-stencil
 // face weights regrouped per pass of P face points as [F*Q/P][C][P] for the
 // linear reconstruction; and stencil-major copies [nSub][C][F*Q] and
-// [nSub][C][I] for the WENO reconstruction.  The Euler kernels walk the
+// [nSub][C][I] (and [nSub][C][NQ^D] for GLM-MHD's interior points) for the
+// WENO reconstruction.  The Euler kernels walk the
 // stencil in their outer loop, so one stencil cell's weights are contiguous.
 void Solver::uploadWeights() {
   const int C = wts_.C, FQ = wts_.F * wts_.Q, I = wts_.I, nSub = wts_.nSub;
@@ -907,9 +909,16 @@ void Solver::uploadWeights() {
   check(hipMemcpy(faceWT_, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(faceWS_, fs.data(), fs.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(derivWS_, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
-  if (cellW_)
+  if (cellW_) {
+    const int NI = wts_.nqCell;
+    std::vector<double> cs(wts_.cell.size());
+    for (int sb = 0; sb < nSub; sb++)
+      for (int n = 0; n < C; n++)
+        for (int p = 0; p < NI; p++) cs[(size_t(sb) * C + n) * NI + p] = wts_.cell[(size_t(sb) * NI + p) * C + n];
     check(hipMemcpy(cellW_, wts_.cell.data(), wts_.cell.size() * sizeof(double), hipMemcpyHostToDevice),
           "copy weights");
+    check(hipMemcpy(cellWS_, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  }
 }
 void Solver::sync() const { check(hipStreamSynchronize(stream_), "sync"); }
 

@@ -129,7 +129,7 @@ class Solver {
   int *ppList_ = nullptr;                      // cells for the limiter's exact sequence
   // GLM-MHD: interior point states / sources, interior weights, differentiation
   // matrix, cleaning-speed maxima
-  double *SRC_ = nullptr, *cellW_ = nullptr, *diffM_ = nullptr;
+  double *SRC_ = nullptr, *cellW_ = nullptr, *cellWS_ = nullptr, *diffM_ = nullptr;
   unsigned long long *chBits_ = nullptr;
   double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr, *faceWS_ = nullptr, *derivWS_ = nullptr;
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;
