@@ -197,6 +197,36 @@ def test_trained_indicator_gains_over_kxrcf_at_equal_l1():
     assert res["learned"][1] < 0.5 * res["plain"][1]
 
 
+def test_mhd_network_tracks_full_weno_like_kxrcf():
+    """The committed GLM-MHD network (trained on 64^3 runs) on the periodic
+    vortex at 96^3, a resolution it never saw, to t = 0.3 with its recipe's
+    threshold: its distance from the full-WENO solution stays within 10% of
+    KXRCF's."""
+    import kfvm_amd
+    from kfvm_amd.train_indicator import RECIPES, load
+    W = load("mhd3d")
+    r = RECIPES["mhd3d"]
+    states, frac = {}, {}
+    for mode in ("full", "kxrcf", "learned"):
+        c = config(n=(96, 96, 96), equation="mhd_glm", riemann="llf", problem="mhd_vortex3d", user=(),
+                   bc="periodic", gamma=5.0 / 3.0, final=0.3, wenothresh=-1.0 if mode == "full" else 1.0)
+        s = kfvm_amd.Solver(init_text=init_text(c))
+        if mode == "learned":
+            s.set_indicator(W, r["threshold"])
+            s.set_nad(r["nad"])
+        nweno = []
+        while s.time[0] < 0.3 - 1e-12:
+            assert s.take_step() in (0, 1)
+            nweno.append(s.counters["last_nweno"])
+        states[mode] = s.state
+        frac[mode] = float(np.mean(nweno)) / 96 ** 3
+        s.close()
+    l1 = {m: float(np.abs(states[m] - states["full"]).mean()) for m in ("kxrcf", "learned")}
+    print(f"[indicator] MHD vortex 96^3: KXRCF L1 {l1['kxrcf']:.4e} WENO {frac['kxrcf']:.3f}; "
+          f"learned L1 {l1['learned']:.4e} WENO {frac['learned']:.3f}", flush=True)
+    assert l1["learned"] <= 1.1 * l1["kxrcf"]
+
+
 @pytest.mark.parametrize("name", ["riemann3d", "quadrant2d", "mhd_blast3d", "orszag_tang2d"])
 def test_learned_selection_flags_the_cells_the_network_picks(name):
     """The selection kernel (features from an LDS-staged brick, network in FP32)
