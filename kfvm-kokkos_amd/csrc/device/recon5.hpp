@@ -246,60 +246,6 @@ __global__ __launch_bounds__(BLK) void k_cell_props(const double *__restrict__ U
   props[nc + c] = Hydro::sound(g.gamma, s);
 }
 
-// Limiter bounds (Common_K.H:329-387) from the per-cell density, internal
-// energy and sound speed of the 3^D neighbourhood.  Both are even in the
-// momenta, so the mirror images of wall boundaries need no sign handling.
-template <int D>
-__device__ inline PPBounds pp_bounds_p(const double *__restrict__ U,
-                                       const double *__restrict__ props, const Grid &g, int i,
-                                       int j, int k, const double *ref) {
-  const size_t nc = g.ncell;
-  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
-  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
-  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
-#pragma unroll 1
-  for (int xo = -1; xo <= 1; xo++) {
-    int sx;
-    const int ii = map_axis(i + xo, g.nx, g.bc[0], g.bc[1], sx);
-#pragma unroll
-    for (int yo = -1; yo <= 1; yo++) {
-      int sy;
-      const int jj = map_axis(j + yo, g.ny, g.bc[2], g.bc[3], sy);
-#pragma unroll
-      for (int zo = zl; zo <= zh; zo++) {
-        int sz;
-        const int kk = D == 3 ? map_axis(k + zo, g.nz, g.bc[4], g.bc[5], sz) : k;
-        const size_t c = (size_t(kk) * g.ny + jj) * g.nx + ii;
-        const double rho = U[c];
-        maxDens = fmax(maxDens, rho);
-        minDens = fmin(minDens, rho);
-        minEint = fmin(minEint, props[c]);
-        minSdSp = fmin(minSdSp, props[nc + c]);
-      }
-    }
-  }
-  const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
-                     load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
-  const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
-                     load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
-  double dzW;
-  if (D == 3)
-    dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
-          load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
-  else
-    dzW = ref[3] / ref[0] - ref[3] / ref[0];
-  const double divV = (dxU + dyV + dzW) / 2.0;
-  double kappa = 0.2;
-  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
-  kappa *= (1.0 - eta);
-  PPBounds b;
-  b.rDens = ref[0];
-  b.maxDens = maxDens * (1.0 + kappa);
-  b.minDens = minDens * (1.0 - kappa);
-  b.minEint = minEint * (1.0 - kappa);
-  return b;
-}
-
 // --------------------------------------------------- KFVM-WENO (faces only)
 // One lane per (cell, variable): linearised-primitive stencil values,
 // smoothness indicators of the full / centre / biased substencils, WENO-AO
@@ -397,7 +343,7 @@ struct WenoChunk {  // face points per accumulation chunk
 };
 
 // Listed cells at list positions t < cap write their states compactly to
-// WS[(point * 5 + var) * cap + t] (whole cache lines, read back coalesced by
+// WS[t][point * 5 + var] (whole cache lines, read back coalesced by
 // k_weno_post); the others, and the unlisted full pass, write FS in place.
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
@@ -423,9 +369,9 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
   for (int t = waveId * CPW + G.slot; t < n; t += nWaves * CPW) {
     const int c = list ? list[t] : t;
     const bool compact = list && t < cap;
-    double *const P = compact ? WS : FS;
-    const size_t ps = compact ? size_t(cap) : nc;
-    const int pc = compact ? t : c;
+    // state q of this cell at P[q * ps]
+    double *const P = compact ? WS + size_t(t) * (FQ * 5) : FS + c;
+    const size_t ps = compact ? 1 : nc;
     int i, j, k;
     cell_ijk<D>(c, g, i, j, k);
     double ref[5];
@@ -477,7 +423,7 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
       for (int h = 0; h < HF; h++) out[h] = 0.0;
       WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, fc, om, out);
 #pragma unroll
-      for (int h = 0; h < HF; h++) P[(size_t(h0 + h) * 5 + G.v) * ps + pc] = out[h];
+      for (int h = 0; h < HF; h++) P[(size_t(h0 + h) * 5 + G.v) * ps] = out[h];
     }
   }
 }
@@ -508,6 +454,55 @@ __device__ inline double grp_max(double x) {
   return x;
 }
 
+// Limiter bounds (Common_K.H:329-387) from the per-cell density, internal
+// energy and sound speed of the 3^D neighbourhood, spread over a group of LPG
+// lanes (lane pl of the group); every lane returns the bounds.  Both
+// quantities are even in the momenta, so the mirror images of wall
+// boundaries need no sign handling.
+template <int D, int LPG>
+__device__ inline PPBounds grp_pp_bounds(const double *__restrict__ U, const double *__restrict__ props,
+                                         const Grid &g, int i, int j, int k, const double *ref, int pl) {
+  constexpr int NB = D == 3 ? 27 : 9;
+  const size_t nc = g.ncell;
+  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+  for (int nb = pl; nb < NB; nb += LPG) {
+    const int xo = D == 3 ? nb / 9 - 1 : nb / 3 - 1;
+    const int yo = D == 3 ? (nb / 3) % 3 - 1 : nb % 3 - 1;
+    const int zo = D == 3 ? nb % 3 - 1 : 0;
+    const CellRef r = map_cell<D, 5>(g, i + xo, j + yo, k + zo);
+    const double rho = U[r.idx];
+    maxDens = fmax(maxDens, rho);
+    minDens = fmin(minDens, rho);
+    minEint = fmin(minEint, props[r.idx]);
+    minSdSp = fmin(minSdSp, props[nc + r.idx]);
+  }
+  maxDens = grp_max<LPG>(maxDens);
+  minDens = grp_min<LPG>(minDens);
+  minEint = grp_min<LPG>(minEint);
+  minSdSp = grp_min<LPG>(minSdSp);
+  const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
+                     load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
+  const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
+                     load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
+  double dzW;
+  if (D == 3)
+    dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
+          load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
+  else
+    dzW = ref[3] / ref[0] - ref[3] / ref[0];
+  const double divV = (dxU + dyV + dzW) / 2.0;
+  double kappa = 0.2;
+  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+  kappa *= (1.0 - eta);
+  PPBounds pb;
+  pb.rDens = ref[0];
+  pb.maxDens = maxDens * (1.0 + kappa);
+  pb.minDens = minDens * (1.0 - kappa);
+  pb.minEint = minEint * (1.0 - kappa);
+  return pb;
+}
+
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
                                                   double *__restrict__ FS,
@@ -517,7 +512,7 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
                                                   const int *__restrict__ list,
                                                   const int *__restrict__ count) {
   constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q, LPG = PPGroup<FQ>::LPG;
-  constexpr int NB = D == 3 ? 27 : 9, GPW = 64 / LPG;
+  constexpr int GPW = 64 / LPG;
   const int n = *count;
   const int lane = threadIdx.x & 63, pl = lane % LPG;
   const int gid = (blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)) * GPW + lane / LPG;
@@ -531,43 +526,7 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
     double ref[5];
 #pragma unroll
     for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
-    // bounds (pp_bounds_p with the neighbourhood spread over the group)
-    double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
-    double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
-    for (int nb = pl; nb < NB; nb += LPG) {
-      const int xo = D == 3 ? nb / 9 - 1 : nb / 3 - 1;
-      const int yo = D == 3 ? (nb / 3) % 3 - 1 : nb % 3 - 1;
-      const int zo = D == 3 ? nb % 3 - 1 : 0;
-      const CellRef r = map_cell<D, 5>(g, i + xo, j + yo, k + zo);
-      const double rho = U[r.idx];
-      maxDens = fmax(maxDens, rho);
-      minDens = fmin(minDens, rho);
-      minEint = fmin(minEint, props[r.idx]);
-      minSdSp = fmin(minSdSp, props[nc + r.idx]);
-    }
-    maxDens = grp_max<LPG>(maxDens);
-    minDens = grp_min<LPG>(minDens);
-    minEint = grp_min<LPG>(minEint);
-    minSdSp = grp_min<LPG>(minSdSp);
-    const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
-                       load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
-    const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
-                       load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
-    double dzW;
-    if (D == 3)
-      dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
-            load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
-    else
-      dzW = ref[3] / ref[0] - ref[3] / ref[0];
-    const double divV = (dxU + dyV + dzW) / 2.0;
-    double kappa = 0.2;
-    const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
-    kappa *= (1.0 - eta);
-    PPBounds pb;
-    pb.rDens = ref[0];
-    pb.maxDens = maxDens * (1.0 + kappa);
-    pb.minDens = minDens * (1.0 - kappa);
-    pb.minEint = minEint * (1.0 - kappa);
+    const PPBounds pb = grp_pp_bounds<D, LPG>(U, props, g, i, j, k, ref, pl);
     // pp_apply with one lane per face point
     const bool own = pl < FQ;
     double st[5];
@@ -594,9 +553,13 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
 
 // ------------------------------------------------- post-WENO pass
 // After WENO: back to conserved variables about the cell's own average
-// (Hydro_K.H:53-64), then the positivity limiter.
-// States of list positions t < cap come from WS (k_wenorec5), are converted
-// and limited there, and then go to their cell's slots in FS.
+// (Hydro_K.H:53-64), then the positivity limiter (Common_K.H:299-643) in its
+// exact sequence, a group of LPG lanes per listed cell and one lane per face
+// point as in k_pp_lanes.  The min over points of each theta is a group
+// reduction, which equals the sequential fold, and when no bound is violated
+// both thetas stay 1 and the states pass unchanged, as after the reference's
+// early exit.  States of list positions t < cap come from WS[t] (k_wenorec5);
+// every state ends in its cell's slots of FS.
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
                                                    double *__restrict__ FS,
@@ -604,50 +567,63 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
                                                    const double *__restrict__ props, Grid g,
                                                    const int *__restrict__ list,
                                                    const int *__restrict__ counter,
-                                                   double *__restrict__ WS, int cap) {
-  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+                                                   const double *__restrict__ WS, int cap) {
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q, LPG = PPGroup<FQ>::LPG;
+  constexpr int GPW = 64 / LPG;
   const int n = list ? *counter : g.ncell;
+  const int lane = threadIdx.x & 63, pl = lane % LPG;
+  const int gid = (blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)) * GPW + lane / LPG;
+  const int ngroups = gridDim.x * (BLK / 64) * GPW;
   const size_t nc = g.ncell;
-  for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
+  const bool own = pl < FQ;
+  for (int t = gid; t < n; t += ngroups) {
     const int c = list ? list[t] : t;
     const bool compact = list && t < cap;
-    double *const P = compact ? WS : FS;
-    const size_t ps = compact ? size_t(cap) : nc;
-    const int pc = compact ? t : c;
     int i, j, k;
     cell_ijk<D>(c, g, i, j, k);
     double ref[5];
 #pragma unroll
     for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
+    const PPBounds pb = grp_pp_bounds<D, LPG>(U, props, g, i, j, k, ref, pl);
     const LinPrim lp(ref, g.gamma);
     const double ru = lp.rho * lp.u, rvv = lp.rho * lp.v, rw = lp.rho * lp.w;
-    double thD = 1.0;
-    bool okE = true;
-    const PPBounds pb = pp_bounds_p<D>(U, props, g, i, j, k, ref);
-#pragma unroll 2
-    for (int fq = 0; fq < FQ; fq++) {
-      double *p = P + (size_t(fq) * 5) * ps + pc;
-      const double d = p[0], mx = p[ps], my = p[2 * ps], mz = p[3 * ps], e = p[4 * ps];
-      double s[5];
-      s[4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + e / lp.gamm);
-      s[1] = (lp.rho * mx + lp.u * d);
-      s[2] = (lp.rho * my + lp.v * d);
-      s[3] = (lp.rho * mz + lp.w * d);
-      s[0] = d;
+    double *const p = FS + (size_t(own ? pl : 0) * 5) * nc + c;
+    double st[5];
+    if (own) {
+      double a[5];
+      if (compact) {
+        const double *q = WS + size_t(t) * (FQ * 5) + pl * 5;
 #pragma unroll
-      for (int w = 1; w < 5; w++) p[w * ps] = s[w];
-      thD = dens_theta(thD, d, pb);
-      okE = okE && Hydro::eint(s) > pb.minEint;
+        for (int w = 0; w < 5; w++) a[w] = q[w];
+      } else {
+#pragma unroll
+        for (int w = 0; w < 5; w++) a[w] = p[w * nc];
+      }
+      const double d = a[0], mx = a[1], my = a[2], mz = a[3], e = a[4];
+      st[4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + e / lp.gamm);
+      st[1] = (lp.rho * mx + lp.u * d);
+      st[2] = (lp.rho * my + lp.v * d);
+      st[3] = (lp.rho * mz + lp.w * d);
+      st[0] = d;
+    } else {
+#pragma unroll
+      for (int w = 0; w < 5; w++) st[w] = ref[w];
     }
-    // most WENO cells need the exact sequence: kept inline (a separate
-    // lane-parallel pass costs more than the divergence it avoids)
-    double th = 1.0;
-    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(P, ps, pc, ref, pb);
-    thetaNew[c] = th;
-    if (compact) {
-#pragma unroll 4
-      for (int q = 0; q < FQ * 5; q++) FS[size_t(q) * nc + c] = WS[size_t(q) * cap + t];
+    const double thD = grp_min<LPG>(own ? dens_theta(1.0, st[0], pb) : 1.0);
+    const bool mixD = thD > 0.0 && thD < 1.0, repD = !mixD && thD < 1.0;
+#pragma unroll
+    for (int w = 0; w < 5; w++) st[w] = repD ? ref[w] : (mixD ? ref[w] + thD * (st[w] - ref[w]) : st[w]);
+    double thE = 1.0;
+    if (own && !(Hydro::eint(st) > pb.minEint)) thE = eint_pos(pb.minEint, ref, st);
+    const double thP = grp_min<LPG>(thE);
+    const bool mixP = thP > 0.0 && thP < 1.0, repP = !mixP && thP < 1.0;
+#pragma unroll
+    for (int w = 0; w < 5; w++) st[w] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[w] - ref[w]) : st[w]);
+    if (own) {
+#pragma unroll
+      for (int w = 0; w < 5; w++) p[w * nc] = st[w];
     }
+    if (pl == 0) thetaNew[c] = fmin(thD, thP);
   }
 }
 
