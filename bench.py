@@ -68,10 +68,15 @@ _SUBSUM = 33 + 7 + 6 * 11                      # 106 stencil columns over all su
 FLOP_WENO = 2 * 19 * 5 * _SUBSUM + 2 * 8 * 19 * 5 + 2 * 6 * 4 * 5 * _SUBSUM  # 47,100
 FLOP_LINEAR = 2 * 6 * 4 * 5 * 33               # 7,920
 BYTES_STAGE = 40 * 5                           # read U_s, Uprev, Uhat; write U_s+1, Uhat
+# Whole-path FP64 work per zone-update of the same configuration, linear path
+# and full WENO (reconstruction, smoothness, WENO faces, Riemann solves at
+# ~200 flop-equivalents each, divergence and RK update): the path's FP64
+# ceiling at a WENO fraction f is 78.6e12 / (F_LIN + f (F_WENO - F_LIN)).
+PATH_F_LIN, PATH_F_WENO = 12.4e3, 51.9e3
 # HBM bytes per launch of a kernel at 256^3, NQ = 2, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes over this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
 # factors a plain copy of known size gives); None where not measured.
-TRAFFIC = {"flux_rk": 25.72e9}
+TRAFFIC = {"flux_rk": 21.88e9}
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -231,6 +236,11 @@ def main():
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = TRAFFIC.get(dom) if n == 256 else None
     roof["kernel"] = dom
+    # the whole path against its FP64 ceiling at the measured WENO fraction
+    f_alg = PATH_F_LIN + head["weno_fraction"] * (PATH_F_WENO - PATH_F_LIN)
+    ceil = PEAK_FP64_TFLOPS * 1e12 / f_alg / 1e6
+    roof["path"] = {"bound": "fp64", "flop_per_zone_update": f_alg, "ceiling_mzone_updates_per_s": ceil,
+                    "frac": head["value"] / ceil}
 
     cpu = None if a.no_cpu else cpu_baseline()
     method = "learned indicator" if learned is not None else "plain (KXRCF)"
