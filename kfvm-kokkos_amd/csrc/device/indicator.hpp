@@ -138,12 +138,50 @@ __device__ inline void wave_append(bool flagged, int c, int *__restrict__ list, 
   if (flagged) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = c;
 }
 
+// ---- the network on the matrix cores
+// v_mfma_f32_16x16x32_bf16: D[16x16] = A[16x32] B[32x16] + C in FP32, lane l
+// holding A[l%16][8(l/16)+i], B[8(l/16)+i][l%16] (i < 8) and
+// D[4(l/16)+i][l%16] (i < 4).  The hidden units are the rows (A = weights,
+// exact in bfloat16), the cells the columns (B = activations).  An FP32
+// activation enters as three bfloat16 terms whose sum is the value to 2^-25,
+// so each product is exact and only the FP32 accumulation order differs from
+// net_logit's.
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ inline void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = (__bf16)x;
+  const float r1 = x - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
+}
+
+// LDS offset (from a cell's brick corner) of the stencil cell behind feature k
+// (the rho and p deviations; 0 for the range features and the padding).
+template <int D>
+constexpr int feature_lds(int k) {
+  using Bx = RBox<D>;
+  constexpr int C = SInfo<D>::C, O = centre_index<D>();
+  int n = -1;
+  if (k < C - 1) n = k < O ? k : k + 1;
+  else if (k < 2 * C - 2) n = (k - (C - 1)) < O ? k - (C - 1) : k - (C - 1) + 1;
+  if (n < 0) return 0;
+  const auto &ST = StTab<D>::ST;
+  return (ST.off[n][0] + Bx::H) + Bx::LX * (ST.off[n][1] + Bx::H) +
+         (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + Bx::H) : 0);
+}
+
 // Learned selection: WENO where the network's logit exceeds thr, where the
 // previous RHS evaluation limited the cell (theta < 1, as the KXRCF selector),
 // or where the last step's admissibility check failed (nadFlag > 0).  One
 // block per brick of the linear reconstruction's geometry: rho and p of the
-// brick and its radius-2 halo are staged in LDS once, and each cell's
-// features (the same arithmetic as cell_features) read them from there.
+// brick and its radius-2 halo are staged in LDS once.  Each thread takes its
+// cell's stencil ranges (the same arithmetic as cell_features); each wave then
+// runs the network for its 64 cells on the matrix cores, 16 cells at a time:
+// lane l builds features 8(l/16)..+7 of cell l%16 of the group straight into
+// the B operand, layer 1 is 2 x ceil(NIN/32) x 3 MFMAs, layer 2 2 x 3 (its
+// K order follows layer 1's output layout, with W2's columns permuted to
+// match), and the last layer a dot product reduced over the four lane groups.
 template <int D, int NV>
 __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__restrict__ U,
                                                                 const double *__restrict__ eint,
@@ -154,56 +192,178 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__
                                                                 int *__restrict__ list,
                                                                 int *__restrict__ counter, Grid g) {
   using Bx = RBox<D>;
+  using N = Net<D>;
   constexpr auto &ST = StTab<D>::ST;
-  constexpr int C = SInfo<D>::C, O = centre_index<D>(), H = Bx::H, NP = Bx::N;
+  constexpr int C = SInfo<D>::C, O = centre_index<D>(), H = Bx::H, NP = Bx::N, NT = Bx::NT;
+  constexpr int NIN = N::NIN, KS = (NIN + 31) / 32;
+  static_assert(NT % 64 == 0 && N::H == 32, "waves of 64 cells, 32 hidden units");
   __shared__ double sr[NP], sp[NP];
+  __shared__ double cr0[NT], cp0[NT], cir[NT], cip[NT];
+  __shared__ float cxr[NT], cxp[NT], cz[NT];
   const int nbx = (g.nx + Bx::BX - 1) / Bx::BX, nby = (g.ny + Bx::BY - 1) / Bx::BY;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int i0 = (b % nbx) * Bx::BX, j0 = ((b / nbx) % nby) * Bx::BY, k0 = (b / (nbx * nby)) * Bx::BZ;
   const int tid = threadIdx.x;
   const double gm1 = g.gamma - 1.0;
-  for (int idx = tid; idx < NP; idx += Bx::NT) {
-    const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, cz = r / Bx::LY;
-    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + cz - H : k0;
+  for (int idx = tid; idx < NP; idx += NT) {
+    const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, cz_ = r / Bx::LY;
+    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + cz_ - H : k0;
     // box cells beyond the grid's far edges by more than the halo are never read
     if (gi < g.nx + H && gj < g.ny + H && (D == 2 || gk < g.nz + H)) {
       const size_t q = map_cell<D, NV>(g, gi, gj, gk).idx;
       sr[idx] = U[q];
       sp[idx] = gm1 * cell_eint<NV>(U, eint, q, g.ncell);
+    } else {
+      sr[idx] = 0.0;
+      sp[idx] = 0.0;
     }
   }
   __syncthreads();
+  auto corner_of = [&](int t) {
+    const int tx = t % Bx::BX, ty = (t / Bx::BX) % Bx::BY, tz = t / (Bx::BX * Bx::BY);
+    return (tz * Bx::LY + ty) * Bx::LX + tx;
+  };
   const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
   const bool live = i < g.nx && j < g.ny && (D == 2 || k < g.nz);
   const int c = live ? (k * g.ny + j) * g.nx + i : 0;
+  {  // this thread's cell: centre values, inverse deviations, range features
+    double r0 = 0.0, p0 = 0.0, ir = 0.0, ip = 0.0;
+    float xr = 0.0f, xp = 0.0f;
+    if (live) {
+      const int corner = corner_of(tid);
+      auto at = [&](int n) {
+        return corner + (ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
+               (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0);
+      };
+      r0 = sr[at(O)];
+      p0 = sp[at(O)];
+      double smr = 0.0, smp = 0.0;
+#pragma unroll
+      for (int n = 0; n < C; n++) {
+        if (n == O) continue;
+        smr = fmax(smr, fabs(sr[at(n)] - r0));
+        smp = fmax(smp, fabs(sp[at(n)] - p0));
+      }
+      ir = 1.0 / (smr + 1e-300);
+      ip = 1.0 / (smp + 1e-300);
+      xr = float(fmin(smr / fabs(r0), 4.0));
+      xp = float(fmin(smp / fabs(p0), 4.0));
+    }
+    cr0[tid] = r0;
+    cp0[tid] = p0;
+    cir[tid] = ir;
+    cip[tid] = ip;
+    cxr[tid] = xr;
+    cxp[tid] = xp;
+  }
+  __syncthreads();
+  // ---- network, per wave on its 64 cells
+  const int lane = tid & 63, lg = lane >> 4, col = lane & 15, wbase = tid & ~63;
+  bf16x8 A1[2][KS], A2[2];
+  float b1v[2][4], b2v[2][4], w3v[2][4];
+  int foff[KS][8];
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int kk = 32 * ks + 8 * lg + e;
+        A1[mt][ks][e] = (__bf16)(kk < NIN ? P[N::oW1 + (16 * mt + col) * NIN + kk] : 0.0f);
+      }
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int j1 = e < 4 ? 4 * lg + e : 16 + 4 * lg + (e - 4);
+      A2[mt][e] = (__bf16)P[N::oW2 + (16 * mt + col) * N::H + j1];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      b1v[mt][e] = P[N::oB1 + 16 * mt + 4 * lg + e];
+      b2v[mt][e] = P[N::oB2 + 16 * mt + 4 * lg + e];
+      w3v[mt][e] = P[N::oW3 + 16 * mt + 4 * lg + e];
+    }
+  }
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int k0f = 32 * ks + e;
+      foff[ks][e] = lg == 0 ? feature_lds<D>(k0f)
+                            : (lg == 1 ? feature_lds<D>(k0f + 8)
+                                       : (lg == 2 ? feature_lds<D>(k0f + 16) : feature_lds<D>(k0f + 24)));
+    }
+  const float b3 = P[N::oB3];
+#pragma unroll 1
+  for (int grp = 0; grp < 4; grp++) {
+    const int cl = wbase + 16 * grp + col;  // the cell (thread index) of this lane's column
+    const int corner = corner_of(cl);
+    const double r0 = cr0[cl], p0 = cp0[cl], ir = cir[cl], ip = cip[cl];
+    const float xr = cxr[cl], xp = cxp[cl];
+    f32x4 acc[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc[mt][e] = b1v[mt][e];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      bf16x8 bh, bm, bl;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int kk = 32 * ks + 8 * lg + e;
+        const double v = kk < C - 1 ? sr[corner + foff[ks][e]] : sp[corner + foff[ks][e]];
+        float x;
+        if (kk < C - 1) x = float((v - r0) * ir);
+        else if (kk < 2 * C - 2) x = float((v - p0) * ip);
+        else if (kk == 2 * C - 2) x = xr;
+        else if (kk == 2 * C - 1) x = xp;
+        else x = 0.0f;
+        __bf16 h, m, l;
+        split3(x, h, m, l);
+        bh[e] = h;
+        bm[e] = m;
+        bl[e] = l;
+      }
+#pragma unroll
+      for (int mt = 0; mt < 2; mt++) {
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[mt][ks], bh, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[mt][ks], bm, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[mt][ks], bl, acc[mt], 0, 0, 0);
+      }
+    }
+    // layer 2: K slot 8(l/16)+e holds hidden unit 4(l/16)+e (e < 4) or 16+4(l/16)+e-4
+    bf16x8 hh, hm, hl;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const float h1 = fmaxf(e < 4 ? acc[0][e] : acc[1][e - 4], 0.0f);
+      __bf16 h, m, l;
+      split3(h1, h, m, l);
+      hh[e] = h;
+      hm[e] = m;
+      hl[e] = l;
+    }
+    f32x4 acc2[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc2[mt][e] = b2v[mt][e];
+      acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[mt], hh, acc2[mt], 0, 0, 0);
+      acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[mt], hm, acc2[mt], 0, 0, 0);
+      acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[mt], hl, acc2[mt], 0, 0, 0);
+    }
+    float z = 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) z = fmaf(w3v[mt][e], fmaxf(acc2[mt][e], 0.0f), z);
+    z += __shfl_xor(z, 16);
+    z += __shfl_xor(z, 32);
+    if (lg == 0) cz[cl] = z + b3;
+  }
+  __syncthreads();
   bool flagged = false;
   if (live) {
-    const int corner = (tz * Bx::LY + ty) * Bx::LX + tx;
-    auto at = [&](int n) {
-      return corner + (ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
-             (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0);
-    };
-    const double r0 = sr[at(O)], p0 = sp[at(O)];
-    double smr = 0.0, smp = 0.0;
-#pragma unroll
-    for (int n = 0; n < C; n++) {
-      if (n == O) continue;
-      smr = fmax(smr, fabs(sr[at(n)] - r0));
-      smp = fmax(smp, fabs(sp[at(n)] - p0));
-    }
-    const double ir = 1.0 / (smr + 1e-300), ip = 1.0 / (smp + 1e-300);
-    float x[Net<D>::NIN];
-#pragma unroll
-    for (int n = 0; n < C; n++) {
-      if (n == O) continue;
-      const int m = n < O ? n : n - 1;
-      x[m] = float((sr[at(n)] - r0) * ir);
-      x[C - 1 + m] = float((sp[at(n)] - p0) * ip);
-    }
-    x[2 * C - 2] = float(fmin(smr / fabs(r0), 4.0));
-    x[2 * C - 1] = float(fmin(smp / fabs(p0), 4.0));
-    flagged = net_logit<D>(P, x) > thr || theta[c] < 1.0 || nadFlag[c] > 0.0;
+    flagged = cz[tid] > thr || theta[c] < 1.0 || nadFlag[c] > 0.0;
     wflag[c] = flagged ? 1.0 : 0.0;
   }
   wave_append(flagged, c, list, counter);
