@@ -197,6 +197,39 @@ def test_trained_indicator_gains_over_kxrcf_at_equal_l1():
     assert res["learned"][1] < 0.5 * res["plain"][1]
 
 
+@pytest.mark.parametrize("name", ["riemann3d", "quadrant2d", "mhd_blast3d"])
+def test_selecting_every_cell_equals_full_weno(name):
+    """A network whose logit is always large lists every cell: the WENO
+    states of the first quarter of the list go through the compact staging
+    rows, the rest are written in place, and two fixed-dt steps match the
+    full-WENO run (no list at all) bit for bit."""
+    import kfvm_amd
+    c = dict(CASES[name])
+    c["fixeddt"] = 0.001
+    dim = c["dim"]
+    _, npar = ir.net_sizes(dim)
+    P = np.zeros(npar, dtype=np.float32)
+    P[-1] = 64.0  # b3
+    W = kfvm_amd.float_to_bf16(P)
+    states = {}
+    for mode in ("full", "learned"):
+        cc = dict(c, wenothresh=-1.0 if mode == "full" else 1.0)
+        s = kfvm_amd.Solver(init_text=init_text(cc))
+        if mode == "learned":
+            s.set_indicator(W, 0.0)
+            s.set_nad(False)
+            s.eval_rhs(0)  # the FSAL register of the constructor came from KXRCF
+        for _ in range(2):
+            assert s.take_step() in (0, 1)
+            if mode == "learned":
+                assert s.counters["last_nweno"] == s.state[0].size
+        states[mode] = s.state
+        s.close()
+    diff = int((states["full"] != states["learned"]).sum())
+    print(f"[indicator] {name}: every cell selected, {diff} values differ from full WENO", flush=True)
+    assert diff == 0
+
+
 def test_mhd_network_tracks_full_weno_like_kxrcf():
     """The committed GLM-MHD network (trained on 64^3 runs) on the periodic
     vortex at 96^3, a resolution it never saw, to t = 0.3 with its recipe's
