@@ -36,7 +36,7 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
-def init_text(n, wenothresh, fixeddt=None):
+def init_text(n, wenothresh, fixeddt=None, nq=2):
     lines = [
         f"mesh:resolution {n} {n} {n}",
         "mesh:extent 0.0 1.0 0.0 1.0 0.0 1.0",
@@ -49,7 +49,7 @@ def init_text(n, wenothresh, fixeddt=None):
         f"eos:wenothresh {wenothresh}",
         "problem:name riemann3d_planar",
         "problem:dim 3",
-        "problem:nquad 2",
+        f"problem:nquad {nq}",
         "problem:equation hydro",
         "problem:riemann hllc",
         "problem:rk ssp4_3_2",
@@ -65,14 +65,20 @@ def init_text(n, wenothresh, fixeddt=None):
 # (FMA = 2 flops), from the stencil sizes C=33, Cc=7, Cb=11, B=6, I=19,
 # F=6 faces, Q=4 points, NV=5.
 _SUBSUM = 33 + 7 + 6 * 11                      # 106 stencil columns over all substencils
-FLOP_WENO = 2 * 19 * 5 * _SUBSUM + 2 * 8 * 19 * 5 + 2 * 6 * 4 * 5 * _SUBSUM  # 47,100
-FLOP_LINEAR = 2 * 6 * 4 * 5 * 33               # 7,920
+
+
+def flop_weno(nq):  # 47,100 at NQ = 2, 78,900 at NQ = 3 (Q = NQ^2 points per face)
+    return 2 * 19 * 5 * _SUBSUM + 2 * 8 * 19 * 5 + 2 * 6 * nq * nq * 5 * _SUBSUM
+
+
+def flop_linear(nq):  # 7,920 at NQ = 2, 17,820 at NQ = 3
+    return 2 * 6 * nq * nq * 5 * 33
 BYTES_STAGE = 40 * 5                           # read U_s, Uprev, Uhat; write U_s+1, Uhat
 # Whole-path FP64 work per zone-update of the same configuration, linear path
 # and full WENO (reconstruction, smoothness, WENO faces, Riemann solves at
 # ~200 flop-equivalents each, divergence and RK update): the path's FP64
 # ceiling at a WENO fraction f is 78.6e12 / (F_LIN + f (F_WENO - F_LIN)).
-PATH_F_LIN, PATH_F_WENO = 12.4e3, 51.9e3
+PATH_F = {2: (12.4e3, 51.9e3), 3: (26.9e3, 87.9e3)}  # (linear path, full WENO) by NQ
 # HBM bytes per launch of a kernel at 256^3, NQ = 2, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes over this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
 # factors a plain copy of known size gives); None where not measured.
@@ -119,7 +125,7 @@ def _mode_run(n, weights, a, profile):
     import numpy as np
     import kfvm_amd
     from kfvm_amd.exact import planar_riemann_l1
-    s = kfvm_amd.Solver(init_text=init_text(n, a.wenothresh))
+    s = kfvm_amd.Solver(init_text=init_text(n, a.wenothresh, nq=a.nq))
     if weights is not None:
         s.set_indicator(weights, 0.0)
     if a.warmup >= 0:
@@ -169,6 +175,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=-1,
                     help="untimed RK steps first; -1: until t = 0.1 (the developed flow)")
     ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--nq", type=int, default=2, choices=(2, 3), help="quadrature points per face direction")
     ap.add_argument("--wenothresh", type=float, default=1.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
@@ -224,20 +231,21 @@ def main():
     d = kern[dom]
     if dom == "weno":
         cells = head["weno_fraction"] * ncell
-        roof = {"bound": "mfma", "achieved": FLOP_WENO * cells / (d["ms_avg"] * 1e-3) / 1e12,
+        roof = {"bound": "mfma", "achieved": flop_weno(a.nq) * cells / (d["ms_avg"] * 1e-3) / 1e12,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
     elif dom == "linear":
-        roof = {"bound": "mfma", "achieved": FLOP_LINEAR * ncell / (d["ms_avg"] * 1e-3) / 1e12,
+        roof = {"bound": "mfma", "achieved": flop_linear(a.nq) * ncell / (d["ms_avg"] * 1e-3) / 1e12,
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
     else:
         # flux/RK kernel: HBM roof on its algorithmic register traffic
         roof = {"bound": "hbm", "achieved": BYTES_STAGE * ncell / (d["ms_avg"] * 1e-3) / 1e9,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = TRAFFIC.get(dom) if n == 256 else None
+    roof["traffic"] = TRAFFIC.get(dom) if n == 256 and a.nq == 2 else None
     roof["kernel"] = dom
     # the whole path against its FP64 ceiling at the measured WENO fraction
-    f_alg = PATH_F_LIN + head["weno_fraction"] * (PATH_F_WENO - PATH_F_LIN)
+    f_lin, f_weno = PATH_F[a.nq]
+    f_alg = f_lin + head["weno_fraction"] * (f_weno - f_lin)
     ceil = PEAK_FP64_TFLOPS * 1e12 / f_alg / 1e6
     roof["path"] = {"bound": "fp64", "flop_per_zone_update": f_alg, "ceiling_mzone_updates_per_s": ceil,
                     "frac": head["value"] / ceil}
@@ -258,7 +266,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3D Euler planar Riemann {n}^3 (x+y+z=1.5, (1,0,0,0,1)/(0.125,0,0,0,0.1)), "
-                               f"HLLC, KFVM-WENO r=2 NQ=2, SSP(4,3,2) adaptive, to t=0.2; value: {method}",
+                               f"HLLC, KFVM-WENO r=2 NQ={a.nq}, SSP(4,3,2) adaptive, to t=0.2; value: {method}",
                    "cells": ncell, "step": "one RK step (4 stages); value counts zone-updates per stage"},
         "plain": plain,
         "learned": learned,

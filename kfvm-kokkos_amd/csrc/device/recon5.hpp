@@ -554,12 +554,19 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
 // ------------------------------------------------- post-WENO pass
 // After WENO: back to conserved variables about the cell's own average
 // (Hydro_K.H:53-64), then the positivity limiter (Common_K.H:299-643) in its
-// exact sequence, a group of LPG lanes per listed cell and one lane per face
-// point as in k_pp_lanes.  The min over points of each theta is a group
-// reduction, which equals the sequential fold, and when no bound is violated
-// both thetas stay 1 and the states pass unchanged, as after the reference's
-// early exit.  States of list positions t < cap come from WS[t] (k_wenorec5);
-// every state ends in its cell's slots of FS.
+// exact sequence, a group of G lanes per listed cell, each lane with the
+// face points pl, pl + G, ... (G = 8 for up to 24 points: 8 cells per wave in
+// flight; 16 for NQ = 3 in 3D).  The min over points of each theta is a lane
+// fold then a group reduction, which equals the sequential fold, and when no
+// bound is violated both thetas stay 1 and the states pass unchanged, as
+// after the reference's early exit.  States of list positions t < cap come
+// from WS[t] (k_wenorec5); every state ends in its cell's slots of FS.
+template <int FQ>
+struct PostGroup {
+  static constexpr int G = FQ <= 24 ? 8 : 16;
+  static constexpr int PPL = (FQ + G - 1) / G;  // points per lane
+};
+
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
                                                    double *__restrict__ FS,
@@ -568,14 +575,13 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
                                                    const int *__restrict__ list,
                                                    const int *__restrict__ counter,
                                                    const double *__restrict__ WS, int cap) {
-  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q, LPG = PPGroup<FQ>::LPG;
-  constexpr int GPW = 64 / LPG;
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  constexpr int G = PostGroup<FQ>::G, PPL = PostGroup<FQ>::PPL, GPW = 64 / G;
   const int n = list ? *counter : g.ncell;
-  const int lane = threadIdx.x & 63, pl = lane % LPG;
-  const int gid = (blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)) * GPW + lane / LPG;
+  const int lane = threadIdx.x & 63, pl = lane % G;
+  const int gid = (blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)) * GPW + lane / G;
   const int ngroups = gridDim.x * (BLK / 64) * GPW;
   const size_t nc = g.ncell;
-  const bool own = pl < FQ;
   for (int t = gid; t < n; t += ngroups) {
     const int c = list ? list[t] : t;
     const bool compact = list && t < cap;
@@ -584,44 +590,59 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
     double ref[5];
 #pragma unroll
     for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
-    const PPBounds pb = grp_pp_bounds<D, LPG>(U, props, g, i, j, k, ref, pl);
+    const PPBounds pb = grp_pp_bounds<D, G>(U, props, g, i, j, k, ref, pl);
     const LinPrim lp(ref, g.gamma);
     const double ru = lp.rho * lp.u, rvv = lp.rho * lp.v, rw = lp.rho * lp.w;
-    double *const p = FS + (size_t(own ? pl : 0) * 5) * nc + c;
-    double st[5];
-    if (own) {
-      double a[5];
-      if (compact) {
-        const double *q = WS + size_t(t) * (FQ * 5) + pl * 5;
+    double st[PPL][5];
+    double thDl = 1.0;
 #pragma unroll
-        for (int w = 0; w < 5; w++) a[w] = q[w];
+    for (int e = 0; e < PPL; e++) {
+      const int pt = pl + G * e;
+      if (pt < FQ) {
+        double a[5];
+        if (compact) {
+          const double *q = WS + size_t(t) * (FQ * 5) + pt * 5;
+#pragma unroll
+          for (int w = 0; w < 5; w++) a[w] = q[w];
+        } else {
+          const double *q = FS + (size_t(pt) * 5) * nc + c;
+#pragma unroll
+          for (int w = 0; w < 5; w++) a[w] = q[w * nc];
+        }
+        const double d = a[0], mx = a[1], my = a[2], mz = a[3], en = a[4];
+        st[e][4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + en / lp.gamm);
+        st[e][1] = (lp.rho * mx + lp.u * d);
+        st[e][2] = (lp.rho * my + lp.v * d);
+        st[e][3] = (lp.rho * mz + lp.w * d);
+        st[e][0] = d;
+        thDl = fmin(thDl, dens_theta(1.0, d, pb));
       } else {
 #pragma unroll
-        for (int w = 0; w < 5; w++) a[w] = p[w * nc];
+        for (int w = 0; w < 5; w++) st[e][w] = ref[w];
       }
-      const double d = a[0], mx = a[1], my = a[2], mz = a[3], e = a[4];
-      st[4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + e / lp.gamm);
-      st[1] = (lp.rho * mx + lp.u * d);
-      st[2] = (lp.rho * my + lp.v * d);
-      st[3] = (lp.rho * mz + lp.w * d);
-      st[0] = d;
-    } else {
-#pragma unroll
-      for (int w = 0; w < 5; w++) st[w] = ref[w];
     }
-    const double thD = grp_min<LPG>(own ? dens_theta(1.0, st[0], pb) : 1.0);
+    const double thD = grp_min<G>(thDl);
     const bool mixD = thD > 0.0 && thD < 1.0, repD = !mixD && thD < 1.0;
+    double thEl = 1.0;
 #pragma unroll
-    for (int w = 0; w < 5; w++) st[w] = repD ? ref[w] : (mixD ? ref[w] + thD * (st[w] - ref[w]) : st[w]);
-    double thE = 1.0;
-    if (own && !(Hydro::eint(st) > pb.minEint)) thE = eint_pos(pb.minEint, ref, st);
-    const double thP = grp_min<LPG>(thE);
+    for (int e = 0; e < PPL; e++) {
+#pragma unroll
+      for (int w = 0; w < 5; w++)
+        st[e][w] = repD ? ref[w] : (mixD ? ref[w] + thD * (st[e][w] - ref[w]) : st[e][w]);
+      if (pl + G * e < FQ && !(Hydro::eint(st[e]) > pb.minEint))
+        thEl = fmin(thEl, eint_pos(pb.minEint, ref, st[e]));
+    }
+    const double thP = grp_min<G>(thEl);
     const bool mixP = thP > 0.0 && thP < 1.0, repP = !mixP && thP < 1.0;
 #pragma unroll
-    for (int w = 0; w < 5; w++) st[w] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[w] - ref[w]) : st[w]);
-    if (own) {
+    for (int e = 0; e < PPL; e++) {
+      const int pt = pl + G * e;
+      if (pt < FQ) {
+        double *const q = FS + (size_t(pt) * 5) * nc + c;
 #pragma unroll
-      for (int w = 0; w < 5; w++) p[w * nc] = st[w];
+        for (int w = 0; w < 5; w++)
+          q[w * nc] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[e][w] - ref[w]) : st[e][w]);
+      }
     }
     if (pl == 0) thetaNew[c] = fmin(thD, thP);
   }
