@@ -581,6 +581,10 @@ struct Brick {
   static constexpr int X = 16, Y = D == 3 ? 4 : 16, Z = D == 3 ? 4 : 1;
   static constexpr int NXF = (X + 1) * Y * Z, NYF = X * (Y + 1) * Z, NZF = D == 3 ? X * Y * (Z + 1) : 0;
   static constexpr int NF = NXF + NYF + NZF;
+  // face work items with each direction padded to whole waves, so that no
+  // wave runs the Riemann solvers of two directions
+  static constexpr int pad64(int n) { return (n + 63) / 64 * 64; }
+  static constexpr int PXF = pad64(NXF), PYF = pad64(NYF), PZF = pad64(NZF), PF = PXF + PYF + PZF;
 };
 
 // Face state of cell (ci,cj,ck) on its face f at point q; cells outside the
@@ -651,21 +655,28 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   for (int q = 0; q < Q; q++) {
     // every face of the brick at point q, solved once
 #pragma unroll 1
-    for (int t = tid; t < Bk::NF; t += BLK) {
-      int dir, fi, fj, fk;
-      if (t < Bk::NXF) {
+    for (int w = tid; w < Bk::PF; w += BLK) {
+      // padded work item w -> face t of the unpadded numbering (x, then y, then z faces)
+      int dir, fi, fj, fk, t;
+      if (w < Bk::PXF) {
+        if (w >= Bk::NXF) continue;
+        t = w;
         dir = 0;
         fi = t % (Bk::X + 1);
         fj = (t / (Bk::X + 1)) % Bk::Y;
         fk = t / ((Bk::X + 1) * Bk::Y);
-      } else if (t < Bk::NXF + Bk::NYF) {
-        const int u = t - Bk::NXF;
+      } else if (w < Bk::PXF + Bk::PYF) {
+        const int u = w - Bk::PXF;
+        if (u >= Bk::NYF) continue;
+        t = Bk::NXF + u;
         dir = 1;
         fi = u % Bk::X;
         fj = (u / Bk::X) % (Bk::Y + 1);
         fk = u / (Bk::X * (Bk::Y + 1));
       } else {
-        const int u = t - Bk::NXF - Bk::NYF;
+        const int u = w - Bk::PXF - Bk::PYF;
+        if (u >= Bk::NZF) continue;
+        t = Bk::NXF + Bk::NYF + u;
         dir = 2;
         fi = u % Bk::X;
         fj = (u / Bk::X) % Bk::Y;
