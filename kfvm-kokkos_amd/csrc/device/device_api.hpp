@@ -101,8 +101,9 @@ hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI,
 hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
                            const double *FT, const double *DT, const double *CT, const int *list,
                            const int *counter, hipStream_t s);
+// props: [2][ncell] scratch for the per-cell internal energy and sound speed
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
-                         const Grid &g, unsigned long long *chBits, hipStream_t s);
+                         const Grid &g, unsigned long long *chBits, double *props, hipStream_t s);
 hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
                               const double *theta, const Grid &g, const double *Dm, hipStream_t s);
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,

@@ -69,8 +69,8 @@ hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double 
   KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FT, DT, CT, list, counter, s));
 }
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
-                         const Grid &g, unsigned long long *chBits, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_pp(U, FS, SRC, thetaNew, g, chBits, s));
+                         const Grid &g, unsigned long long *chBits, double *props, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_pp(U, FS, SRC, thetaNew, g, chBits, props, s));
 }
 hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
                               const double *theta, const Grid &g, const double *Dm, hipStream_t s) {

@@ -91,9 +91,10 @@ struct Launch {
     return hipGetLastError();
   }
   static hipError_t mhd_pp(const double *U, double *FS, double *SRC, double *thetaNew, const Grid &g,
-                           unsigned long long *chBits, hipStream_t s) {
+                           unsigned long long *chBits, double *props, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_mhd_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
     hipLaunchKernelGGL((dev::k_mhd_pp<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
-                       thetaNew, g, chBits);
+                       thetaNew, g, chBits, props);
     return hipGetLastError();
   }
   static hipError_t mhd_sources(const double *U, const double *FS, double *SRC, const double *theta,

@@ -286,6 +286,22 @@ __device__ inline void mhd_face_speed(double gamma, const double *s, double &max
   maxLam = lam > maxLam ? lam : maxLam;
 }
 
+// Per-cell internal energy and sound speed (MHD_GLM_EOS.H), the two
+// neighbourhood quantities of the limiter bounds; both are even in the
+// momenta and the field, so mirrored wall cells need no sign.
+template <int D>
+__global__ __launch_bounds__(BLK) void k_mhd_props(const double *__restrict__ U, double *__restrict__ props,
+                                                   Grid g) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  const size_t nc = g.ncell;
+  double s[9];
+#pragma unroll
+  for (int v = 0; v < 9; v++) s[v] = U[size_t(v) * nc + c];
+  props[c] = Mhd::eint(s);
+  props[nc + c] = Mhd::sound(g.gamma, s);
+}
+
 // Limiter of every cell, then the cleaning-speed maxima of its limited face
 // states: the energy pass already holds each face state, so the speeds come
 // from those registers unless that pass rescales the states again.  A block
@@ -295,7 +311,8 @@ __global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
                                                 double *__restrict__ FS,
                                                 double *__restrict__ SRC,
                                                 double *__restrict__ thetaNew, Grid g,
-                                                unsigned long long *__restrict__ chBits) {
+                                                unsigned long long *__restrict__ chBits,
+                                                const double *__restrict__ props) {
   constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
   __shared__ double sh[BLK];
   const int c = blockIdx.x * BLK + threadIdx.x;
@@ -317,14 +334,12 @@ __global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
       for (int yo = -1; yo <= 1; yo++)
 #pragma unroll 1
         for (int zo = zl; zo <= zh; zo++) {
-          const CellRef r = map_cell<D, 9>(g, i + xo, j + yo, k + zo);
-          double s[9];
-#pragma unroll
-          for (int v = 0; v < 9; v++) s[v] = ld(U, nc, r, v);
-          maxDens = fmax(maxDens, s[0]);
-          minDens = fmin(minDens, s[0]);
-          minEint = fmin(minEint, Mhd::eint(s));
-          minSdSp = fmin(minSdSp, Mhd::sound(g.gamma, s));
+          const size_t q = map_cell<D, 9>(g, i + xo, j + yo, k + zo).idx;
+          const double rho = U[q];
+          maxDens = fmax(maxDens, rho);
+          minDens = fmin(minDens, rho);
+          minEint = fmin(minEint, props[q]);
+          minSdSp = fmin(minSdSp, props[nc + q]);
         }
     const double dxU = load_u<D, 9>(U, g, i + 1, j, k, 1) / load_u<D, 9>(U, g, i + 1, j, k, 0) -
                        load_u<D, 9>(U, g, i - 1, j, k, 1) / load_u<D, 9>(U, g, i - 1, j, k, 0);

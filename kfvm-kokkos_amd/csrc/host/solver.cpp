@@ -497,7 +497,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
   check(hipMemsetAsync(chBits_, 0, 2 * sizeof(unsigned long long), stream_), "clear speeds");
-  timed("pp", [&] { return launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thNew, grid_, chBits_, stream_); });
+  timed("pp", [&] { return launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thNew, grid_, chBits_, props_, stream_); });
   th_ ^= 1;
   timed("sources", [&] { return launch_mhd_sources(dim, nq, Uin, FS_, SRC_, thNew, grid_, diffM_, stream_); });
   RKArgs a = a0;
@@ -759,7 +759,7 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
     check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
     check(launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr, stream_),
           "mhd_weno");
-    check(launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thScratch, grid_, chBits_, stream_), "mhd_pp");
+    check(launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thScratch, grid_, chBits_, props_, stream_), "mhd_pp");
     check(launch_labels(dim, nq, 9, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
     check(launch_features(dim, 9, Uin, nullptr, X, grid_, stream_), "features");
     check(hipStreamSynchronize(stream_), "sync");
