@@ -61,8 +61,11 @@ struct Launch {
                               const Grid &g, const int *list, const int *counter, double *WS, int cap,
                               hipStream_t s) {
     const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
-    hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
-                       counter, WS, WS ? cap : 0);
+    if (list)
+      hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
+                         counter, WS, WS ? cap : 0);
+    else
+      hipLaunchKernelGGL((dev::k_weno_post_cells<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g);
     return hipGetLastError();
   }
   // ---- GLM-MHD

@@ -13,7 +13,8 @@
 //                   neighbourhood quantities of the limiter bounds.
 //   k_wenorec5      Stencil_K.H:164-283, 416-454, 506-589: one lane per
 //                   (cell, variable) over the compacted WENO list.
-//   k_weno_post     Hydro_K.H:53-64 back-conversion + Common_K.H:299-643.
+//   k_weno_post     Hydro_K.H:53-64 back-conversion + Common_K.H:299-643
+//                   over the listed cells (k_weno_post_cells: every cell).
 #pragma once
 
 namespace kfvm {
@@ -551,6 +552,60 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
   }
 }
 
+// Limiter bounds (Common_K.H:329-387) from the per-cell density, internal
+// energy and sound speed of the 3^D neighbourhood.  Both are even in the
+// momenta, so the mirror images of wall boundaries need no sign handling.
+template <int D>
+__device__ inline PPBounds pp_bounds_p(const double *__restrict__ U,
+                                       const double *__restrict__ props, const Grid &g, int i,
+                                       int j, int k, const double *ref) {
+  const size_t nc = g.ncell;
+  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
+#pragma unroll 1
+  for (int xo = -1; xo <= 1; xo++) {
+    int sx;
+    const int ii = map_axis(i + xo, g.nx, g.bc[0], g.bc[1], sx);
+#pragma unroll
+    for (int yo = -1; yo <= 1; yo++) {
+      int sy;
+      const int jj = map_axis(j + yo, g.ny, g.bc[2], g.bc[3], sy);
+#pragma unroll
+      for (int zo = zl; zo <= zh; zo++) {
+        int sz;
+        const int kk = D == 3 ? map_axis(k + zo, g.nz, g.bc[4], g.bc[5], sz) : k;
+        const size_t c = (size_t(kk) * g.ny + jj) * g.nx + ii;
+        const double rho = U[c];
+        maxDens = fmax(maxDens, rho);
+        minDens = fmin(minDens, rho);
+        minEint = fmin(minEint, props[c]);
+        minSdSp = fmin(minSdSp, props[nc + c]);
+      }
+    }
+  }
+  const double dxU = load_u<D, 5>(U, g, i + 1, j, k, 1) / load_u<D, 5>(U, g, i + 1, j, k, 0) -
+                     load_u<D, 5>(U, g, i - 1, j, k, 1) / load_u<D, 5>(U, g, i - 1, j, k, 0);
+  const double dyV = load_u<D, 5>(U, g, i, j + 1, k, 2) / load_u<D, 5>(U, g, i, j + 1, k, 0) -
+                     load_u<D, 5>(U, g, i, j - 1, k, 2) / load_u<D, 5>(U, g, i, j - 1, k, 0);
+  double dzW;
+  if (D == 3)
+    dzW = load_u<D, 5>(U, g, i, j, k + 1, 3) / load_u<D, 5>(U, g, i, j, k + 1, 0) -
+          load_u<D, 5>(U, g, i, j, k - 1, 3) / load_u<D, 5>(U, g, i, j, k - 1, 0);
+  else
+    dzW = ref[3] / ref[0] - ref[3] / ref[0];
+  const double divV = (dxU + dyV + dzW) / 2.0;
+  double kappa = 0.2;
+  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+  kappa *= (1.0 - eta);
+  PPBounds b;
+  b.rDens = ref[0];
+  b.maxDens = maxDens * (1.0 + kappa);
+  b.minDens = minDens * (1.0 - kappa);
+  b.minEint = minEint * (1.0 - kappa);
+  return b;
+}
+
 // ------------------------------------------------- post-WENO pass
 // After WENO: back to conserved variables about the cell's own average
 // (Hydro_K.H:53-64), then the positivity limiter (Common_K.H:299-643) in its
@@ -645,6 +700,48 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
       }
     }
     if (pl == 0) thetaNew[c] = fmin(thD, thP);
+  }
+}
+
+// The same pass over every cell (full WENO, no list): one thread per cell,
+// so that each access of a state plane covers consecutive cells.
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_weno_post_cells(const double *__restrict__ U,
+                                                         double *__restrict__ FS,
+                                                         double *__restrict__ thetaNew,
+                                                         const double *__restrict__ props, Grid g) {
+  constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  const size_t nc = g.ncell;
+  for (int c = blockIdx.x * BLK + threadIdx.x; c < g.ncell; c += gridDim.x * BLK) {
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    double ref[5];
+#pragma unroll
+    for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
+    const LinPrim lp(ref, g.gamma);
+    const double ru = lp.rho * lp.u, rvv = lp.rho * lp.v, rw = lp.rho * lp.w;
+    double thD = 1.0;
+    bool okE = true;
+    const PPBounds pb = pp_bounds_p<D>(U, props, g, i, j, k, ref);
+#pragma unroll 2
+    for (int fq = 0; fq < FQ; fq++) {
+      double *p = FS + (size_t(fq) * 5) * nc + c;
+      const double d = p[0], mx = p[nc], my = p[2 * nc], mz = p[3 * nc], e = p[4 * nc];
+      double st[5];
+      st[4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + e / lp.gamm);
+      st[1] = (lp.rho * mx + lp.u * d);
+      st[2] = (lp.rho * my + lp.v * d);
+      st[3] = (lp.rho * mz + lp.w * d);
+      st[0] = d;
+#pragma unroll
+      for (int w = 1; w < 5; w++) p[w * nc] = st[w];
+      thD = dens_theta(thD, d, pb);
+      okE = okE && Hydro::eint(st) > pb.minEint;
+    }
+    // no bound violated: the states stay as they are (theta 1)
+    double th = 1.0;
+    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, c, ref, pb);
+    thetaNew[c] = th;
   }
 }
 
