@@ -55,8 +55,8 @@ RECIPES = {
                     runs=[dict(n=(m, m), problem="shear2d", user=(), bc="periodic", seed=sd, t_end=1.0,
                                dt_snap=0.1) for m in (256, 512) for sd in range(5)]),
     # 3D GLM-MHD (the evaluation is the periodic vortex at 256^3 to t = 0.5):
-    # plain runs of magnetised blasts at 64^3 and of the vortex at 64^3,
-    # snapshots every 0.05 (blasts) / 0.1 (vortex).  A lower label threshold
+    # plain runs of magnetised blasts at 64^3 and of the vortex at 64^3 and
+    # 128^3, snapshots every 0.05 (blasts) / 0.1 (vortex).  A lower label threshold
     # and logit threshold than Euler's: the flow is smooth but steep almost
     # everywhere.  The admissibility check is off: it flags about 6% of the
     # cells in every step whatever the selection (growing smooth extrema), so
@@ -64,8 +64,8 @@ RECIPES = {
     "mhd3d": dict(dim=3, tau=0.02, threshold=-2.0, nad=False, iters=2000, batch=8192, lr=2e-3, seed=1,
                   runs=[dict(n=(64, 64, 64), problem="mhd_blast", user=u, bc="periodic", t_end=0.2,
                              dt_snap=0.05, **_MHD) for u in ((0.1, 0.5, 0.0), (0.15, 0.0, 1.0))]
-                  + [dict(n=(64, 64, 64), problem="mhd_vortex3d", user=(), bc="periodic", t_end=0.5,
-                          dt_snap=0.1, **_MHD)]),
+                  + [dict(n=(m, m, m), problem="mhd_vortex3d", user=(), bc="periodic", t_end=0.5,
+                          dt_snap=0.1, **_MHD) for m in (64, 128)]),
 }
 
 
