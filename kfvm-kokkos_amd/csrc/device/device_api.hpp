@@ -49,6 +49,10 @@ struct RKArgs {
   // PRIME / FSAL modes of a batch: write K and Uhat only for slices with
   // sliceWrite[s] != 0 (the others keep the values of their accepted step)
   const int *sliceWrite;
+  // Euler sparse WENO: per-cell row in the compact WENO staging (-1: the
+  // cell's states are in FS) and that staging, [t][F*Q*5]
+  const int *pos;
+  const double *ws;
 };
 
 // Euler: per-cell internal energy and sound speed, props [2][ncell], read by
@@ -68,9 +72,10 @@ hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, d
 hipError_t launch_pp_lanes(int dim, int nq, const double *U, double *FS, double *thetaNew,
                            const double *props, const double *wflag, const Grid &g,
                            const int *list, const int *count, hipStream_t s);
+// pos (may be null): per-cell list position below cap, -1 otherwise
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                          double *wflag, int *list, int *counter, double wThresh, const Grid &g,
-                         hipStream_t s);
+                         int *pos, int cap, hipStream_t s);
 // KFVM-WENO reconstruction of the listed cells (list == nullptr: all).  Euler
 // leaves the face states in linearised primitive variables for
 // launch_weno_post; linear advection completes here.
@@ -133,7 +138,7 @@ int net_train_block();    // samples per training block
 // properties) for nv 5; unused for nv 9
 hipError_t launch_learned_select(int dim, int nv, const double *U, const double *eint, const double *theta,
                                  const double *nadFlag, const float *P, float thr, double *wflag, int *list,
-                                 int *counter, const Grid &g, hipStream_t s);
+                                 int *counter, const Grid &g, int *pos, int cap, hipStream_t s);
 hipError_t launch_features(int dim, int nv, const double *U, const double *eint, float *X, const Grid &g,
                            hipStream_t s);
 hipError_t launch_labels(int dim, int nq, int nv, const double *U, const double *FSL, const double *FSW, float tau,

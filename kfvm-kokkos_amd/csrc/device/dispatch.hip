@@ -33,8 +33,8 @@ hipError_t launch_pp_lanes(int dim, int nq, const double *U, double *FS, double 
 
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                          double *wflag, int *list, int *counter, double wThresh, const Grid &g,
-                         hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::select(U, WI, theta, wflag, list, counter, wThresh, g, s));
+                         int *pos, int cap, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::select(U, WI, theta, wflag, list, counter, wThresh, g, pos, cap, s));
 }
 
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
@@ -144,9 +144,9 @@ int net_train_block() { return dev::TB; }
 
 hipError_t launch_learned_select(int dim, int nv, const double *U, const double *eint, const double *theta,
                                  const double *nadFlag, const float *P, float thr, double *wflag, int *list,
-                                 int *counter, const Grid &g, hipStream_t s) {
+                                 int *counter, const Grid &g, int *pos, int cap, hipStream_t s) {
   if (nv != 5 && nv != 9) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, 2, L::learned_select(nv, U, eint, theta, nadFlag, P, thr, wflag, list, counter, g, s));
+  KFVM_SELECT(dim, 2, L::learned_select(nv, U, eint, theta, nadFlag, P, thr, wflag, list, counter, g, pos, cap, s));
 }
 hipError_t launch_features(int dim, int nv, const double *U, const double *eint, float *X, const Grid &g,
                            hipStream_t s) {

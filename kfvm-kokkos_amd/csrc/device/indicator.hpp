@@ -126,18 +126,6 @@ __device__ inline float net_logit(const float *__restrict__ P, const float *x) {
   return z;
 }
 
-__device__ inline void wave_append(bool flagged, int c, int *__restrict__ list, int *__restrict__ counter) {
-  const unsigned long long mask = __ballot(flagged);
-  if (mask == 0ull) return;
-  const int lane = threadIdx.x & 63;
-  const int cnt = __popcll(mask);
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(counter, cnt);
-  base = __shfl(base, leader);
-  if (flagged) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = c;
-}
-
 // ---- the network on the matrix cores
 // v_mfma_f32_16x16x32_bf16: D[16x16] = A[16x32] B[32x16] + C in FP32, lane l
 // holding A[l%16][8(l/16)+i], B[8(l/16)+i][l%16] (i < 8) and
@@ -190,7 +178,8 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__
                                                                 const float *__restrict__ P, float thr,
                                                                 double *__restrict__ wflag,
                                                                 int *__restrict__ list,
-                                                                int *__restrict__ counter, Grid g) {
+                                                                int *__restrict__ counter, Grid g,
+                                                                int *__restrict__ pos, int cap) {
   using Bx = RBox<D>;
   using N = Net<D>;
   constexpr auto &ST = StTab<D>::ST;
@@ -366,7 +355,7 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__
     flagged = cz[tid] > thr || theta[c] < 1.0 || nadFlag[c] > 0.0;
     wflag[c] = flagged ? 1.0 : 0.0;
   }
-  wave_append(flagged, c, list, counter);
+  list_append(live, flagged, c, list, counter, pos, cap);
 }
 
 // Features of every cell, feature-major rows: X[c][NIN] (training data).

@@ -32,9 +32,10 @@ struct Launch {
     return hipGetLastError();
   }
   static hipError_t select(const double *U, const double *WI, const double *theta, double *wflag,
-                           int *list, int *counter, double wThresh, const Grid &g, hipStream_t s) {
+                           int *list, int *counter, double wThresh, const Grid &g, int *pos, int cap,
+                           hipStream_t s) {
     hipLaunchKernelGGL((dev::k_select<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, WI, theta, wflag,
-                       list, counter, wThresh, g);
+                       list, counter, wThresh, g, pos, cap);
     return hipGetLastError();
   }
   static hipError_t weno(int nv, const double *U, double *FS, double *thetaNew, const Grid &g,
@@ -79,7 +80,7 @@ struct Launch {
   static hipError_t mhd_select(const double *U, const double *WI, const double *theta, double *wflag,
                                int *list, int *counter, double wThresh, const Grid &g, hipStream_t s) {
     hipLaunchKernelGGL((dev::k_select<D, NQ, dev::Mhd>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, WI,
-                       theta, wflag, list, counter, wThresh, g);
+                       theta, wflag, list, counter, wThresh, g, nullptr, 0);
     return hipGetLastError();
   }
   static hipError_t mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FT,
@@ -111,13 +112,13 @@ struct Launch {
   // the internal energy comes from the conserved variables)
   static hipError_t learned_select(int nv, const double *U, const double *eint, const double *theta,
                                    const double *nadFlag, const float *P, float thr, double *wflag, int *list,
-                                   int *counter, const Grid &g, hipStream_t s) {
+                                   int *counter, const Grid &g, int *pos, int cap, hipStream_t s) {
     if (nv == 9)
       hipLaunchKernelGGL((dev::k_learned_select<D, 9>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
-                         theta, nadFlag, P, thr, wflag, list, counter, g);
+                         theta, nadFlag, P, thr, wflag, list, counter, g, pos, cap);
     else
       hipLaunchKernelGGL((dev::k_learned_select<D, 5>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
-                         theta, nadFlag, P, thr, wflag, list, counter, g);
+                         theta, nadFlag, P, thr, wflag, list, counter, g, pos, cap);
     return hipGetLastError();
   }
   static hipError_t features(int nv, const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {

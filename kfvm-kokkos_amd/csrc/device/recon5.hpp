@@ -615,7 +615,8 @@ __device__ inline PPBounds pp_bounds_p(const double *__restrict__ U,
 // fold then a group reduction, which equals the sequential fold, and when no
 // bound is violated both thetas stay 1 and the states pass unchanged, as
 // after the reference's early exit.  States of list positions t < cap come
-// from WS[t] (k_wenorec5); every state ends in its cell's slots of FS.
+// from WS[t] (k_wenorec5) and stay there (the flux kernel reads them through
+// the per-cell list positions); the other listed cells are limited in FS.
 template <int FQ>
 struct PostGroup {
   static constexpr int G = FQ <= 24 ? 8 : 16;
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
                                                    const double *__restrict__ props, Grid g,
                                                    const int *__restrict__ list,
                                                    const int *__restrict__ counter,
-                                                   const double *__restrict__ WS, int cap) {
+                                                   double *__restrict__ WS, int cap) {
   constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
   constexpr int G = PostGroup<FQ>::G, PPL = PostGroup<FQ>::PPL, GPW = 64 / G;
   const int n = list ? *counter : g.ncell;
@@ -693,10 +694,11 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
     for (int e = 0; e < PPL; e++) {
       const int pt = pl + G * e;
       if (pt < FQ) {
-        double *const q = FS + (size_t(pt) * 5) * nc + c;
+        double *const q = compact ? WS + size_t(t) * (FQ * 5) + pt * 5 : FS + (size_t(pt) * 5) * nc + c;
+        const size_t qs = compact ? 1 : nc;
 #pragma unroll
         for (int w = 0; w < 5; w++)
-          q[w * nc] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[e][w] - ref[w]) : st[e][w]);
+          q[w * qs] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[e][w] - ref[w]) : st[e][w]);
       }
     }
     if (pl == 0) thetaNew[c] = fmin(thD, thP);

@@ -279,6 +279,29 @@ __global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
   }
 }
 
+// Wave-aggregated append of the flagged cells to the compacted list (one
+// atomic per wave).  With pos, every live cell also records its list position
+// when that is below cap (its WENO states then stay in the compact rows of
+// WS), -1 otherwise.
+__device__ inline void list_append(bool live, bool flagged, int c, int *__restrict__ list,
+                                   int *__restrict__ counter, int *__restrict__ pos, int cap) {
+  const unsigned long long mask = __ballot(flagged);
+  int idx = -1;
+  if (mask != 0ull) {
+    const int lane = threadIdx.x & 63;
+    const int cnt = __popcll(mask);
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, cnt);
+    base = __shfl(base, leader);
+    if (flagged) {
+      idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+      list[idx] = c;
+    }
+  }
+  if (pos && live) pos[c] = idx >= 0 && idx < cap ? idx : -1;
+}
+
 // ------------------------------------------------------------ KXRCF selector
 template <int D, int NQ, class P = Hydro>
 __global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
@@ -286,7 +309,7 @@ __global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
                                                 const double *__restrict__ theta,
                                                 double *__restrict__ wflag, int *__restrict__ list,
                                                 int *__restrict__ counter, double wThresh,
-                                                Grid g) {
+                                                Grid g, int *__restrict__ pos, int cap) {
   constexpr int F = Shape<D, NQ>::F, Q = Shape<D, NQ>::Q;
   const int c = blockIdx.x * BLK + threadIdx.x;
   bool flagged = false;
@@ -327,19 +350,7 @@ __global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
     flagged = ind >= wThresh || theta[c] < 1.0;
     wflag[c] = flagged ? 1.0 : 0.0;
   }
-  // wave-aggregated append to the compacted list
-  const unsigned long long mask = __ballot(flagged);
-  if (mask == 0ull) return;
-  const int lane = threadIdx.x & 63;
-  const int cnt = __popcll(mask);
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(counter, cnt);
-  base = __shfl(base, leader);
-  if (flagged) {
-    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-    list[base + rank] = c;
-  }
+  list_append(c < g.ncell, flagged, c, list, counter, pos, cap);
 }
 
 // ---------------------------------------------------------------- KFVM-WENO
@@ -590,9 +601,12 @@ struct Brick {
 // Face state of cell (ci,cj,ck) on its face f at point q; cells outside the
 // domain resolve through periodic wrap or the face boundary condition (copy of
 // the inner state, with the mirror's sign for reflecting / noslip walls).
+// row >= 0 (Euler): the cell's states are row `row` of the compact WENO
+// staging ws[t][F*Q*5] instead of its FS slots.
 template <int D, int NQ, int NV>
 __device__ inline void face_state(const double *__restrict__ FS, const Grid &g, int ci, int cj,
-                                  int ck, int f, int q, const double *inner, double *out) {
+                                  int ck, int f, int q, const double *inner, double *out, int row = -1,
+                                  const double *__restrict__ ws = nullptr) {
   constexpr int Q = Shape<D, NQ>::Q;
   const int ax = f >> 1;
   const int n_ax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
@@ -616,6 +630,12 @@ __device__ inline void face_state(const double *__restrict__ FS, const Grid &g, 
   }
   const size_t cell = (size_t(ck) * g.ny + cj) * g.nx + ci;
   const size_t nc = g.ncell;
+  if (NV == 5 && row >= 0) {
+    const double *r = ws + size_t(row) * (Shape<D, NQ>::F * Q * 5) + (f * Q + q) * 5;
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = r[v];
+    return;
+  }
 #pragma unroll
   for (int v = 0; v < NV; v++) out[v] = FS[(size_t(f * Q + q) * NV + v) * nc + cell];
 }
@@ -627,6 +647,10 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   using Bk = Brick<D>;
   constexpr int Q = Shape<D, NQ>::Q;
   __shared__ double fl[NV][Bk::NF];
+  // compact WENO rows of the brick's cells and their face neighbours (Euler
+  // sparse WENO; -1: states in FS)
+  constexpr int RX = Bk::X + 2, RY = Bk::Y + 2, RZ = D == 3 ? Bk::Z + 2 : 1;
+  __shared__ int rowTab[NV == 5 ? RX * RY * RZ : 1];
   __shared__ double sh[BLK];
   const int tid = threadIdx.x + Bk::X * (threadIdx.y + Bk::Y * threadIdx.z);
   // one-dimensional grid of bricks, x fastest, renumbered per XCD
@@ -650,6 +674,33 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   double rhs[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) rhs[v] = 0.0;
+
+  const bool useRows = NV == 5 && a.pos != nullptr;
+  if (useRows) {
+    for (int e = tid; e < RX * RY * RZ; e += BLK) {
+      const int x = i0 - 1 + e % RX, y = j0 - 1 + (e / RX) % RY, z = D == 3 ? k0 - 1 + e / (RX * RY) : k0;
+      // a cell beyond a non-periodic edge is never read (its face takes the
+      // boundary state); a periodic one wraps
+      const bool wall = (x < 0 && g.bc[0] != 0) || (x >= g.nx && g.bc[1] != 0) || (y < 0 && g.bc[2] != 0) ||
+                        (y >= g.ny && g.bc[3] != 0) ||
+                        (D == 3 && ((z < 0 && g.bc[4] != 0) || (z >= g.nz && g.bc[5] != 0)));
+      int r = -1;
+      if (!wall) {
+        const int xw = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+        const int yw = y < 0 ? y + g.ny : (y >= g.ny ? y - g.ny : y);
+        const int zw = D == 3 ? (z < 0 ? z + g.nz : (z >= g.nz ? z - g.nz : z)) : z;
+        if (xw >= 0 && xw < g.nx && yw >= 0 && yw < g.ny && zw >= 0 && (D == 2 || zw < g.nz))
+          r = a.pos[(size_t(zw) * g.ny + yw) * g.nx + xw];
+      }
+      if (NV == 5) rowTab[e] = r;
+    }
+    __syncthreads();
+  }
+  auto row_of = [&](int x, int y, int z) {
+    if (!useRows) return -1;
+    const int e = (x - i0 + 1) + RX * ((y - j0 + 1) + RY * (D == 3 ? z - k0 + 1 : 0));
+    return NV == 5 ? rowTab[e] : -1;
+  };
 
 #pragma unroll 1
   for (int q = 0; q < Q; q++) {
@@ -697,11 +748,11 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
         double uL[NV], uR[NV];
         const int lo = 2 * dir + 1, hi = 2 * dir;  // left cell's upper face, right cell's lower face
         if (rIn) {
-          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, nullptr, uR);
-          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, uR, uL);
+          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, nullptr, uR, row_of(ri, rj, rk), a.ws);
+          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, uR, uL, row_of(li, lj, lk), a.ws);
         } else {
-          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, nullptr, uL);
-          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, uL, uR);
+          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, nullptr, uL, row_of(li, lj, lk), a.ws);
+          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, uL, uR, row_of(ri, rj, rk), a.ws);
         }
         double sp;
         if (dir == 0) sp = riemann<NV, 0>(g, ch, uL, uR, Fl);

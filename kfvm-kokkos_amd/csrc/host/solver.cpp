@@ -114,6 +114,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   if (nv_ == 5) {  // compact WENO staging for up to a quarter of the cells
     wsCap_ = int(std::max<size_t>(1, (nc + 3) / 4));
     WS_ = dalloc<double>(size_t(wsCap_) * FQ * nv_, "compact WENO states");
+    pos_ = dalloc<int>(nc, "compact WENO rows");
   }
   WI_ = dalloc<double>(nc * FQ, "indicator");
   props_ = dalloc<double>(nc * 2, "cell properties");
@@ -399,7 +400,7 @@ std::string Solver::writeCkpt() {
 Solver::~Solver() {
   profileEnable(false);
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -477,7 +478,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
     if (lrn) {
       timed("select", [&] {
         return launch_learned_select(dim, 9, Uin, nullptr, thPrev, nadFlag_, netP_, netThr_, wflag_, list_,
-                                     counter_, grid_, stream_);
+                                     counter_, grid_, nullptr, 0, stream_);
       });
     } else {
       const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
@@ -532,12 +533,13 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
       if (learned_ && !forcePlain_) {
         timed("select", [&] {
           return launch_learned_select(dim, 5, Uin, props_, thPrev, nadFlag_, netP_, netThr_, wflag_, list_, counter_,
-                                       grid_, stream_);
+                                       grid_, pos_, wsCap_, stream_);
         });
       } else {
         const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
         timed("select", [&] {
-          return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, stream_);
+          return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, pos_, wsCap_,
+                               stream_);
         });
       }
       timed("weno", [&] {
@@ -563,7 +565,13 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
   if (hydro) th_ ^= 1;
-  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, a, stream_); });
+  // listed Euler cells: their WENO states stay in the compact rows of WS
+  RKArgs ar = a;
+  if (hydro && !lastRhsFull_) {
+    ar.pos = pos_;
+    ar.ws = WS_;
+  }
+  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, ar, stream_); });
   nRhsEval++;
 }
 

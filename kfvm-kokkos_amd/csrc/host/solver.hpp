@@ -121,6 +121,7 @@ class Solver {
   double *Uprev_ = nullptr, *Uhat_ = nullptr, *Utmp_ = nullptr, *K_ = nullptr;
   double *FS_ = nullptr, *WI_ = nullptr, *wflag_ = nullptr;
   double *WS_ = nullptr;  // compact WENO states of the first wsCap_ listed cells (Euler)
+  int *pos_ = nullptr;    // per cell: its row in WS_, or -1
   int wsCap_ = 0;
   double *props_ = nullptr;  // per-cell internal energy and sound speed [2][ncell]
   double *theta_[2] = {nullptr, nullptr};  // positivity theta: previous RHS / this RHS
