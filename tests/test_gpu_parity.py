@@ -56,6 +56,10 @@ CASES = {
                                   user=(), wenothresh=-1.0, fixeddt=0.002),
     "shear2d_periodic": config(dim=2, n=(48, 48, 1), problem="shear2d", seed=1, user=(),
                                bc="periodic", fixeddt=0.002),
+    # WENO cells on every periodic edge (the discontinuity plane wraps), NQ=3
+    # and a grid that is no multiple of the bricks
+    "riemann3d_periodic_nq3_sparse": config(n=(18, 13, 10), nq=3, bc="periodic", wenothresh=1.0,
+                                            fixeddt=0.002),
     "linadv2d_full_nq3": config(dim=2, n=(32, 32, 1), nq=3, equation="linadv", riemann="exact",
                                 problem="linadv_sine", user=(1.0, 1.0, 0.0), bc="periodic",
                                 wenothresh=-1.0, advvel=(1.0, 1.0, 0.0), fixeddt=0.01),
