@@ -24,6 +24,16 @@ struct Grid {
   int rs;            // 0 LLF, 1 HLL, 2 HLLC, 3 exact upwind, 4 KEPES (GLM-MHD)
 };
 
+// Face-state storage FS: cells in blocks of FSB consecutive cells, each block
+// holding all np planes of its cells contiguously, so that the states of a
+// brick of cells sit in a few pages (address translation stays cheap):
+// plane p of cell c is FS[fs_base(c, np) + p * FSB].
+constexpr int FSB = 64;
+__host__ __device__ inline size_t fs_base(size_t c, int np) {
+  return (c / FSB) * size_t(np) * FSB + (c % FSB);
+}
+inline size_t fs_elems(size_t ncell, int np) { return (ncell + FSB - 1) / FSB * FSB * size_t(np); }
+
 enum RKMode : int { RK_PRIME = 0, RK_STAGE = 1, RK_FSAL = 2 };
 
 // Register-update epilogue of the flux kernel.

@@ -412,7 +412,8 @@ __global__ __launch_bounds__(BLK) void k_labels(const double *__restrict__ U, co
   double d = 0.0;
 #pragma unroll 1
   for (int fq = 0; fq < FQ; fq++) {
-    const size_t o0 = (size_t(fq) * NV + 0) * nc + c, o4 = (size_t(fq) * NV + 4) * nc + c;
+    const size_t fb = fs_base(c, FQ * NV);
+    const size_t o0 = fb + (size_t(fq) * NV + 0) * FSB, o4 = fb + (size_t(fq) * NV + 4) * FSB;
     d = fmax(d, fabs(FSW[o0] - FSL[o0]) / sc[0]);
     d = fmax(d, fabs(FSW[o4] - FSL[o4]) / sc[1]);
   }

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U
       double acc = 0.0;
 #pragma unroll
       for (int n = 0; n < C; n++) acc = fma(W[fq * C + n], vs[n], acc);
-      FS[(size_t(fq) * 9 + v) * nc + c] = acc;
+      FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB] = acc;
     }
 #pragma unroll 2
     for (int p = 0; p < NI; p++) {
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U
     for (int fq = 0; fq < FQ; fq++) {
       double s[9];
 #pragma unroll
-      for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
+      for (int v = 0; v < 9; v++) s[v] = FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB];
       WI[size_t(fq) * nc + c] = Mhd::weno(g.gamma, s);
     }
   }
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ 
       for (int h = 0; h < HF; h++) out[h] = 0.0;
       WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, (const cdouble *)fc0, om, out);
 #pragma unroll
-      for (int h = 0; h < HF; h++) FS[(size_t(h0 + h) * 9 + v) * nc + c] = out[h];
+      for (int h = 0; h < HF; h++) FS[fs_base(c, FQ * 9) + (size_t(h0 + h) * 9 + v) * FSB] = out[h];
     }
 #pragma unroll 1
     for (int p0 = 0; p0 < NI; p0 += HI) {
@@ -225,10 +225,11 @@ __global__ __launch_bounds__(BLK) void k_mhd_weno_post(const double *__restrict_
     const MhdLin lp(ref, g.gamma);
 #pragma unroll 2
     for (int p = 0; p < FQ + NI; p++) {
-      double *P = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
+      double *P = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
+      const size_t ps = p < FQ ? size_t(FSB) : nc;
       double a[9];
 #pragma unroll
-      for (int w = 0; w < 9; w++) a[w] = P[w * nc];
+      for (int w = 0; w < 9; w++) a[w] = P[w * ps];
       const double d = a[0];
       double e = lp.vsq * d;
       const double mx = lp.rho * a[1] + lp.u * d;
@@ -242,10 +243,10 @@ __global__ __launch_bounds__(BLK) void k_mhd_weno_post(const double *__restrict_
       e = e + lp.bz * a[7];
       e = e + a[4] / lp.gamm;
       e = e + lp.rPsi * a[8];
-      P[nc] = mx;
-      P[2 * nc] = my;
-      P[3 * nc] = mz;
-      P[4 * nc] = e;
+      P[ps] = mx;
+      P[2 * ps] = my;
+      P[3 * ps] = mz;
+      P[4 * ps] = e;
     }
   }
 }
@@ -271,9 +272,10 @@ __device__ inline void mhd_scale(double *__restrict__ FS, double *__restrict__ S
   constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
 #pragma unroll 1
   for (int p = 0; p < FQ + NI; p++) {
-    double *b = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
+    double *b = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
+    const size_t bs = p < FQ ? size_t(FSB) : nc;
 #pragma unroll
-    for (int v = 0; v < 9; v++) b[v * nc] = replace ? ref[v] : ref[v] + th * (b[v * nc] - ref[v]);
+    for (int v = 0; v < 9; v++) b[v * bs] = replace ? ref[v] : ref[v] + th * (b[v * bs] - ref[v]);
   }
 }
 
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
     double thD = 1.0;
 #pragma unroll 1
     for (int p = 0; p < FQ + NI; p++) {
-      const double rho = p < FQ ? FS[size_t(p) * 9 * nc + c] : SRC[size_t(p - FQ) * 9 * nc + c];
+      const double rho = p < FQ ? FS[fs_base(c, FQ * 9) + size_t(p) * 9 * FSB] : SRC[size_t(p - FQ) * 9 * nc + c];
       thD = dens_theta(thD, rho, b);
     }
     if (thD > 0.0 && thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, false);
@@ -376,10 +378,11 @@ __global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
     double thP = 1.0;
 #pragma unroll 1
     for (int p = 0; p < FQ + NI; p++) {
-      const double *q = p < FQ ? FS + size_t(p) * 9 * nc + c : SRC + size_t(p - FQ) * 9 * nc + c;
+      const double *q = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
+      const size_t qs = p < FQ ? size_t(FSB) : nc;
       double s[9];
 #pragma unroll
-      for (int v = 0; v < 9; v++) s[v] = q[v * nc];
+      for (int v = 0; v < 9; v++) s[v] = q[v * qs];
       const double ei = Mhd::eint(s);
       thP = ei > minEint ? thP : fmin(thP, mhd_eint_pos(minEint, ref, s));
       if (p < FQ) mhd_face_speed(g.gamma, s, maxLam, maxVel);
@@ -392,7 +395,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
       for (int fq = 0; fq < FQ; fq++) {
         double s[9];
 #pragma unroll
-        for (int v = 0; v < 9; v++) s[v] = FS[(size_t(fq) * 9 + v) * nc + c];
+        for (int v = 0; v < 9; v++) s[v] = FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB];
         mhd_face_speed(g.gamma, s, maxLam, maxVel);
       }
     }
@@ -420,13 +423,13 @@ __global__ __launch_bounds__(BLK) void k_mhd_sources(const double *__restrict__ 
                                                      double *__restrict__ SRC,
                                                      const double *__restrict__ theta, Grid g,
                                                      const double *__restrict__ Dm) {
-  constexpr int Q = MShape<D, NQ>::Q, NI = MShape<D, NQ>::NI, NZ = MShape<D, NQ>::NZ;
+  constexpr int Q = MShape<D, NQ>::Q, NI = MShape<D, NQ>::NI, NZ = MShape<D, NQ>::NZ, FQ = MShape<D, NQ>::FQ;
   const int c = blockIdx.x * BLK + threadIdx.x;
   if (c >= g.ncell) return;
   int i, j, k;
   cell_ijk<D>(c, g, i, j, k);
   const size_t nc = g.ncell;
-  auto own = [&](int f, int q, int v) { return FS[(size_t(f * Q + q) * 9 + v) * nc + c]; };
+  auto own = [&](int f, int q, int v) { return FS[fs_base(c, FQ * 9) + (size_t(f * Q + q) * 9 + v) * FSB]; };
   // normal field averaged over the two states of each face point
   auto bav = [&](int f, int q) {
     const int ax = f >> 1, hi = f & 1;

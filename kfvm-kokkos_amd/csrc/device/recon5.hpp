@@ -187,7 +187,7 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
       __builtin_amdgcn_sched_group_barrier(0x002, P, 0);
     }
 #pragma unroll
-    for (int q = 0; q < P; q++) FS[(size_t(ax * P + q) * 5 + v) * nc + cell] = acc[q];
+    for (int q = 0; q < P; q++) FS[fs_base(cell, Shape<D, NQ>::F * Shape<D, NQ>::Q * 5) + (size_t(ax * P + q) * 5 + v) * FSB] = acc[q];
     // keep the scheduler from hoisting the next variable's stencil loads
     __builtin_amdgcn_sched_barrier(0);
   };
@@ -371,8 +371,8 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
     const int c = list ? list[t] : t;
     const bool compact = list && t < cap;
     // state q of this cell at P[q * ps]
-    double *const P = compact ? WS + size_t(t) * (FQ * 5) : FS + c;
-    const size_t ps = compact ? 1 : nc;
+    double *const P = compact ? WS + size_t(t) * (FQ * 5) : FS + fs_base(c, FQ * 5);
+    const size_t ps = compact ? 1 : FSB;
     int i, j, k;
     cell_ijk<D>(c, g, i, j, k);
     double ref[5];
@@ -531,9 +531,9 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
     // pp_apply with one lane per face point
     const bool own = pl < FQ;
     double st[5];
-    double *p = FS + (size_t(own ? pl : 0) * 5) * nc + c;
+    double *p = FS + fs_base(c, FQ * 5) + (size_t(own ? pl : 0) * 5) * FSB;
 #pragma unroll
-    for (int w = 0; w < 5; w++) st[w] = own ? p[w * nc] : ref[w];
+    for (int w = 0; w < 5; w++) st[w] = own ? p[w * FSB] : ref[w];
     const double thD = grp_min<LPG>(own ? dens_theta(1.0, st[0], pb) : 1.0);
     const bool mixD = thD > 0.0 && thD < 1.0, repD = !mixD && thD < 1.0;
 #pragma unroll
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(BLK) void k_pp_lanes(const double *__restrict__ U,
     for (int w = 0; w < 5; w++) st[w] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[w] - ref[w]) : st[w]);
     if (own && (thD < 1.0 || thP < 1.0)) {
 #pragma unroll
-      for (int w = 0; w < 5; w++) p[w * nc] = st[w];
+      for (int w = 0; w < 5; w++) p[w * FSB] = st[w];
     }
     if (pl == 0) thetaNew[c] = fmin(thD, thP);
   }
@@ -661,9 +661,9 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
 #pragma unroll
           for (int w = 0; w < 5; w++) a[w] = q[w];
         } else {
-          const double *q = FS + (size_t(pt) * 5) * nc + c;
+          const double *q = FS + fs_base(c, FQ * 5) + (size_t(pt) * 5) * FSB;
 #pragma unroll
-          for (int w = 0; w < 5; w++) a[w] = q[w * nc];
+          for (int w = 0; w < 5; w++) a[w] = q[w * FSB];
         }
         const double d = a[0], mx = a[1], my = a[2], mz = a[3], en = a[4];
         st[e][4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + en / lp.gamm);
@@ -694,8 +694,8 @@ __global__ __launch_bounds__(BLK) void k_weno_post(const double *__restrict__ U,
     for (int e = 0; e < PPL; e++) {
       const int pt = pl + G * e;
       if (pt < FQ) {
-        double *const q = compact ? WS + size_t(t) * (FQ * 5) + pt * 5 : FS + (size_t(pt) * 5) * nc + c;
-        const size_t qs = compact ? 1 : nc;
+        double *const q = compact ? WS + size_t(t) * (FQ * 5) + pt * 5 : FS + fs_base(c, FQ * 5) + (size_t(pt) * 5) * FSB;
+        const size_t qs = compact ? 1 : FSB;
 #pragma unroll
         for (int w = 0; w < 5; w++)
           q[w * qs] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[e][w] - ref[w]) : st[e][w]);
@@ -727,8 +727,8 @@ __global__ __launch_bounds__(BLK) void k_weno_post_cells(const double *__restric
     const PPBounds pb = pp_bounds_p<D>(U, props, g, i, j, k, ref);
 #pragma unroll 2
     for (int fq = 0; fq < FQ; fq++) {
-      double *p = FS + (size_t(fq) * 5) * nc + c;
-      const double d = p[0], mx = p[nc], my = p[2 * nc], mz = p[3 * nc], e = p[4 * nc];
+      double *p = FS + fs_base(c, FQ * 5) + (size_t(fq) * 5) * FSB;
+      const double d = p[0], mx = p[FSB], my = p[2 * FSB], mz = p[3 * FSB], e = p[4 * FSB];
       double st[5];
       st[4] = (lp.vsq * d + ru * mx + rvv * my + rw * mz + e / lp.gamm);
       st[1] = (lp.rho * mx + lp.u * d);
@@ -736,13 +736,13 @@ __global__ __launch_bounds__(BLK) void k_weno_post_cells(const double *__restric
       st[3] = (lp.rho * mz + lp.w * d);
       st[0] = d;
 #pragma unroll
-      for (int w = 1; w < 5; w++) p[w * nc] = st[w];
+      for (int w = 1; w < 5; w++) p[w * FSB] = st[w];
       thD = dens_theta(thD, d, pb);
       okE = okE && Hydro::eint(st) > pb.minEint;
     }
     // no bound violated: the states stay as they are (theta 1)
     double th = 1.0;
-    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, c, ref, pb);
+    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS + fs_base(c, FQ * 5), FSB, 0, ref, pb);
     thetaNew[c] = th;
   }
 }

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
       double acc = 0.0;
 #pragma unroll
       for (int n = 0; n < C; n++) acc = fma(W[fq * C + n], vs[n], acc);
-      FS[(size_t(fq) * NV + v) * nc + c] = acc;
+      FS[fs_base(c, FQ * NV) + (size_t(fq) * NV + v) * FSB] = acc;
       if (NV == 5) {
         if (v == 0) {
           rhoF[fq] = acc;
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
   }
   if (NV == 5) {
     double th = 1.0;
-    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS, nc, c, ref, pb);
+    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS + fs_base(c, FQ * NV), FSB, 0, ref, pb);
     thetaNew[c] = th;
   }
 }
@@ -469,10 +469,10 @@ __device__ inline void weno_cell(const double *__restrict__ U, double *__restric
           }
           acc = fma(om[s], t, acc);
         }
-        double *out = FS + (size_t(fq) * NV + v) * nc + c;
+        double *out = FS + fs_base(c, FQ * NV) + (size_t(fq) * NV + v) * FSB;
         if (NV == 5) {
-          double *dens = FS + (size_t(fq) * NV + 0) * nc + c;
-          double *etot = FS + (size_t(fq) * NV + 4) * nc + c;
+          double *dens = FS + fs_base(c, FQ * NV) + (size_t(fq) * NV + 0) * FSB;
+          double *etot = FS + fs_base(c, FQ * NV) + (size_t(fq) * NV + 4) * FSB;
           if (v == 0) {
             *out = acc;
             *etot = vsq * acc;
@@ -509,7 +509,8 @@ __global__ __launch_bounds__(BLK) void k_weno(const double *__restrict__ U,
 #pragma unroll
       for (int v = 0; v < 5; v++) ref[v] = U[size_t(v) * g.ncell + c];
       const PPBounds pb = pp_bounds<D>(U, g, i, j, k, ref);
-      thetaNew[c] = pp_apply<D, NQ>(FS, size_t(g.ncell), c, ref, pb);
+      constexpr int NPL = Shape<D, NQ>::F * Shape<D, NQ>::Q * 5;
+      thetaNew[c] = pp_apply<D, NQ>(FS + fs_base(c, NPL), FSB, 0, ref, pb);
     }
   }
 }
@@ -637,7 +638,8 @@ __device__ inline void face_state(const double *__restrict__ FS, const Grid &g, 
     return;
   }
 #pragma unroll
-  for (int v = 0; v < NV; v++) out[v] = FS[(size_t(f * Q + q) * NV + v) * nc + cell];
+  for (int v = 0; v < NV; v++)
+    out[v] = FS[fs_base(cell, Shape<D, NQ>::F * Q * NV) + (size_t(f * Q + q) * NV + v) * FSB];
 }
 
 template <int D, int NQ, int NV>

@@ -110,7 +110,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
   Uhat_ = dalloc<double>(nc * nv_, "Uhat");
   K_ = dalloc<double>(nc * nv_, "K");
   if (rk_.needsTmp()) Utmp_ = dalloc<double>(nc * nv_, "Utmp");
-  FS_ = dalloc<double>(nc * FQ * nv_, "face states");
+  FS_ = dalloc<double>(fs_elems(nc, int(FQ) * nv_), "face states");
   if (nv_ == 5) {  // compact WENO staging for up to a quarter of the cells
     wsCap_ = int(std::max<size_t>(1, (nc + 3) / 4));
     WS_ = dalloc<double>(size_t(wsCap_) * FQ * nv_, "compact WENO states");
@@ -759,7 +759,7 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
   const int dim = cfg_.dim, nq = cfg_.nQuad;
   const double *Uin = stateInCur_ ? U_[cur_] : Uprev_;
   double *thScratch = theta_[th_ ^ 1];
-  const size_t fsBytes = size_t(ncell_) * wts_.F * wts_.Q * nv_ * sizeof(double);
+  const size_t fsBytes = fs_elems(size_t(ncell_), wts_.F * wts_.Q * nv_) * sizeof(double);
   if (!FSlin_) FSlin_ = dalloc<double>(fsBytes / sizeof(double), "linear face states");
   if (nv_ == 9) {
     // labels compare the linear face states with the limited WENO ones
