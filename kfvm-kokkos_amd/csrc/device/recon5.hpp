@@ -742,7 +742,7 @@ __global__ __launch_bounds__(BLK) void k_weno_post_cells(const double *__restric
     }
     // no bound violated: the states stay as they are (theta 1)
     double th = 1.0;
-    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS + fs_base(c, FQ * 5), FSB, 0, ref, pb);
+    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS + fs_base(c, FQ * 5), FSB, ref, pb);
     thetaNew[c] = th;
   }
 }

@@ -180,41 +180,41 @@ __device__ inline double eint_pos(double emin, const double *ref, const double *
   return a;
 }
 
+// The states of one cell are reached as P[(fq * 5 + v) * ps]: its face-state
+// block (P = FS + fs_base(c, np), ps = FSB) or its compact WENO row (ps = 1).
 template <int D, int NQ>
-__device__ inline void pp_scale(double *__restrict__ FS, size_t nc, int c, const double *ref,
-                                double th, bool replace) {
+__device__ inline void pp_scale(double *__restrict__ P, size_t ps, const double *ref, double th, bool replace) {
   constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
 #pragma unroll 1
   for (int fq = 0; fq < FQ; fq++)
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-      double *p = FS + (size_t(fq) * 5 + v) * nc + c;
+      double *p = P + (size_t(fq) * 5 + v) * ps;
       *p = replace ? ref[v] : ref[v] + th * (*p - ref[v]);
     }
 }
 
-// The limiter on the face states of cell c held in FS, in the reference's
-// order: density theta, mix, internal-energy theta (10-step bisection), mix.
+// The limiter on the face states of one cell, in the reference's order:
+// density theta, mix, internal-energy theta (10-step bisection), mix.
 template <int D, int NQ>
-__device__ inline double pp_apply(double *__restrict__ FS, size_t nc, int c, const double *ref,
-                                  const PPBounds &b) {
+__device__ inline double pp_apply(double *__restrict__ P, size_t ps, const double *ref, const PPBounds &b) {
   constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
   double thD = 1.0;
 #pragma unroll 1
-  for (int fq = 0; fq < FQ; fq++) thD = dens_theta(thD, FS[(size_t(fq) * 5) * nc + c], b);
-  if (thD > 0.0 && thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, false);
-  else if (thD < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thD, true);
+  for (int fq = 0; fq < FQ; fq++) thD = dens_theta(thD, P[(size_t(fq) * 5) * ps], b);
+  if (thD > 0.0 && thD < 1.0) pp_scale<D, NQ>(P, ps, ref, thD, false);
+  else if (thD < 1.0) pp_scale<D, NQ>(P, ps, ref, thD, true);
   double thP = 1.0;
 #pragma unroll 1
   for (int fq = 0; fq < FQ; fq++) {
     double s[5];
 #pragma unroll
-    for (int v = 0; v < 5; v++) s[v] = FS[(size_t(fq) * 5 + v) * nc + c];
+    for (int v = 0; v < 5; v++) s[v] = P[(size_t(fq) * 5 + v) * ps];
     const double ei = Hydro::eint(s);
     thP = ei > b.minEint ? thP : fmin(thP, eint_pos(b.minEint, ref, s));
   }
-  if (thP > 0.0 && thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, false);
-  else if (thP < 1.0) pp_scale<D, NQ>(FS, nc, c, ref, thP, true);
+  if (thP > 0.0 && thP < 1.0) pp_scale<D, NQ>(P, ps, ref, thP, false);
+  else if (thP < 1.0) pp_scale<D, NQ>(P, ps, ref, thP, true);
   return fmin(thD, thP);
 }
 
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
   }
   if (NV == 5) {
     double th = 1.0;
-    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS + fs_base(c, FQ * NV), FSB, 0, ref, pb);
+    if (!(thD == 1.0 && okE)) th = pp_apply<D, NQ>(FS + fs_base(c, FQ * NV), FSB, ref, pb);
     thetaNew[c] = th;
   }
 }
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(BLK) void k_weno(const double *__restrict__ U,
       for (int v = 0; v < 5; v++) ref[v] = U[size_t(v) * g.ncell + c];
       const PPBounds pb = pp_bounds<D>(U, g, i, j, k, ref);
       constexpr int NPL = Shape<D, NQ>::F * Shape<D, NQ>::Q * 5;
-      thetaNew[c] = pp_apply<D, NQ>(FS + fs_base(c, NPL), FSB, 0, ref, pb);
+      thetaNew[c] = pp_apply<D, NQ>(FS + fs_base(c, NPL), FSB, ref, pb);
     }
   }
 }
