@@ -83,7 +83,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U
       double s[9];
 #pragma unroll
       for (int v = 0; v < 9; v++) s[v] = FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB];
-      WI[size_t(fq) * nc + c] = Mhd::weno(g.gamma, s);
+      WI[fs_base(c, FQ) + size_t(fq) * FSB] = Mhd::weno(g.gamma, s);
     }
   }
 }

@@ -213,7 +213,8 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
       const double ei = t[q] - 0.5 * msq[q] / rho[q];
       okE = okE && ei > pb.minEint;
       if (WI) {  // the KXRCF indicator's input; not needed by the learned selection
-        WI[size_t(ax * P + q) * nc + cell] = (gm1 * ei) / pow(rho[q], g.gamma);
+        WI[fs_base(cell, Shape<D, NQ>::F * Shape<D, NQ>::Q) + size_t(ax * P + q) * FSB] =
+            (gm1 * ei) / pow(rho[q], g.gamma);
         __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
       }
     }

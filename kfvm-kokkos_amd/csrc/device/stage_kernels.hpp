@@ -5,9 +5,11 @@
 //                                    conditions are applied as index maps
 //                                    (mirror / periodic / signed mirror) when
 //                                    stencils are gathered
-//   FS[(f*Q + q)*NV + v][cell]       reconstructed state of each cell on its own
-//                                    face f (W,E,S,N,B,T) at face point q
-//   WI[f*Q + q][cell]                KXRCF indicator p/rho^gamma at those points
+//   FS, value (f*Q + q)*NV + v       reconstructed state of each cell on its own
+//                                    face f (W,E,S,N,B,T) at face point q, in
+//                                    blocks of FSB cells (fs_base)
+//   WI, value f*Q + q                KXRCF indicator p/rho^gamma at those points,
+//                                    same blocked layout
 //   theta[cell], wflag[cell]         positivity theta and WENO flag
 //
 // One stage = linear recon (+indicator)  -> KXRCF select (compacted list)
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(BLK) void k_linear(const double *__restrict__ U,
         } else {
           const double ei = acc - 0.5 * msq[fq] / rhoF[fq];
           okE = okE && ei > pb.minEint;
-          WI[size_t(fq) * nc + c] = (g.gamma - 1.0) * ei / pow(rhoF[fq], g.gamma);
+          WI[fs_base(c, FQ) + size_t(fq) * FSB] = (g.gamma - 1.0) * ei / pow(rhoF[fq], g.gamma);
         }
       }
     }
@@ -338,8 +340,8 @@ __global__ __launch_bounds__(BLK) void k_select(const double *__restrict__ U,
       const int fo = f ^ 1;  // opposite face of the neighbour
 #pragma unroll
       for (int q = 0; q < Q; q++) {
-        const double win = WI[size_t(f * Q + q) * nc + c];
-        const double wout = inside ? WI[size_t(fo * Q + q) * nc + cn] : win;
+        const double win = WI[fs_base(c, F * Q) + size_t(f * Q + q) * FSB];
+        const double wout = inside ? WI[fs_base(cn, F * Q) + size_t(fo * Q + q) * FSB] : win;
         maxJump = fmax(maxJump, fabs(wout - win));
       }
     }

@@ -116,7 +116,7 @@ Solver::Solver(const Config &cfg) : cfg_(cfg) {
     WS_ = dalloc<double>(size_t(wsCap_) * FQ * nv_, "compact WENO states");
     pos_ = dalloc<int>(nc, "compact WENO rows");
   }
-  WI_ = dalloc<double>(nc * FQ, "indicator");
+  WI_ = dalloc<double>(fs_elems(nc, int(FQ)), "indicator");
   props_ = dalloc<double>(nc * 2, "cell properties");
   if (cfg_.eq == Eq::MhdGlm) {
     const QuadRule qr2 = quad_rule(cfg_.nQuad);
