@@ -382,13 +382,17 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
     const LinPrim lp(ref, g.gamma);
     const double uv = G.v == 1 ? lp.u : (G.v == 2 ? lp.v : lp.w);
     double vs[C];
+    // each lane loads its own variable of the stencil cells; the density and
+    // momenta the other variables' conversions need come from the cell's
+    // other lanes (lanes 5 slot + w hold variable w)
+    const int gbase = LPC * G.slot;
 #pragma unroll
     for (int m = 0; m < C; m++) {
       const CellRef r = map_cell<D, 5>(g, i + ST.off[m][0], j + ST.off[m][1], k + ST.off[m][2]);
-      const double s0 = ld(U, nc, r, 0);
       const double sv = ld(U, nc, r, G.v);
+      const double s0 = __shfl(sv, gbase);
+      const double s1 = __shfl(sv, gbase + 1), s2 = __shfl(sv, gbase + 2), s3 = __shfl(sv, gbase + 3);
       if (G.v == 4) {
-        const double s1 = ld(U, nc, r, 1), s2 = ld(U, nc, r, 2), s3 = ld(U, nc, r, 3);
         vs[m] = lp.gamm * (lp.vsq * s0 - lp.u * s1 - lp.v * s2 - lp.w * s3 + sv);
       } else {
         vs[m] = G.v == 0 ? s0 : (sv - uv * s0) / lp.rho;
