@@ -141,21 +141,25 @@ __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ 
     for (int w = 0; w < 9; w++) ref[w] = U[size_t(w) * nc + c];
     const MhdLin lp(ref, g.gamma);
     const double uv = v == 1 ? lp.u : (v == 2 ? lp.v : lp.w);
-    // stencil values of variable v, linearised primitive (cons2LinPrim)
+    // stencil values of variable v, linearised primitive (cons2LinPrim): each
+    // lane loads its own variable, the others come from the cell's lanes
+    // (lane 9 slot + w holds variable w)
+    const int gbase = MLPC * slot;
     double vs[C];
 #pragma unroll
     for (int m = 0; m < C; m++) {
       const CellRef r = map_cell<D, 9>(g, i + ST.off[m][0], j + ST.off[m][1], k + ST.off[m][2]);
-      if (v == 4) {
-        double q[9];
+      const double sv = ld(U, nc, r, v);
+      double q[9];
 #pragma unroll
-        for (int w = 0; w < 9; w++) q[w] = ld(U, nc, r, w);
+      for (int w = 0; w < 9; w++) q[w] = __shfl(sv, gbase + w);
+      if (v == 4) {
         vs[m] = lp.gamm * (lp.vsq * q[0] + q[4] - lp.u * q[1] - lp.v * q[2] - lp.w * q[3] - lp.bx * q[5] -
                            lp.by * q[6] - lp.bz * q[7] - lp.rPsi * q[8]);
       } else if (v >= 1 && v <= 3) {
-        vs[m] = (ld(U, nc, r, v) - uv * ld(U, nc, r, 0)) / lp.rho;
+        vs[m] = (sv - uv * q[0]) / lp.rho;
       } else {
-        vs[m] = ld(U, nc, r, v);
+        vs[m] = sv;
       }
     }
     // weight tables opaque to loop-invariant code motion (see k_linear_fused)
