@@ -678,6 +678,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   double rhs[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) rhs[v] = 0.0;
+  const double rdx = 1.0 / g.dx, rdy = 1.0 / g.dy, rdz = 1.0 / g.dz;
 
   const bool useRows = NV == 5 && a.pos != nullptr;
   if (useRows) {
@@ -773,20 +774,22 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
       const int xW = tx + (Bk::X + 1) * (ty + Bk::Y * tz), xE = xW + 1;
       const int yS = Bk::NXF + tx + Bk::X * (ty + (Bk::Y + 1) * tz), yN = yS + Bk::X;
       const int zB = Bk::NXF + Bk::NYF + tx + Bk::X * (ty + Bk::Y * tz), zT = zB + Bk::X * Bk::Y;
+      // the reference divides each term by the spacing; multiplying by its
+      // reciprocal differs by rounding only and saves 12 NV divisions
       if (D == 2) {
         const double w = g.wt[q];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          rhs[v] += w * (fl[v][xW] - fl[v][xE]) / g.dx;
-          rhs[v] += w * (fl[v][yS] - fl[v][yN]) / g.dy;
+          rhs[v] += w * (fl[v][xW] - fl[v][xE]) * rdx;
+          rhs[v] += w * (fl[v][yS] - fl[v][yN]) * rdy;
         }
       } else {
         const double w = g.wt[q / NQ] * g.wt[q % NQ];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          rhs[v] += w * (fl[v][xW] - fl[v][xE]) / g.dx;
-          rhs[v] += w * (fl[v][yS] - fl[v][yN]) / g.dy;
-          rhs[v] += w * (fl[v][zB] - fl[v][zT]) / g.dz;
+          rhs[v] += w * (fl[v][xW] - fl[v][xE]) * rdx;
+          rhs[v] += w * (fl[v][yS] - fl[v][yN]) * rdy;
+          rhs[v] += w * (fl[v][zB] - fl[v][zT]) * rdz;
         }
       }
     }
