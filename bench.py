@@ -210,7 +210,9 @@ def main():
     weights = None
     if not a.plain_only:
         if a.retrain:
-            weights, _ = train_indicator.train("euler3d", out_dir=os.path.join(ROOT, "gpurun_out", "weights"),
+            import tempfile
+            # the retrained weights go to a scratch directory, not over the committed ones
+            weights, _ = train_indicator.train("euler3d", out_dir=tempfile.mkdtemp(prefix="kfvm_weights_"),
                                                log=lambda m: print(m, flush=True))
         else:
             weights = train_indicator.load("euler3d")
