@@ -844,7 +844,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
         st[v] = u;
         hasNan = hasNan || isnan(u);
         const double e = (u - uh) / (a.atol + a.rtol * fmax(fabs(u), fabs(uh)));
-        errLoc += pow(e, 2.0);
+        errLoc += e * e;  // the reference's pow(e, 2), as a product (rounding-level difference)
       }
       posLoc = hasNan ? -1.0
                       : (NV == 5 ? Hydro::positivity(st) : (NV == 9 ? fmin(st[0], Mhd::eint(st)) : 1.0));
