@@ -810,7 +810,7 @@ struct Solver : Base {
       }
     }
     // smoothness indicators (findSmoothness)
-    double om[8][NV];
+    double om[8][NV] = {};
     for (int s = 0; s < nSub; s++) {
       const int n = s == 0 ? C : (s == 1 ? Cc : Cb);
       for (int v = 0; v < NV; v++) om[s][v] = 0.0;
