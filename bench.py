@@ -128,12 +128,15 @@ def _mode_run(n, weights, a, profile):
     s = kfvm_amd.Solver(init_text=init_text(n, a.wenothresh, nq=a.nq))
     if weights is not None:
         s.set_indicator(weights, 0.0)
+    nwarm = 0
     if a.warmup >= 0:
         for _ in range(a.warmup):
             s.take_step()
+            nwarm += 1
     else:
         while s.time[0] < 0.1 and not s.time[0] >= 0.2:
             s.take_step()
+            nwarm += 1
     s.sync()
     c0 = s.counters
     t0 = time.perf_counter()
@@ -159,13 +162,42 @@ def _mode_run(n, weights, a, profile):
         if s.take_step() not in (0, 1, 2):
             break
     l1 = planar_riemann_l1(s.state[0], s.time[0])
-    out = {"value": value, "ms_per_step": el / a.steps * 1e3, "stages": stages, "seconds": el,
+    out = {"value": value, "ms_per_step": el / a.steps * 1e3, "warmup_steps": nwarm, "stages": stages,
+           "seconds": el,
            "weno_fraction": nweno / a.steps / n ** 3, "l1_rho_t0.2": l1, "t_final": s.time[0],
            "steps_total": s.time[2] - 1}
     if weights is not None:
         out.update(s.indicator_counters)
     s.close()
     return out, kern
+
+
+def _roofline(kern, head, ncell, n, nq):
+    """Roofline of the kernel with the largest profiled time, plus the whole
+    path against its FP64 ceiling at the measured WENO fraction."""
+    dom = max(kern, key=lambda k: kern[k]["ms_total"])
+    d = kern[dom]
+    if dom == "weno":
+        cells = head["weno_fraction"] * ncell
+        roof = {"bound": "mfma", "achieved": flop_weno(nq) * cells / (d["ms_avg"] * 1e-3) / 1e12,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
+    elif dom == "linear":
+        roof = {"bound": "mfma", "achieved": flop_linear(nq) * ncell / (d["ms_avg"] * 1e-3) / 1e12,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        # flux/RK kernel: HBM roof on its algorithmic register traffic
+        roof = {"bound": "hbm", "achieved": BYTES_STAGE * ncell / (d["ms_avg"] * 1e-3) / 1e9,
+                "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = TRAFFIC.get(dom) if n == 256 and nq == 2 else None
+    roof["kernel"] = dom
+    # the whole path against its FP64 ceiling at the measured WENO fraction
+    f_lin, f_weno = PATH_F[nq]
+    f_alg = f_lin + head["weno_fraction"] * (f_weno - f_lin)
+    ceil = PEAK_FP64_TFLOPS * 1e12 / f_alg / 1e6
+    roof["path"] = {"bound": "fp64", "flop_per_zone_update": f_alg, "ceiling_mzone_updates_per_s": ceil,
+                    "frac": head["value"] / ceil}
+    return roof
 
 
 def main():
@@ -229,29 +261,7 @@ def main():
     head = learned if learned is not None else plain
     kern = kl if learned is not None else kp
     ncell = n ** 3
-    dom = max(kern, key=lambda k: kern[k]["ms_total"])
-    d = kern[dom]
-    if dom == "weno":
-        cells = head["weno_fraction"] * ncell
-        roof = {"bound": "mfma", "achieved": flop_weno(a.nq) * cells / (d["ms_avg"] * 1e-3) / 1e12,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
-    elif dom == "linear":
-        roof = {"bound": "mfma", "achieved": flop_linear(a.nq) * ncell / (d["ms_avg"] * 1e-3) / 1e12,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
-    else:
-        # flux/RK kernel: HBM roof on its algorithmic register traffic
-        roof = {"bound": "hbm", "achieved": BYTES_STAGE * ncell / (d["ms_avg"] * 1e-3) / 1e9,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = TRAFFIC.get(dom) if n == 256 and a.nq == 2 else None
-    roof["kernel"] = dom
-    # the whole path against its FP64 ceiling at the measured WENO fraction
-    f_lin, f_weno = PATH_F[a.nq]
-    f_alg = f_lin + head["weno_fraction"] * (f_weno - f_lin)
-    ceil = PEAK_FP64_TFLOPS * 1e12 / f_alg / 1e6
-    roof["path"] = {"bound": "fp64", "flop_per_zone_update": f_alg, "ceiling_mzone_updates_per_s": ceil,
-                    "frac": head["value"] / ceil}
-
+    roof = _roofline(kern, head, ncell, n, a.nq) if kern else None
     cpu = None if a.no_cpu else cpu_baseline()
     method = "learned indicator" if learned is not None else "plain (KXRCF)"
     out = {
@@ -260,7 +270,7 @@ def main():
         "unit": "Mzone-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
-        "warmup": a.warmup,
+        "warmup": head["warmup_steps"],
         "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",

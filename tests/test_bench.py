@@ -26,3 +26,20 @@ def test_work_figures_match_the_stencil_counts():
     assert bench.BYTES_STAGE == 200
     lin, weno = bench.PATH_F[2]
     assert lin < weno
+
+
+def test_roofline_of_the_dominant_kernel():
+    import bench
+    head = {"weno_fraction": 0.05, "value": 900.0}
+    kern = {"linear": {"ms_total": 10.0, "ms_avg": 1.0},
+            "flux_rk": {"ms_total": 20.0, "ms_avg": 2.0}}
+    r = bench._roofline(kern, head, 256 ** 3, 256, 2)
+    assert r["kernel"] == "flux_rk" and r["bound"] == "hbm"
+    # 200 B per cell over 2 ms
+    assert abs(r["achieved"] - 200 * 256 ** 3 / 2e-3 / 1e9) < 1e-9
+    assert abs(r["frac"] - r["achieved"] / bench.PEAK_HBM_GBS) < 1e-15
+    assert r["traffic"] == bench.TRAFFIC["flux_rk"]
+    lin, weno = bench.PATH_F[2]
+    assert abs(r["path"]["flop_per_zone_update"] - (lin + 0.05 * (weno - lin))) < 1e-9
+    # another size: no counter figure for the traffic
+    assert bench._roofline(kern, head, 128 ** 3, 128, 2)["traffic"] is None
