@@ -129,9 +129,9 @@ def train(name, out_dir=WEIGHTS, log=print):
 
 
 EVALUATIONS = {
-    # configuration 3: 2D shear layer at 1024^2, seed 0, periodic, to t = 1
+    # 2D shear layer at 1024^2, seed 0, periodic, to t = 1
     "shear2d": dict(dim=2, n=(1024, 1024), problem="shear2d", user=(), bc="periodic", seed=0, t_end=1.0),
-    # configuration 5: 3D GLM-MHD periodic vortex at 256^3, LLF, to t = 0.5
+    # 3D GLM-MHD periodic vortex at 256^3, LLF, to t = 0.5
     "mhd3d": dict(dim=3, n=(256, 256, 256), problem="mhd_vortex3d", user=(), bc="periodic", seed=0,
                   t_end=0.5, **_MHD),
 }

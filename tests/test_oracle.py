@@ -9,8 +9,9 @@ the HIP library, each part is tied to something the mathematics fixes:
   * directional symmetry of the 3D planar Riemann problem under x<->y<->z;
   * mirror symmetry with reflecting walls;
   * convergence: linear advection of sin(2 pi (x+y)) converges at high order to
-    the exact cell averages (closed form) -- configuration "2D LinAdv" of the
-    path; Sod's shock tube converges to the exact Riemann solution in L1;
+    the exact cell averages (closed form), as the reference's 2D LinAdv
+    convergence case; Sod's shock tube converges to the exact Riemann solution
+    in L1;
   * RK tables: observed temporal order of each low-storage FSAL table on the
     linear (no-WENO) advection semi-discretisation.
 """
