@@ -82,7 +82,7 @@ PATH_F = {2: (12.4e3, 51.9e3), 3: (26.9e3, 87.9e3)}  # (linear path, full WENO) 
 # HBM bytes per launch of a kernel at 256^3, NQ = 2, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes over this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
 # factors a plain copy of known size gives); None where not measured.
-TRAFFIC = {"flux_rk": 23.12e9}
+TRAFFIC = {"flux_rk": 23.14e9}
 
 
 def cpu_baseline(seconds_target=15.0):
