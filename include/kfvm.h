@@ -44,7 +44,12 @@ typedef struct kfvm_info {
  * (src/Solver.cpp:36-81, src/ProblemSetup.H:142-205): parse the init file
  * (reals with std::stof semantics), build the GP weight tables, fill the
  * initial condition of the registry problem named by problem:name, prime the
- * FSAL register and choose dt0 = 0.01 cfl dmin / vmax. */
+ * FSAL register and choose dt0 = 0.01 cfl dmin / vmax.  The reference
+ * compiles its problem in, so its init files carry no problem:name: add that
+ * line (a registry problem of a reference exec, e.g. riem_3d_1c, also sets
+ * that exec's dimension, quadrature, equation, Riemann solver, RK table and
+ * boundary conditions).  An init file without it fails with a message that
+ * lists the registry.  Skipped lines are reported on stderr. */
 int kfvm_create(const char *init_path, kfvm_solver **out);
 /* Same, from the text of an init file. */
 int kfvm_create_from_text(const char *init_text, kfvm_solver **out);
@@ -68,10 +73,14 @@ int kfvm_take_step(kfvm_solver *s, int *status);
  * accepted.  No reference counterpart: it is the benchmark driver of the
  * per-stage update and follows TakeStep's stage sequence. */
 int kfvm_advance_stages(kfvm_solver *s, int64_t nstages, double dt);
-/* Solver::evalRHS (src/Solver.cpp:410-530) on the current solution
- * (which = 0) or the step-start solution (which = 1); the result becomes the
- * FSAL register.  *max_vel gets the largest wave speed over all faces. */
-int kfvm_eval_rhs(kfvm_solver *s, int which, double *max_vel);
+/* Solver::evalRHS(ConsDataView, Real t) (src/Solver.H:129,
+ * src/Solver.cpp:410-530) on the current solution (which = 0) or the
+ * step-start solution (which = 1) at time t; the result becomes the FSAL
+ * register.  *max_vel gets the largest wave speed over all faces.  t is the
+ * time the reference hands to time-dependent user boundary conditions and
+ * sources; the compiled-in problems have none, so it is only checked to be
+ * finite. */
+int kfvm_eval_rhs(kfvm_solver *s, int which, double t, double *max_vel);
 
 /* Solution / RHS / WENO-flag access (U_halo interior, RHS, wenoFlagView). */
 int kfvm_get_state(kfvm_solver *s, double *host, int64_t n);
@@ -171,6 +180,10 @@ int kfvm_write_output_file(const char *init_text, int checkpoint, const double *
                            int64_t len);
 int kfvm_read_checkpoint_file(const char *init_text, const char *path, double *U, int64_t n, int *step,
                               double *time, double *dt, double *err_est);
+/* IntegrateIC_K (src/numeric/Numeric_K.H:31-107) without a device: the
+ * NQ^D-point Gauss cell averages of the problem the init text names, written
+ * to U [NV][nz][ny][nx] (n values) exactly as kfvm_create fills the solver. */
+int kfvm_initial_condition(const char *init_text, double *U, int64_t n);
 /* Feature and parameter counts of the learned indicator's network. */
 int kfvm_indicator_sizes(int dim, int64_t *ninputs, int64_t *nparams);
 

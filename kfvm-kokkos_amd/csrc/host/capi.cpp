@@ -2,12 +2,14 @@
 #include "../../../include/kfvm.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <string>
 #include <vector>
 
 #include "indicator.hpp"
+#include "problems.hpp"
 #include "solver.hpp"
 
 struct kfvm_solver {
@@ -40,9 +42,10 @@ int create(const kfvm::Config &cfg, const std::string &warnings, kfvm_solver **o
   if (!out) return fail("null output pointer");
   *out = nullptr;
   return guarded([&] {
+    // ProblemSetup::setFromFile reports every skipped line (ProblemSetup.H:190-195)
+    if (!warnings.empty()) std::fputs(warnings.c_str(), stderr);
     kfvm::Solver *s = new kfvm::Solver(cfg);
     *out = new kfvm_solver{s};
-    if (!warnings.empty() && cfg.verbose > 0) std::fputs(warnings.c_str(), stderr);
     return 0;
   });
 }
@@ -129,9 +132,10 @@ int kfvm_advance_stages(kfvm_solver *s, int64_t nstages, double dt) {
   });
 }
 
-int kfvm_eval_rhs(kfvm_solver *s, int which, double *max_vel) {
+int kfvm_eval_rhs(kfvm_solver *s, int which, double t, double *max_vel) {
   CHECK_SOLVER(s);
   if (which != 0 && which != 1) return fail("which must be 0 (current) or 1 (step start)");
+  if (!std::isfinite(t)) return fail("time must be finite");
   return guarded([&] {
     const double v = s->s->evalRHS(which);
     if (max_vel) *max_vel = v;
@@ -578,10 +582,27 @@ int kfvm_parse_init(const char *init_text, char *json, int64_t len) {
     if (c.haveAdvVel) o << c.advVel[0] << "," << c.advVel[1] << "," << c.advVel[2] << "]";
     o << ",\"user\":[";
     for (int i = 0; i < 8; i++) o << (i ? "," : "") << c.user[i];
-    o << "],\"seed\":" << c.seed << ",\"warnings\":" << std::count(warn.begin(), warn.end(), '\n') << "}";
+    o << "],\"seed\":" << c.seed << ",\"problem_set\":" << (c.problemSet ? "true" : "false") << ",\"warnings\":" << std::count(warn.begin(), warn.end(), '\n') << "}";
     const std::string s = o.str();
     if (int64_t(s.size()) + 1 > len) return fail("json buffer too small");
     std::memcpy(json, s.c_str(), s.size() + 1);
+    return 0;
+  });
+}
+
+int kfvm_initial_condition(const char *init_text, double *U, int64_t n) {
+  if (!init_text || !U) return fail("null argument");
+  return guarded([&] {
+    kfvm::Config c;
+    std::string warn;
+    kfvm::parse_init_text(init_text, c, &warn);
+    if (!c.problemSet) return fail("the init text names no problem (problem:name)");
+    if (!kfvm::problem_known(c.problem, c.eq)) return fail("unknown problem '" + c.problem + "' for this equation");
+    if (c.dim == 2) c.nZ = 1;
+    std::vector<double> V;
+    kfvm::fill_initial_condition(c, V);
+    if (n != int64_t(V.size())) return fail("U must hold NV*ncell doubles");
+    std::copy(V.begin(), V.end(), U);
     return 0;
   });
 }

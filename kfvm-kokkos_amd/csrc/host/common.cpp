@@ -3,6 +3,7 @@
 //   rk_table       follows src/numeric/Numeric.H:117-282
 //   parse_init_*   follows src/ProblemSetup.H:142-367 and src/ParameterStructs.cpp:15-38
 #include "common.hpp"
+#include "problems.hpp"
 
 #include <cmath>
 #include <fstream>
@@ -298,6 +299,7 @@ bool apply_line(const std::string &category, const std::string &setting, Config 
       if (key.size() == 2 && key[0] == 'p' && key[1] >= '0' && key[1] <= '7')
         c.user[key[1] - '0'] = std::stod(val(1));
       else if (key == "seed") c.seed = std::stoull(val(1));
+      else if (tok.size() >= 2) c.userNamed.emplace_back(key, val(1));
       else return false;
     } else if (category == "io") {
       if (key == "plotfreq") c.plotFreq = stof_d(val(1));
@@ -312,7 +314,10 @@ bool apply_line(const std::string &category, const std::string &setting, Config 
       } else return false;
     } else if (category == "problem") {
       // Replaces the reference's compile-time Definitions.H / ProblemSetup.cpp
-      if (key == "name") c.problem = val(1);
+      if (key == "name") {
+        c.problem = val(1);
+        c.problemSet = !c.problem.empty();
+      }
       else if (key == "dim") c.dim = std::stoi(val(1));
       else if (key == "nquad") c.nQuad = std::stoi(val(1));
       else if (key == "radius") c.rad = std::stoi(val(1));
@@ -357,23 +362,35 @@ bool apply_line(const std::string &category, const std::string &setting, Config 
 }  // namespace
 
 bool parse_init_text(const std::string &text, Config &cfg, std::string *warnings) {
+  // First pass: the problem's name, whose compiled-in scheme choices (the
+  // reference's Definitions.H) become the defaults the explicit keys override.
+  {
+    std::istringstream in(text);
+    std::string line;
+    while (std::getline(in, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream ls(line);
+      std::string category, setting;
+      std::getline(ls, category, ':');
+      std::getline(ls, setting, ':');
+      const std::vector<std::string> tok = split_ws(setting);
+      if (category == "problem" && tok.size() >= 2 && tok[0] == "name") cfg.problem = tok[1];
+    }
+    apply_problem_preset(cfg);
+  }
   std::istringstream in(text);
   std::string line;
-  bool sawZ = false;
   while (std::getline(in, line)) {
     if (line.empty() || line[0] == '#') continue;
     std::istringstream ls(line);
     std::string category, setting;
     std::getline(ls, category, ':');
     std::getline(ls, setting, ':');
-    if (category == "mesh" && setting.rfind("resolution", 0) == 0 &&
-        split_ws(setting).size() > 3)
-      sawZ = true;
     if (!apply_line(category, setting, cfg)) {
       if (warnings) *warnings += "[Warning] Ignoring line from input file: " + line + "\n";
     }
   }
-  (void)sawZ;
+  resolve_user_keys(cfg, warnings);
   if (cfg.dim == 2) cfg.nZ = 1;
   return true;
 }

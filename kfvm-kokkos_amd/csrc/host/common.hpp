@@ -73,6 +73,7 @@ struct Config {
   std::array<BC, 6> bc = {BC::Outflow, BC::Outflow, BC::Outflow,
                           BC::Outflow, BC::Outflow, BC::Outflow};
   std::string problem = "riemann3d_planar";
+  bool problemSet = false;  // the init file named its problem (problem:name)
   // time
   double finalTime = 1.0, minDt = 1.e-10, cfl = -1.0;  // cfl<0: dimension default
   double atol = 1.e-4, rtol = 1.e-4, rejectionThresh = 0.8;
@@ -85,6 +86,9 @@ struct Config {
   // user parameters of the problem registry
   double user[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint64_t seed = 0;
+  // user:<name> <value> lines of a reference problem's UserParameters.H,
+  // resolved against the registry once the problem is known
+  std::vector<std::pair<std::string, std::string>> userNamed;
   // 2D batch: `slices` independent problems of this configuration advanced
   // together (problem:slices), each with its own seed (seed + s unless given)
   int slices = 1;

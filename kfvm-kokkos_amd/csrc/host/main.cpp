@@ -1,18 +1,37 @@
-// kfvm.ex <file.init> -- command-line driver with the reference's contract
-// (src/main.cpp:23-55): parse the init file, build the solver, run Solve().
+// kfvm.ex <file.init> [problem] -- command-line driver with the reference's
+// contract (src/main.cpp:23-55): parse the init file, build the solver, run
+// Solve().  The reference compiles its problem into kfvm.ex; here the optional
+// second argument names the registry problem, as a problem:name line appended
+// to the file would, so a reference init file runs unchanged.
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <string>
 
 #include "../../../include/kfvm.h"
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: %s <file.init>\n", argv[0]);
+    std::fprintf(stderr, "usage: %s <file.init> [problem]\n", argv[0]);
     return 1;
   }
   std::printf("Setting options from input file: %s\n", argv[1]);
   kfvm_solver *s = nullptr;
-  if (kfvm_create(argv[1], &s) != 0) {
+  int rc;
+  if (argc > 2) {
+    std::ifstream f(argv[1]);
+    if (!f.is_open()) {
+      std::fprintf(stderr, "Error: could not open input file: %s\n", argv[1]);
+      return 1;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf() << "\nproblem:name " << argv[2] << "\n";
+    rc = kfvm_create_from_text(ss.str().c_str(), &s);
+  } else {
+    rc = kfvm_create(argv[1], &s);
+  }
+  if (rc != 0) {
     std::fprintf(stderr, "Error: %s\n", kfvm_last_error());
     return 1;
   }

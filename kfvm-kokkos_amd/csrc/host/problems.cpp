@@ -133,10 +133,86 @@ void ic_mhd_blast(const Config &c, uint64_t, double x, double y, double z, doubl
             0.0, s);
 }
 
+// Octant Riemann problem of exec/Hydro/Riem_3d_1c/InitialCondition.H: eight
+// constant states split at x = y = z = 0 (the state table is the problem's
+// data).
+void ic_riem_3d_1c(const Config &c, uint64_t, double x, double y, double z, double *s) {
+  // {rho, u, v, w, p} per octant, index (x >= 0) * 4 + (y >= 0) * 2 + (z >= 0)
+  static const double tab[8][5] = {
+      {1.016216216, -0.401442839, -0.401442839, -0.401442839, 1.4},  // bottom south west
+      {0.8, 0.0, 0.0, -0.727606875, 1.0},                             // top south west
+      {0.8, 0.0, -0.727606875, 0.0, 1.0},                             // bottom north west
+      {1.0, 0.727606875, 0.0, 0.0, 1.0},                              // top north west
+      {0.8, -0.727606875, 0.0, 0.0, 1.0},                             // bottom south east
+      {1.0, 0.0, 0.727606875, 0.0, 1.0},                              // top south east
+      {1.0, 0.0, 0.0, 0.727606875, 1.0},                              // bottom north east
+      {0.53125, 0.0, 0.0, 0.0, 0.4}};                                 // top north east
+  const double *t = tab[(x < 0.0 ? 0 : 4) + (y < 0.0 ? 0 : 2) + (z < 0.0 ? 0 : 1)];
+  prim2cons(c.gamma, t[0], t[1], t[2], t[3], t[4], s);
+}
+// Two-dimensional Riemann problem of exec/Hydro/Riem_2d_3c/InitialCondition.H
+// (quadrants split at x = y = 0.8), in its own conserved-variable form.
+void ic_riem_2d_3c(const Config &c, uint64_t, double x, double y, double, double *s) {
+  const double gamm = c.gamma - 1.0, a = 4.0 / std::sqrt(11.0);
+  s[3] = 0.0;
+  if (y > 0.8 && x > 0.8) {
+    s[0] = 1.5;
+    s[1] = s[2] = 0.0;
+    s[4] = 1.5 / gamm;
+  } else if (y > 0.8) {
+    s[0] = 33.0 / 62.0;
+    s[1] = s[0] * a;
+    s[2] = 0.0;
+    s[4] = 0.3 / gamm + s[0] * (8.0 / 11.0);
+  } else if (x > 0.8) {
+    s[0] = 33.0 / 62.0;
+    s[1] = 0.0;
+    s[2] = s[0] * a;
+    s[4] = 0.3 / gamm + s[0] * (8.0 / 11.0);
+  } else {
+    s[0] = 77.0 / 558.0;
+    s[1] = s[0] * a;
+    s[2] = s[0] * a;
+    s[4] = (9.0 / 310.0) / gamm + s[0] * (16.0 / 11.0);
+  }
+}
+// Linear-advection dispersion test of exec/LinAdv/DispRel/InitialCondition.H:
+// sin(2 pi freq (x cos ang + y sin ang)), ang = p0, freq = p1.
+void ic_disprel(const Config &c, uint64_t, double x, double y, double, double *s) {
+  const double xpar = x * std::cos(c.user[0]) + y * std::sin(c.user[0]);
+  s[0] = std::sin(2.0 * M_PI * c.user[1] * xpar);
+}
+
+// Scheme choices a reference problem compiles in (its Definitions.H and the
+// bcType array of its ProblemSetup.cpp); explicit problem: keys override them.
+struct Preset {
+  int dim, nq;
+  Eq eq;
+  Riemann rs;
+  RK rk;
+  BC bc[6];
+};
+// Named user parameter of a reference problem (its UserParameters.H): the
+// value goes to user slot `slot`, parsed with std::stof and multiplied by pi
+// when `timesPi` (DispRel's ang).
+struct UserKey {
+  const char *name;
+  int slot;
+  bool timesPi;
+  double dflt;
+};
+constexpr BC O = BC::Outflow, P = BC::Periodic;
+const Preset kRiem3d1c{3, 2, Eq::Hydro, Riemann::HLLC, RK::SSP4_3_2, {O, O, O, O, O, O}};  // Riem_3d_1c
+const Preset kRiem2d3c{2, 2, Eq::Hydro, Riemann::HLLC, RK::SSP4_3_2, {O, O, O, O, O, O}};  // Riem_2d_3c
+const Preset kDispRel{3, 2, Eq::LinAdv, Riemann::Exact, RK::SSP4_3_2, {P, P, P, P, P, P}};  // DispRel
+const UserKey kDispRelKeys[] = {{"ang", 0, true, 0.0}, {"freq", 1, false, 1.0}, {nullptr, 0, false, 0.0}};
+
 struct Entry {
   const char *name;
   Eq eq;
   PointIC fn;
+  const Preset *preset = nullptr;
+  const UserKey *keys = nullptr;
 };
 const Entry kRegistry[] = {
     {"linadv_sine", Eq::LinAdv, ic_linadv_sine},
@@ -151,20 +227,65 @@ const Entry kRegistry[] = {
     {"circ_alfven", Eq::MhdGlm, ic_circ_alfven},
     {"mhd_uniform", Eq::MhdGlm, ic_mhd_uniform},
     {"mhd_blast", Eq::MhdGlm, ic_mhd_blast},
+    {"riem_3d_1c", Eq::Hydro, ic_riem_3d_1c, &kRiem3d1c},
+    {"riem_2d_3c", Eq::Hydro, ic_riem_2d_3c, &kRiem2d3c},
+    {"disprel", Eq::LinAdv, ic_disprel, &kDispRel, kDispRelKeys},
 };
+
+const Entry *find_entry(const std::string &name) {
+  for (const Entry &e : kRegistry)
+    if (name == e.name) return &e;
+  return nullptr;
+}
 
 }  // namespace
 
 bool problem_known(const std::string &name, Eq eq) {
-  for (const Entry &e : kRegistry)
-    if (name == e.name) return e.eq == eq;
-  return false;
+  const Entry *e = find_entry(name);
+  return e && e->eq == eq;
+}
+
+std::string problem_names() {
+  std::string s;
+  for (const Entry &e : kRegistry) s += (s.empty() ? "" : ", ") + std::string(e.name);
+  return s;
+}
+
+void apply_problem_preset(Config &c) {
+  const Entry *e = find_entry(c.problem);
+  if (!e) return;
+  c.eq = e->eq;
+  if (e->preset) {
+    c.dim = e->preset->dim;
+    c.nQuad = e->preset->nq;
+    c.rs = e->preset->rs;
+    c.rk = e->preset->rk;
+    for (int f = 0; f < 6; f++) c.bc[f] = e->preset->bc[f];
+  }
+  if (e->keys)
+    for (const UserKey *k = e->keys; k->name; k++) c.user[k->slot] = k->dflt;
+}
+
+void resolve_user_keys(Config &c, std::string *warnings) {
+  const Entry *e = find_entry(c.problem);
+  for (const auto &kv : c.userNamed) {
+    bool done = false;
+    if (e && e->keys)
+      for (const UserKey *k = e->keys; k->name && !done; k++)
+        if (kv.first == k->name) {
+          try {
+            c.user[k->slot] = stof_d(kv.second) * (k->timesPi ? M_PI : 1.0);
+            done = true;
+          } catch (...) {
+          }
+        }
+    if (!done && warnings)
+      *warnings += "[Warning] Ignoring line from input file: user:" + kv.first + " " + kv.second + "\n";
+  }
 }
 
 void fill_initial_condition(const Config &c, std::vector<double> &U) {
-  const Entry *entry = nullptr;
-  for (const Entry &e : kRegistry)
-    if (c.problem == e.name) entry = &e;
+  const Entry *entry = find_entry(c.problem);
   if (!entry) throw std::runtime_error("unknown problem: " + c.problem);
   if (entry->eq != c.eq) throw std::runtime_error("problem " + c.problem + " does not match problem:equation");
   const int NV = num_vars(c.eq);

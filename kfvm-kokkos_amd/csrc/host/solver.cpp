@@ -34,6 +34,22 @@ T *dalloc(size_t n, const char *what) {
 }  // namespace
 
 Solver::Solver(const Config &cfg) : cfg_(cfg) {
+  // a failed set-up (bad restart file, device out of memory, ...) releases
+  // what it allocated before the exception leaves the constructor
+  try {
+    setup();
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void Solver::setup() {
+  if (!cfg_.problemSet)
+    throw std::runtime_error(
+        "the init file names no problem (problem:name): the reference compiles its problem in "
+        "(Definitions.H, InitialCondition.H), this library selects it from its registry: " +
+        problem_names());
   if (cfg_.dim != 2 && cfg_.dim != 3) throw std::runtime_error("problem:dim must be 2 or 3");
   if (cfg_.rad != 2) throw std::runtime_error("only stencil radius 2 is built");
   if (cfg_.nQuad != 2 && cfg_.nQuad != 3) throw std::runtime_error("problem:nquad must be 2 or 3");
@@ -397,8 +413,13 @@ std::string Solver::writeCkpt() {
   return writer_->writeCkpt(f, int(nTS), time, dt, errEst);
 }
 
-Solver::~Solver() {
-  profileEnable(false);
+Solver::~Solver() { release(); }
+
+void Solver::release() noexcept {
+  try {
+    profileEnable(false);
+  } catch (...) {
+  }
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
                   faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_};
   if (stream_) (void)hipStreamSynchronize(stream_);

@@ -96,6 +96,8 @@ class Solver {
     double ms = 0.0;
     int64_t count = 0;
   };
+  void setup();
+  void release() noexcept;
   template <class F>
   void timed(const char *name, F &&launch);
   void harvest();

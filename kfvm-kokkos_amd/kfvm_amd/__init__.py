@@ -36,6 +36,7 @@ EXPORTS = [
     "kfvm_indicator_get_weights", "kfvm_indicator_gradient", "kfvm_indicator_logits",
     "kfvm_indicator_sizes", "kfvm_set_nad", "kfvm_write_plot", "kfvm_write_checkpoint",
     "kfvm_write_output_file", "kfvm_read_checkpoint_file", "kfvm_create_batch", "kfvm_get_slice_times",
+    "kfvm_initial_condition",
 ]
 
 
@@ -69,7 +70,7 @@ def load_library(path=None):
         "kfvm_destroy": [P],
         "kfvm_solve": [P, ctypes.POINTER(I64)],
         "kfvm_take_step": [P, ctypes.POINTER(ctypes.c_int)],
-        "kfvm_eval_rhs": [P, ctypes.c_int, D],
+        "kfvm_eval_rhs": [P, ctypes.c_int, ctypes.c_double, D],
         "kfvm_advance_stages": [P, I64, ctypes.c_double],
         "kfvm_get_state": [P, D, I64],
         "kfvm_set_state": [P, D, I64],
@@ -92,6 +93,7 @@ def load_library(path=None):
         "kfvm_build_cell_weights": [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [D, I64],
         "kfvm_diff_matrix": [ctypes.c_int, D, I64],
         "kfvm_parse_init": [ctypes.c_char_p, ctypes.c_char_p, I64],
+        "kfvm_initial_condition": [ctypes.c_char_p, D, I64],
         "kfvm_set_indicator": [P, U16, I64, ctypes.c_double],
         "kfvm_get_indicator_counters": [P, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(ctypes.c_int)],
         "kfvm_set_nad": [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double],
@@ -311,6 +313,20 @@ def read_checkpoint_file(init_text, path, shape):
     return U, st.value, t.value, dt.value, e.value
 
 
+def initial_condition(text):
+    """The cell averages kfvm_create fills for an init text (host only, no GPU):
+    an array [NV, nz, ny, nx]."""
+    import math
+    lib = load_library()
+    p = parse_init(text)
+    nv = {0: 5, 1: 1, 2: 9}[p["eq"]]
+    n = p["n"]
+    nz = n[2] if p["dim"] == 3 else 1
+    U = np.empty((nv, nz, n[1], n[0]))
+    _check(lib, lib.kfvm_initial_condition(text.encode(), _dptr(U), U.size), "kfvm_initial_condition")
+    return U
+
+
 def parse_init(text):
     """The library's reading of an init file, as a dict."""
     import json
@@ -376,9 +392,13 @@ class Solver:
         self._call("kfvm_take_step", self._h, ctypes.byref(st))
         return st.value
 
-    def eval_rhs(self, which=0):
+    def eval_rhs(self, which=0, t=None):
+        """Solver::evalRHS of the current (0) or step-start (1) state at time t
+        (default: the solver's time)."""
         v = ctypes.c_double()
-        self._call("kfvm_eval_rhs", self._h, int(which), ctypes.byref(v))
+        if t is None:
+            t = self.time[0]
+        self._call("kfvm_eval_rhs", self._h, int(which), float(t), ctypes.byref(v))
         return v.value
 
     def advance_stages(self, n, dt):

@@ -460,6 +460,21 @@ struct Mhd {
   }
 };
 
+// RKFSAL_ErrEst_K (Numeric_K.H:262-290) for one cell: adds the squared scaled
+// differences (u - uhat) / (atol + rtol max(|u|, |uhat|)) to err and lowers
+// posFlag to the cell's positivity (-1 when u holds a NaN).  eq 0 hydro,
+// 1 linear advection, 2 GLM-MHD.
+inline void err_est_cell(int nv, int eq, const double *u, const double *uh, double atol, double rtol,
+                         double &err, double &posFlag) {
+  bool hasNan = false;
+  for (int v = 0; v < nv; v++) {
+    hasNan = hasNan || std::isnan(u[v]);
+    err += std::pow((u[v] - uh[v]) / (atol + rtol * std::fmax(std::fabs(u[v]), std::fabs(uh[v]))), 2);
+  }
+  const double pos = hasNan ? -1.0 : (eq == 0 ? Hydro::positivity(u) : (eq == 2 ? Mhd::positivity(u) : 1.0));
+  posFlag = posFlag > pos ? pos : posFlag;
+}
+
 struct Base {
   virtual ~Base() {}
   virtual void set_state(const double *) = 0;
@@ -473,6 +488,7 @@ struct Base {
   virtual int take_step() = 0;
   virtual void set_mhd_tables(const double *, const double *) {}
   virtual void face_states(int, double *) {}
+  virtual void weno_omegas(int64_t, int64_t, int64_t, double *) {}
   orc_config cfg{};
   double time = 0.0, dt = 0.0, errEst = 1.0;
   int64_t nTS = 1, nRhsEval = 0, nWeno = 0;
@@ -766,7 +782,8 @@ struct Solver : Base {
   }
 
   // ---- KFVM-WENO for one cell (KernelWenoRecon_K)
-  void recon_weno_cell(const std::vector<double> &V, int64_t i, int64_t j, int64_t k) {
+  void recon_weno_cell(const std::vector<double> &V, int64_t i, int64_t j, int64_t k,
+                       double *omegasOut = nullptr) {
     const int C = st.C, Cc = st.Cc, Cb = st.Cb, nSub = st.nSub, I = st.I;
     double Vf[33][NV];
     const double *ref = &V[uc(i, j, k)];
@@ -842,6 +859,9 @@ struct Solver : Base {
       om[0][v] /= gam_[0];
       for (int s = 1; s < nSub; s++) om[s][v] -= om[0][v] * gam_[s];
     }
+    if (omegasOut)
+      for (int s = 0; s < nSub; s++)
+        for (int v = 0; v < NV; v++) omegasOut[s * NV + v] = om[s][v];
     // face states (fillRiemannStates) then back to conserved variables
     for (int f = 0; f < F; f++)
       for (int q = 0; q < Q; q++) {
@@ -1304,6 +1324,11 @@ struct Solver : Base {
     cell_bcs(U);
     if (which == 0) {
       recon_linear(U);
+    } else if (which == 2) {
+      const std::vector<double> keep = flag;
+      recon_linear(U);
+      pos_pres(U);
+      flag = keep;
     } else {
       const std::vector<double> keep = flag;
 #pragma omp parallel for collapse(2) schedule(static)
@@ -1321,6 +1346,12 @@ struct Solver : Base {
             for (int q = 0; q < Q; q++)
               for (int v = 0; v < NV; v++)
                 out[(size_t(f * Q + q) * NV + v) * ncl + (size_t(k) * ny + j) * nx + i] = own(f, i, j, k, q)[v];
+  }
+
+  // Nonlinear WENO-AO weights of cell (i,j,k) of U, [nSub][NV]
+  void weno_omegas(int64_t i, int64_t j, int64_t k, double *out) override {
+    cell_bcs(U);
+    recon_weno_cell(U, i, j, k, out);
   }
 
   // ---- state access
@@ -1457,16 +1488,12 @@ struct Solver : Base {
       for (int64_t i = 0; i < nx; i++)
         for (int64_t j = 0; j < ny; j++)
           for (int64_t k = 0; k < nz; k++) {
-            bool hasNan = false;
-            double st_[NV];
+            double u[NV], uh[NV];
             for (int vv = 0; vv < NV; vv++) {
-              const double u = U[uc(i, j, k) + vv], uh = Uhat[ci(i, j, k) * NV + vv];
-              st_[vv] = u;
-              hasNan = hasNan || std::isnan(u);
-              errSum += std::pow((u - uh) / (cfg.atol + cfg.rtol * std::fmax(std::fabs(u), std::fabs(uh))), 2);
+              u[vv] = U[uc(i, j, k) + vv];
+              uh[vv] = Uhat[ci(i, j, k) * NV + vv];
             }
-            const double pos = hasNan ? -1.0 : (hydro ? Hydro::positivity(st_) : (mhd ? Mhd::positivity(st_) : 1.0));
-            posFlag = posFlag > pos ? pos : posFlag;
+            err_est_cell(NV, hydro ? 0 : (mhd ? 2 : 1), u, uh, cfg.atol, cfg.rtol, errSum, posFlag);
           }
       const double nDofs = double(nc) * NV;
       double errNew = 1.0 / std::sqrt(errSum / nDofs);
@@ -1570,6 +1597,16 @@ void orc_force_full_weno(orc_solver *s) { s->impl->useSparseWeno = false; }
 int64_t orc_num_weno(const orc_solver *s) { return s->impl->nWeno; }
 int orc_last_step(const orc_solver *s) { return s->impl->lastTimeStep ? 1 : 0; }
 void orc_face_states(orc_solver *s, int which, double *out) { s->impl->face_states(which, out); }
+void orc_weno_omegas(orc_solver *s, int64_t i, int64_t j, int64_t k, double *out) {
+  s->impl->weno_omegas(i, j, k, out);
+}
+void orc_error_norm(int nv, int eq, int64_t ncell, const double *U, const double *Uhat, double atol, double rtol,
+                    double *errSum, double *posFlag) {
+  double e = 0.0, p = 1.0;
+  for (int64_t c = 0; c < ncell; c++) orc::err_est_cell(nv, eq, U + c * nv, Uhat + c * nv, atol, rtol, e, p);
+  *errSum = e;
+  *posFlag = p;
+}
 double orc_riemann_point(int eq, int rs, double gamma, double ch, int dir, const double *uL,
                          const double *uR, double *F) {
   if (eq == 2) {

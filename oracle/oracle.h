@@ -69,6 +69,16 @@ void orc_set_mhd_tables(orc_solver *s, const double *cellW, const double *Dm);
 /* Own face states of every cell from the current state: linear (which = 0) or
  * full WENO + positivity limiter (which = 1); out[((f*Q + q)*NV + v)*ncell + cell]. */
 void orc_face_states(orc_solver *s, int which, double *out);
+/* which = 2 of orc_face_states: linear reconstruction followed by the
+ * positivity limiter (the limiter's theta is not kept). */
+/* Nonlinear WENO-AO weights [nSub][NV] of interior cell (i, j, k) of the
+ * current state (KernelWenoRecon_K::createOmegas, Stencil_K.H:556-580). */
+void orc_weno_omegas(orc_solver *s, int64_t i, int64_t j, int64_t k, double *out);
+/* RKFSAL_ErrEst_K (Numeric_K.H:262-290) over ncell cells of interleaved
+ * states U[cell][nv], Uhat[cell][nv]: the sum of squared scaled differences
+ * and the minimum positivity (-1 for a NaN).  eq as in orc_config. */
+void orc_error_norm(int nv, int eq, int64_t ncell, const double *U, const double *Uhat, double atol,
+                    double rtol, double *errSum, double *posFlag);
 /* One face-point Riemann solve in direction dir (0 x, 1 y, 2 z): eq 0 hydro
  * (rs 0 LLF, 1 HLL, 2 HLLC) or 2 GLM-MHD (rs 0 LLF, 1 HLL, 4 KEPES, cleaning
  * speed ch).  Writes the flux to F and returns the maximum wave speed. */

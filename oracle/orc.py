@@ -78,6 +78,10 @@ def lib():
         L.orc_set_mhd_tables.restype = None
         L.orc_face_states.argtypes = [P, ctypes.c_int, D]
         L.orc_face_states.restype = None
+        L.orc_weno_omegas.argtypes = [P, I64, I64, I64, D]
+        L.orc_weno_omegas.restype = None
+        L.orc_error_norm.argtypes = [ctypes.c_int, ctypes.c_int, I64, D, D, ctypes.c_double, ctypes.c_double, D, D]
+        L.orc_error_norm.restype = None
         L.orc_riemann_point.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, D, D, D]
         L.orc_riemann_point.restype = ctypes.c_double
@@ -97,6 +101,18 @@ def riemann_point(eq, rs, gamma, ch, direction, uL, uR):
     F = np.zeros_like(a)
     s = L.orc_riemann_point(EQ[eq], RS[rs], gamma, ch, direction, _dp(a), _dp(b), _dp(F))
     return F, s
+
+
+def error_norm(eq, U, Uhat, atol, rtol):
+    """RKFSAL_ErrEst_K over cells: (sum of squared scaled differences, min
+    positivity).  U, Uhat: [ncell, nv]."""
+    L = lib()
+    a = np.ascontiguousarray(U, dtype=np.float64)
+    b = np.ascontiguousarray(Uhat, dtype=np.float64)
+    assert a.shape == b.shape and a.ndim == 2
+    e, p = ctypes.c_double(), ctypes.c_double()
+    L.orc_error_norm(a.shape[1], EQ[eq], a.shape[0], _dp(a), _dp(b), atol, rtol, ctypes.byref(e), ctypes.byref(p))
+    return e.value, p.value
 
 
 class Oracle:
@@ -191,6 +207,25 @@ class Oracle:
         a = np.empty((F * Q, self.nv) + self.shape[1:])
         assert a.size == F * Q * self.nv * nc
         self.L.orc_face_states(self.h, 1 if weno else 0, _dp(a))
+        return a
+
+    def face_states_limited(self):
+        """Own face states after the linear reconstruction and the positivity
+        limiter, [F*Q, NV, nz, ny, nx] (the limiter's theta is not kept)."""
+        nc = int(np.prod(self.shape[1:]))
+        F = 2 * self.cfg.dim
+        Q = self.cfg.nq ** (self.cfg.dim - 1)
+        a = np.empty((F * Q, self.nv) + self.shape[1:])
+        assert a.size == F * Q * self.nv * nc
+        self.L.orc_face_states(self.h, 2, _dp(a))
+        return a
+
+    def weno_omegas(self, i, j, k=0):
+        """Nonlinear WENO-AO weights [nSub, NV] of cell (i, j, k)."""
+        nsub = 8 if self.cfg.dim == 3 else 6
+        assert 0 <= i < self.shape[3] and 0 <= j < self.shape[2] and 0 <= k < self.shape[1]
+        a = np.empty((nsub, self.nv))
+        self.L.orc_weno_omegas(self.h, i, j, k, _dp(a))
         return a
 
     def eval_rhs(self, which=0):

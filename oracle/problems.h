@@ -130,6 +130,41 @@ inline bool point_ic(const char *name, int dim, double gamma, const double *user
     hydro_prim2cons(gamma, user[0], user[1], user[2], user[3], user[4], s);
     return true;
   }
+  if (!std::strcmp(name, "riem_3d_1c")) {
+    /* exec/Hydro/Riem_3d_1c/InitialCondition.H: octants split at the origin */
+    double rho, u = 0.0, v = 0.0, w = 0.0, p = 1.0;
+    const double a = 0.727606875;
+    const bool ex = x >= 0.0, ny_ = y >= 0.0, tz = z >= 0.0;
+    if (!ex && !ny_ && !tz) { rho = 1.016216216; u = v = w = -0.401442839; p = 1.4; }
+    else if (!ex && !ny_) { rho = 0.8; w = -a; }
+    else if (!ex && !tz) { rho = 0.8; v = -a; }
+    else if (!ex) { rho = 1.0; u = a; }
+    else if (!ny_ && !tz) { rho = 0.8; u = -a; }
+    else if (!ny_) { rho = 1.0; v = a; }
+    else if (!tz) { rho = 1.0; w = a; }
+    else { rho = 0.53125; p = 0.4; }
+    hydro_prim2cons(gamma, rho, u, v, w, p, s);
+    return true;
+  }
+  if (!std::strcmp(name, "riem_2d_3c")) {
+    /* exec/Hydro/Riem_2d_3c/InitialCondition.H, quadrants split at 0.8 */
+    const double gm = gamma - 1.0, a = 4.0 / std::sqrt(11.0);
+    double r, mx = 0.0, my = 0.0, e;
+    if (y > 0.8) {
+      if (x > 0.8) { r = 1.5; e = 1.5 / gm; }
+      else { r = 33.0 / 62.0; mx = r * a; e = 0.3 / gm + r * (8.0 / 11.0); }
+    } else {
+      if (x > 0.8) { r = 33.0 / 62.0; my = r * a; e = 0.3 / gm + r * (8.0 / 11.0); }
+      else { r = 77.0 / 558.0; mx = r * a; my = r * a; e = (9.0 / 310.0) / gm + r * (16.0 / 11.0); }
+    }
+    s[0] = r; s[1] = mx; s[2] = my; s[3] = 0.0; s[4] = e;
+    return true;
+  }
+  if (!std::strcmp(name, "disprel")) {
+    /* exec/LinAdv/DispRel/InitialCondition.H with ang = user[0], freq = user[1] */
+    s[0] = std::sin(2.0 * M_PI * user[1] * (x * std::cos(user[0]) + y * std::sin(user[0])));
+    return true;
+  }
   if (!std::strcmp(name, "smooth_wave")) {
     /* smooth density/pressure wave advected by a uniform velocity */
     const double ph = 2.0 * M_PI * (x + (dim >= 2 ? y : 0.0) + (dim == 3 ? z : 0.0));

@@ -77,10 +77,12 @@ def test_solver_fails_loudly_without_a_gpu(have_gpu):
 
 def test_solver_rejects_invalid_configuration_before_device_work():
     # configuration errors are reported before any HIP call
-    for text, msg in [("problem:dim 4\n", "dim"),
+    for text, msg in [("problem:name riemann3d_planar\nproblem:dim 4\n", "dim"),
                       ("mesh:resolution 8 8 8\nproblem:equation linadv\nproblem:name linadv_sine\n", "exact"),
                       ("mesh:resolution 8 8 8\nproblem:name nope\n", "unknown problem"),
-                      ("mesh:resolution 8 8 8\nproblem:bc periodic outflow outflow outflow outflow outflow\n",
-                       "periodic")]:
+                      ("mesh:resolution 8 8 8\nproblem:name riemann3d_planar\n"
+                       "problem:bc periodic outflow outflow outflow outflow outflow\n", "periodic"),
+                      # a reference init file as it is: its problem is compiled in there
+                      ("mesh:resolution 8 8 8\ntime:final 0.1\n", "names no problem")]:
         with pytest.raises(kfvm_amd.KfvmError, match=msg):
             kfvm_amd.Solver(init_text=text)

@@ -1,0 +1,93 @@
+"""Final-time parity of whole runs: the HIP library and the CPU oracle each
+start from the initial condition they compute themselves, take the same
+fixed-dt steps (time:fixeddt, so the step sequences match) and are compared
+after the first step (per-cell relative difference <= 1e-12) and at the final
+time (L1 difference <= 1e-10, the mean over cells and variables of |a - b|).
+The reference loop these runs follow is Solver::Solve (Solver.cpp:84-140).
+
+  * 2D linear advection of sin(2 pi (x + y)), 128^2, NQ = 3, full WENO,
+    velocity (1, 1), to t = 1, where the exact solution is the initial cell
+    average again (its L1 error is reported);
+  * the 3D planar Riemann problem at 32^3 to t = 0.2 with KXRCF-sparse and
+    with full WENO; WENO-flag flips between the two runs are counted and
+    reported (the flags of both sides are compared after every step).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from helpers import cell_rel_diff, config, init_text, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(c):
+    import kfvm_amd
+    text = init_text(c)
+    s = kfvm_amd.Solver(init_text=text)
+    p = kfvm_amd.parse_init(text)
+    f, d = s.weights
+    o = oracle_for(c, p, f, d)
+    o.set_ic(c["problem"], p["user"], p["seed"])
+    d0 = np.abs(s.state - o.state).max() / np.abs(o.state).max()
+    assert d0 <= 1e-15, f"initial conditions differ: {d0:.3e}"
+    fl = np.zeros((2,) + s.shape[1:])
+    fl[1] = 1.0
+    o.flags = fl
+    o.prime()
+    return s, o, p
+
+
+def _run_to_final(name, c, expect_steps):
+    s, o, p = _pair(c)
+    t0 = time.time()
+    assert s.take_step() == 0 and o.take_step() == 0
+    r1 = cell_rel_diff(s.state, o.state).max()
+    print(f"[final] {name}: step 1 max cell rel diff {r1:.3e}", flush=True)
+    assert r1 <= 1e-12
+    flips = 0
+    nsteps = 1
+    while not o.L.orc_last_step(o.h):
+        assert s.take_step() == 0 and o.take_step() == 0
+        flips += int((s.flags[0] != o.flags[0]).sum())
+        nsteps += 1
+        # the last step may be a sliver: dt is clipped to the final time
+        assert nsteps <= expect_steps + 1
+    tg, _, _ = s.time
+    tc, _, _ = o.time
+    assert tg == tc == p["final"]
+    l1 = float(np.mean(np.abs(s.state - o.state)))
+    print(f"[final] {name}: {nsteps} steps to t = {tc}, L1 difference {l1:.3e}, WENO flag flips {flips}, "
+          f"{time.time() - t0:.1f} s", flush=True)
+    assert expect_steps <= nsteps <= expect_steps + 1
+    return s, o, l1, flips
+
+
+def test_linadv_128_full_weno_to_t1():
+    """The CPU-runnable configuration: 2D linear advection on [0,1]^2, 128^2,
+    u0 = sin(2 pi (x + y)), velocity (1, 1), NQ = 3, full WENO, SSP(4,3,2),
+    fixed dt = 0.8 dx (cfl 0.8), to t = 1."""
+    n = 128
+    c = config(dim=2, n=(n, n, 1), nq=3, equation="linadv", riemann="exact", problem="linadv_sine",
+               user=(1.0, 1.0, 0.0), bc="periodic", wenothresh=-1.0, advvel=(1.0, 1.0, 0.0),
+               cfl=0.8, fixeddt=0.8 / n, final=1.0)
+    s, o, l1, flips = _run_to_final("linadv 128^2 NQ=3 full WENO", c, 160)
+    assert l1 <= 1e-10
+    # exact solution at t = 1: the initial cell averages
+    xc = (np.arange(n) + 0.5) / n
+    exact = np.sin(2 * np.pi * (xc[None, :] + xc[:, None])) * np.sinc(1.0 / n) ** 2
+    err = float(np.mean(np.abs(s.state[0, 0] - exact)))
+    print(f"[final] linadv 128^2: L1 error vs exact {err:.3e}", flush=True)
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("wenothresh", [1.0, -1.0], ids=["kxrcf_sparse", "full_weno"])
+def test_riemann3d_32_to_t02(wenothresh):
+    """The 3D planar Riemann problem (x + y + z = 1.5), HLLC, NQ = 2, outflow,
+    at 32^3 with a fixed dt of 0.0025 (80 steps) to t = 0.2."""
+    c = config(n=(32, 32, 32), wenothresh=wenothresh, fixeddt=0.0025, final=0.2)
+    s, o, l1, flips = _run_to_final(f"riemann3d 32^3 wenothresh {wenothresh}", c, 80)
+    assert l1 <= 1e-10
+    if wenothresh > 0:
+        assert flips == 0

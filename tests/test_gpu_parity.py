@@ -1,10 +1,11 @@
 """Parity of the HIP stage library with the CPU oracle on the same inputs.
 
 Gate (per the path's contract): per-cell relative difference <= 1e-12 after one
-fixed-dt step, same weight table and initial state on both sides.  The
-relative difference of a cell is max_v |a_v - b_v| / max_v |b_v|.  Cells whose
-discrete decisions differ (WENO flag) are counted separately and must be zero
-here.
+fixed-dt step, same weight table on both sides, each side starting from the
+initial condition it computes itself (the two are first checked to agree to
+1e-15).  The relative difference of a cell is max_v |a_v - b_v| / max_v |b_v|.
+Cells whose discrete decisions differ (WENO flag) are counted separately and
+must be zero here.
 """
 import os
 import sys
@@ -25,13 +26,18 @@ def _pair(c):
     parsed = kfvm_amd.parse_init(init_text(c))
     f, d = s.weights
     o = oracle_for(c, parsed, f, d)
+    o.parsed = parsed
     return s, o
 
 
-def _start_oracle(s, o):
-    """Give the oracle the library's initial state and flags, then FSAL-prime it."""
-    U0 = s.state
-    o.state = U0
+def _start_oracle(s, o, c):
+    """The oracle computes its own initial condition (orc_set_ic), which must
+    equal the library's; then it FSAL-primes like the solver's constructor."""
+    p = o.parsed
+    o.set_ic(c["problem"], p["user"], p["seed"])
+    U0, V0 = s.state, o.state
+    d = np.abs(U0 - V0).max() / max(np.abs(V0).max(), 1e-300)
+    assert d <= 1e-15, f"initial conditions differ: {d:.3e}"
     fl = np.zeros((2,) + s.shape[1:])
     fl[1] = 1.0
     o.flags = fl
@@ -63,6 +69,15 @@ CASES = {
     "linadv2d_full_nq3": config(dim=2, n=(32, 32, 1), nq=3, equation="linadv", riemann="exact",
                                 problem="linadv_sine", user=(1.0, 1.0, 0.0), bc="periodic",
                                 wenothresh=-1.0, advvel=(1.0, 1.0, 0.0), fixeddt=0.01),
+    # the reference's LinAdv velocity (cos gamma, sin gamma) from eos:gamma
+    # (LinAdv_Riemann.H:30,76), no z flux, with the DispRel problem
+    "disprel_default_velocity": config(n=(16, 12, 8), equation="linadv", riemann="exact", problem="disprel",
+                                       user=(0.25, 2.0), bc="periodic", gamma=0.3, wenothresh=-1.0,
+                                       fixeddt=0.01),
+    # the two remaining RK tables (Numeric.H:163-282)
+    "riemann3d_fourstarp": config(n=(16, 16, 16), rk="fourstarp", fixeddt=0.002),
+    "quadrant2d_ssp10": config(dim=2, n=(32, 32, 1), problem="quadrant2d", seed=5, user=(), rk="ssp10_4_3",
+                               fixeddt=0.004),
     "linadv3d_sparse": config(n=(16, 16, 16), equation="linadv", riemann="exact",
                               problem="linadv_sine", user=(1.0, 1.0, 1.0), bc="periodic",
                               advvel=(1.0, 0.5, -0.25), fixeddt=0.01),
@@ -95,7 +110,7 @@ CASES = {
 def test_first_rhs_and_one_step(name):
     c = CASES[name]
     s, o = _pair(c)
-    _start_oracle(s, o)
+    _start_oracle(s, o, c)
     # first RHS (the FSAL prime) and the flags it leaves
     K_gpu, K_cpu = s.rhs, o.rhs
     scale = max(np.abs(K_cpu).max(), 1e-300)
@@ -123,7 +138,7 @@ def test_three_adaptive_steps_match():
     """Adaptive dt: the controller sees the same error norm up to summation order."""
     c = config(n=(16, 16, 16), wenothresh=1.0)
     s, o = _pair(c)
-    _start_oracle(s, o)
+    _start_oracle(s, o, c)
     t0g, dt0g, _ = s.time
     o.set_time(0.0, dt0g)
     for _ in range(3):
@@ -142,7 +157,7 @@ def test_unphysical_rejection_forces_full_weno():
     full WENO, exactly like the reference's state machine."""
     c = config(n=(16, 16, 16), fixeddt=None, cfl=50.0)
     s, o = _pair(c)
-    _start_oracle(s, o)
+    _start_oracle(s, o, c)
     _, dt, _ = s.time
     s.set_time(0.0, 0.2)
     o.set_time(0.0, 0.2)
