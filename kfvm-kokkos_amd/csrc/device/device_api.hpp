@@ -34,6 +34,45 @@ __host__ __device__ inline size_t fs_base(size_t c, int np) {
 }
 inline size_t fs_elems(size_t ncell, int np) { return (ncell + FSB - 1) / FSB * FSB * size_t(np); }
 
+// Surface planes of the fused flux path: the reconstructed states of every
+// brick's outer faces, one plane family per axis a, planes at the brick
+// boundaries (plane p lies below brick p along a; p = 0 .. nb_a).  Side 0 holds
+// the state of the cell below the plane (on its upper face), side 1 that of the
+// cell above it (on its lower face).  Value (point q, variable v) of the cell
+// with transverse index t sits at
+//   SB[off[a] + ((p * 2 + side) * (Q * 5) + q * 5 + v) * T[a] + t]
+// with t = k ny + j (a = x), k nx + i (a = y), j nx + i (a = z), so that one
+// store or load of a wave row covers consecutive cells.
+struct Surf {
+  long long off[3];
+  int T[3];
+  int B[3];  // brick extent along each axis
+};
+// Surface planes for a grid of bricks bx x by x bz holding q face points;
+// returns the number of doubles and fills sb.
+inline size_t surf_layout(int dim, const Grid &g, int bx, int by, int bz, int q, Surf &sb) {
+  const int n[3] = {g.nx, g.ny, g.nz}, b[3] = {bx, by, bz};
+  const int T[3] = {g.ny * g.nz, g.nx * g.nz, g.nx * g.ny};
+  size_t off = 0;
+  for (int a = 0; a < 3; a++) {
+    sb.off[a] = (long long)off;
+    sb.T[a] = T[a];
+    sb.B[a] = b[a];
+    if (a < dim) off += size_t((n[a] + b[a] - 1) / b[a] + 1) * 2 * size_t(q) * 5 * size_t(T[a]);
+  }
+  return off;
+}
+// Index of (face f of cell (i,j,k), point q, variable v) in SB.
+template <int D, int Q>
+__host__ __device__ inline size_t surf_index(const Surf &sb, const Grid &g, int f, int i, int j, int k, int q,
+                                             int v) {
+  const int a = f >> 1, hi = f & 1;
+  const int c = a == 0 ? i : (a == 1 ? j : k);
+  const int p = c / sb.B[a] + hi;
+  const int t = a == 0 ? k * g.ny + j : (a == 1 ? k * g.nx + i : j * g.nx + i);
+  return size_t(sb.off[a]) + (size_t(p * 2 + (hi ? 0 : 1)) * (Q * 5) + q * 5 + v) * size_t(sb.T[a]) + t;
+}
+
 enum RKMode : int { RK_PRIME = 0, RK_STAGE = 1, RK_FSAL = 2 };
 
 // Register-update epilogue of the flux kernel.
@@ -82,6 +121,29 @@ hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, d
 hipError_t launch_pp_lanes(int dim, int nq, const double *U, double *FS, double *thetaNew,
                            const double *props, const double *wflag, const Grid &g,
                            const int *list, const int *count, hipStream_t s);
+// ---- fused path of the Euler equations in sparse mode.  Rows: the listed
+// cells' limited face states, row t = [F*Q*5] (WENO cells at their list
+// positions, then the cells of the limiter's exact sequence), pos[cell] = the
+// cell's row or -1.
+// Surface planes for the grid (doubles; fills sb).
+size_t surf_elems(int dim, int nq, const Grid &g, Surf &sb);
+// Linear reconstruction with the fast limiter test (slow cells to ppList, the
+// others get theta 1) and the KXRCF indicator (WI may be null), storing only
+// the states on brick-boundary faces, into SB.
+hipError_t launch_linear_surf(int dim, int nq, const double *U, double *WI, double *thetaNew,
+                              const double *props, const Grid &g, const double *W0T, int *ppList, int *ppCount,
+                              double *SB, const Surf &sb, hipStream_t s);
+// The limiter's exact sequence on the slow cells of launch_linear_surf not
+// marked for WENO: recomputed linear states, limited, into rows
+// *wenoCount + 0, 1, ... (counted in *rowCount, zeroed by the caller).
+hipError_t launch_pp_rows(int dim, int nq, const double *U, double *rows, int *pos, double *thetaNew,
+                          const double *props, const double *wflag, const Grid &g, const double *W0,
+                          const int *list, const int *count, const int *wenoCount, int *rowCount, hipStream_t s);
+// Linear states of the cells without a row (recomputed, weights WQ
+// [Q][C][F]), rows of the others, SB across brick boundaries; Riemann solves,
+// divergence and the RK epilogue of launch_flux_rk.
+hipError_t launch_flux_fused(int dim, int nq, const double *U, const double *WQ, const double *rows, const int *pos,
+                             const double *SB, const Surf &sb, const Grid &g, const RKArgs &a, hipStream_t s);
 // pos (may be null): per-cell list position below cap, -1 otherwise
 hipError_t launch_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                          double *wflag, int *list, int *counter, double wThresh, const Grid &g,

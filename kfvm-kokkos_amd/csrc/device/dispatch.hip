@@ -25,6 +25,26 @@ hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, d
   KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, W0T, ppList, ppCount, s));
 }
 
+hipError_t launch_linear_surf(int dim, int nq, const double *U, double *WI, double *thetaNew,
+                              const double *props, const Grid &g, const double *W0T, int *ppList, int *ppCount,
+                              double *SB, const Surf &sb, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::linear_surf(U, WI, thetaNew, props, g, W0T, ppList, ppCount, SB, sb, s));
+}
+hipError_t launch_pp_rows(int dim, int nq, const double *U, double *rows, int *pos, double *thetaNew,
+                          const double *props, const double *wflag, const Grid &g, const double *W0,
+                          const int *list, const int *count, const int *wenoCount, int *rowCount, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::pp_rows(U, rows, pos, thetaNew, props, wflag, g, W0, list, count, wenoCount, rowCount, s));
+}
+hipError_t launch_flux_fused(int dim, int nq, const double *U, const double *WQ, const double *rows, const int *pos,
+                             const double *SB, const Surf &sb, const Grid &g, const RKArgs &a, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::flux_fused(U, WQ, rows, pos, SB, sb, g, a, s));
+}
+size_t surf_elems(int dim, int nq, const Grid &g, Surf &sb) {
+  const int q = dim == 3 ? nq * nq : nq;
+  return dim == 3 ? surf_layout(3, g, dev::RBox<3>::BX, dev::RBox<3>::BY, dev::RBox<3>::BZ, q, sb)
+                  : surf_layout(2, g, dev::RBox<2>::BX, dev::RBox<2>::BY, dev::RBox<2>::BZ, q, sb);
+}
+
 hipError_t launch_pp_lanes(int dim, int nq, const double *U, double *FS, double *thetaNew,
                            const double *props, const double *wflag, const Grid &g,
                            const int *list, const int *count, hipStream_t s) {

@@ -50,6 +50,28 @@ struct Launch {
                          dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
     return hipGetLastError();
   }
+  // ---- fused path (Euler, sparse WENO)
+  static hipError_t linear_surf(const double *U, double *WI, double *thetaNew, const double *props,
+                                const Grid &g, const double *W0T, int *ppList, int *ppCount, double *SB,
+                                const Surf &sb, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_linear_fused<D, NQ, true>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U,
+                       nullptr, WI, thetaNew, props, g, W0T, ppList, ppCount, SB, sb);
+    return hipGetLastError();
+  }
+  static hipError_t pp_rows(const double *U, double *rows, int *pos, double *thetaNew, const double *props,
+                            const double *wflag, const Grid &g, const double *W0, const int *list,
+                            const int *count, const int *wenoCount, int *rowCount, hipStream_t s) {
+    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 4096u);
+    hipLaunchKernelGGL((dev::k_pp_rows<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, rows, pos, thetaNew, props,
+                       wflag, g, W0, list, count, wenoCount, rowCount);
+    return hipGetLastError();
+  }
+  static hipError_t flux_fused(const double *U, const double *WQ, const double *rows, const int *pos,
+                               const double *SB, const Surf &sb, const Grid &g, const RKArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_flux_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, WQ, rows, pos,
+                       SB, sb, g, a);
+    return hipGetLastError();
+  }
   static hipError_t pp_lanes(const double *U, double *FS, double *thetaNew, const double *props,
                              const double *wflag, const Grid &g, const int *list, const int *count,
                              hipStream_t s) {

@@ -34,6 +34,13 @@ struct Group {
   }
 };
 
+// The stencil tables as a static constexpr member, so that fully unrolled
+// loops fold the offsets into immediates.
+template <int D>
+struct StTab {
+  static constexpr StencilTables<D> ST = make_stencil_tables<D>();
+};
+
 // Read-only data in the constant address space: uniform loads become scalar
 // (s_load) loads.
 typedef __attribute__((address_space(4))) double cdouble;
@@ -111,6 +118,99 @@ __device__ inline PPBounds pp_bounds_lds(const double *__restrict__ lds, int c0,
   return b;
 }
 
+// The linear states on the brick's outer faces (each face of the brick: the
+// cells next to it, all Q points of their face on it), for the neighbouring
+// bricks' flux kernel.  Work items are (face, variable, cell of the face)
+// with the face's items padded to whole waves, so the weights of a wave are
+// uniform (scalar loads); each item sums its Q points over the stencil in the
+// same order as the main passes, so the values equal theirs bit for bit.
+template <int D>
+struct SurfItems {
+  using Bx = RBox<D>;
+  static constexpr int T[3] = {Bx::BY * Bx::BZ, Bx::BX * Bx::BZ, Bx::BX * Bx::BY};
+  static constexpr int pad(int n) { return (n + 63) / 64 * 64; }
+  static constexpr int slots(int f) { return pad(5 * T[f >> 1]); }
+  static constexpr int total() {
+    int s = 0;
+    for (int f = 0; f < 2 * D; f++) s += slots(f);
+    return s;
+  }
+};
+
+template <int D, int NQ>
+__device__ inline void surface_states(const double *__restrict__ lds, const Grid &g, const double *__restrict__ WT,
+                                      double *__restrict__ SB, const Surf &sb, int i0, int j0, int k0, int tid) {
+  using Bx = RBox<D>;
+  using SI = SurfItems<D>;
+  constexpr auto ST = make_stencil_tables<D>();
+  constexpr int C = SInfo<D>::C, Q = Shape<D, NQ>::Q, H = Bx::H, NP = Bx::N;
+  constexpr int P = linear_pass_points(D, NQ);
+  const int lastX = min(Bx::BX, g.nx - i0) - 1, lastY = min(Bx::BY, g.ny - j0) - 1;
+  const int lastZ = D == 3 ? min(Bx::BZ, g.nz - k0) - 1 : 0;
+#pragma unroll 1
+  for (int slot = tid; slot < SI::total(); slot += Bx::NT) {
+    // face f of the brick (uniform over the wave) and item e within it
+    int f = 0, e = slot;
+#pragma unroll
+    for (int ff = 0; ff < 2 * D - 1; ff++)
+      if (f == ff && e >= SI::slots(ff)) {
+        e -= SI::slots(ff);
+        f = ff + 1;
+      }
+    const int ax = f >> 1, Tf = SI::T[ax];
+    if (e >= 5 * Tf) continue;
+    const int v = e / Tf, tl = e % Tf;
+    int x, y, z;
+    if (ax == 0) {
+      x = (f & 1) ? lastX : 0;
+      y = tl % Bx::BY;
+      z = tl / Bx::BY;
+    } else if (ax == 1) {
+      x = tl % Bx::BX;
+      y = (f & 1) ? lastY : 0;
+      z = tl / Bx::BX;
+    } else {
+      x = tl % Bx::BX;
+      y = tl / Bx::BX;
+      z = (f & 1) ? lastZ : 0;
+    }
+    if (x > lastX || y > lastY || z > lastZ) continue;
+    const int corner = (z * Bx::LY + y) * Bx::LX + x;
+    const double *p = lds + v * NP + corner;
+    // this face's Q points sit at positions p0 .. p0 + Q - 1 of pass fq0 / P
+    // f is the same on every lane of the wave (faces are padded to whole waves)
+    const int fu = __builtin_amdgcn_readfirstlane(f);
+    const int fq0 = fu * Q, pass = fq0 / P, p0 = fq0 % P;
+    double acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) acc[q] = 0.0;
+    // the stencil in chunks, the weight pointer made opaque per chunk so that
+    // at most one chunk's weights are in scalar registers at a time
+    constexpr int CH = 3;
+#pragma unroll
+    for (int n0 = 0; n0 < C; n0 += CH) {
+      const double *wt0 = WT + size_t(pass) * C * P + p0 + n0 * P;
+      asm volatile("" : "+s"(wt0));
+      const cdouble *wt = (const cdouble *)wt0;
+#pragma unroll
+      for (int m = 0; m < CH && n0 + m < C; m++) {
+        const int n = n0 + m;
+        const double xv = p[(ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
+                            (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0)];
+#pragma unroll
+        for (int q = 0; q < Q; q++) acc[q] = fma(wt[m * P + q], xv, acc[q]);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, Q, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    double *d = SB + surf_index<D, Q>(sb, g, f, i0 + x, j0 + y, k0 + z, 0, v);
+    const size_t T = size_t(sb.T[ax]);
+#pragma unroll
+    for (int q = 0; q < Q; q++) d[size_t(q) * 5 * T] = acc[q];
+  }
+}
+
 // Linear reconstruction + KXRCF indicator + limiter, one thread per cell of a
 // staged brick.  All five conserved variables of the brick and its halo sit
 // in LDS.  The face points are produced in passes of P points (both faces of
@@ -119,11 +219,16 @@ __device__ inline PPBounds pp_bounds_lds(const double *__restrict__ lds, int c0,
 // the uniform weights and stores the states; density and |m|^2 of the pass'
 // points stay in registers until the energy completes them, which gives
 // p/rho^gamma and the limiter's fast test without reading the states back.
-template <int D, int NQ>
+//
+// SURF = true (the fused flux path): no face state is stored except those of
+// the brick's outer faces (surface_states), into the surface planes SB (see
+// Surf), from which the neighbouring bricks' flux kernel reads them.
+template <int D, int NQ, bool SURF = false>
 __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
     const double *__restrict__ U, double *__restrict__ FS, double *__restrict__ WI,
     double *__restrict__ thetaNew, const double *__restrict__ props, Grid g,
-    const double *__restrict__ WT, int *__restrict__ ppList, int *__restrict__ ppCount) {
+    const double *__restrict__ WT, int *__restrict__ ppList, int *__restrict__ ppCount,
+    double *__restrict__ SB = nullptr, Surf sb = Surf{}) {
   using Bx = RBox<D>;
   constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = SInfo<D>::C, Q = Shape<D, NQ>::Q, H = Bx::H, NP = Bx::N;
@@ -151,9 +256,10 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
   __syncthreads();
   const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
-  if (!(i < g.nx && j < g.ny && (D == 2 || k < g.nz))) return;  // no barrier follows
+  if (i < g.nx && j < g.ny && (D == 2 || k < g.nz)) {  // no barrier inside
   const int cell = (k * g.ny + j) * g.nx + i;
   const int corner = (tz * Bx::LY + ty) * Bx::LX + tx;  // stencil box corner (-H,-H,-H)
+
   const int c0 = corner + H + Bx::LX * H + (D == 3 ? Bx::LX * Bx::LY * H : 0);
   double ref[5];
 #pragma unroll
@@ -186,8 +292,14 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
       __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
       __builtin_amdgcn_sched_group_barrier(0x002, P, 0);
     }
+    // FS is null on the fused path (SURF); the test stays a run-time one: with
+    // the stores compiled out the scheduler hoists the next variable's weight
+    // loads and runs out of scalar registers
+    if (FS) {
 #pragma unroll
-    for (int q = 0; q < P; q++) FS[fs_base(cell, Shape<D, NQ>::F * Shape<D, NQ>::Q * 5) + (size_t(ax * P + q) * 5 + v) * FSB] = acc[q];
+      for (int q = 0; q < P; q++)
+        FS[fs_base(cell, Shape<D, NQ>::F * Shape<D, NQ>::Q * 5) + (size_t(ax * P + q) * 5 + v) * FSB] = acc[q];
+    }
     // keep the scheduler from hoisting the next variable's stencil loads
     __builtin_amdgcn_sched_barrier(0);
   };
@@ -233,6 +345,9 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
     base = __shfl(base, leader);
     if (slow) ppList[base + __popcll(mask & ((1ull << lane) - 1ull))] = cell;
   }
+  }
+  // every thread of the block, live or not, takes surface items
+  if (SURF) surface_states<D, NQ>(lds, g, WT, SB, sb, i0, j0, k0, tid);
 }
 
 template <int D>
@@ -262,10 +377,6 @@ __global__ __launch_bounds__(BLK) void k_cell_props(const double *__restrict__ U
 // unrolled body with compile-time stencil indices (one fully unrolled loop
 // nest over all substencils exceeds the unroller's limits and would leave the
 // stencil values in scratch).
-template <int D>
-struct StTab {
-  static constexpr StencilTables<D> ST = make_stencil_tables<D>();
-};
 
 template <int D, int S>
 struct SubSten {

@@ -867,6 +867,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
 }  // namespace kfvm
 
 #include "recon5.hpp"
+#include "fused5.hpp"
 #include "mhd.hpp"
 #include "indicator.hpp"
 

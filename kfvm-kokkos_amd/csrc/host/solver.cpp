@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -127,10 +128,21 @@ void Solver::setup() {
   K_ = dalloc<double>(nc * nv_, "K");
   if (rk_.needsTmp()) Utmp_ = dalloc<double>(nc * nv_, "Utmp");
   FS_ = dalloc<double>(fs_elems(nc, int(FQ) * nv_), "face states");
-  if (nv_ == 5) {  // compact WENO staging for up to a quarter of the cells
-    wsCap_ = int(std::max<size_t>(1, (nc + 3) / 4));
-    WS_ = dalloc<double>(size_t(wsCap_) * FQ * nv_, "compact WENO states");
-    pos_ = dalloc<int>(nc, "compact WENO rows");
+  if (nv_ == 5) {
+    // fused reconstruction + flux path unless KFVM_FUSED=0 (then the linear
+    // pass stores every face state and a compact WENO staging holds the
+    // WENO states of up to a quarter of the cells)
+    const char *ev = std::getenv("KFVM_FUSED");
+    fused_ = !(ev && std::string(ev) == "0");
+    pos_ = dalloc<int>(nc, "row of each cell");
+    if (fused_) {
+      wsCap_ = int(nc);  // every listed cell has a row, in FS_
+      SB_ = dalloc<double>(surf_elems(cfg_.dim, cfg_.nQuad, grid_, sb_), "surface planes");
+      faceWQ_ = dalloc<double>(size_t(wts_.F) * wts_.Q * wts_.C, "face weights (point-major)");
+    } else {
+      wsCap_ = int(std::max<size_t>(1, (nc + 3) / 4));
+      WS_ = dalloc<double>(size_t(wsCap_) * FQ * nv_, "compact WENO states");
+    }
   }
   WI_ = dalloc<double>(fs_elems(nc, int(FQ)), "indicator");
   props_ = dalloc<double>(nc * 2, "cell properties");
@@ -149,7 +161,7 @@ void Solver::setup() {
   theta_[1] = dalloc<double>(nc, "theta");
   wflag_ = dalloc<double>(nc, "wflag");
   list_ = dalloc<int>(nc, "weno list");
-  counter_ = dalloc<int>(2, "weno / limiter counts");
+  counter_ = dalloc<int>(4, "weno / limiter counts");
   ppList_ = dalloc<int>(nc, "limiter list");
   faceW_ = dalloc<double>(wts_.face.size(), "face weights");
   derivW_ = dalloc<double>(wts_.deriv.size(), "deriv weights");
@@ -421,7 +433,7 @@ void Solver::release() noexcept {
   } catch (...) {
   }
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);
@@ -543,6 +555,46 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     timed("cell_props", [&] { return launch_cell_props(dim, Uin, props_, grid_, stream_); });
   }
   lastRhsFull_ = !useSparseWeno;
+  if (useSparseWeno && fused_) {
+    // linear pass (fast limiter test, KXRCF indicator, brick-boundary states)
+    // -> selection -> WENO and the limiter's exact sequence into rows ->
+    // fused reconstruction + Riemann + divergence + RK update
+    const bool lrn = learned_ && !forcePlain_;
+    check(hipMemsetAsync(counter_ + 2, 0, sizeof(int), stream_), "clear row count");
+    timed("linear", [&] {
+      return launch_linear_surf(dim, nq, Uin, lrn ? nullptr : WI_, thNew, props_, grid_, faceWT_, ppList_,
+                                counter_ + 1, SB_, sb_, stream_);
+    });
+    if (lrn) {
+      timed("select", [&] {
+        return launch_learned_select(dim, 5, Uin, props_, thPrev, nadFlag_, netP_, netThr_, wflag_, list_, counter_,
+                                     grid_, pos_, wsCap_, stream_);
+      });
+    } else {
+      const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
+      timed("select", [&] {
+        return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, pos_, wsCap_,
+                             stream_);
+      });
+    }
+    timed("weno", [&] {
+      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, list_, counter_,
+                         FS_, wsCap_, stream_);
+    });
+    timed("weno_post", [&] {
+      return launch_weno_post(dim, nq, Uin, FS_, thNew, props_, grid_, list_, counter_, FS_, wsCap_, stream_);
+    });
+    timed("pp", [&] {
+      return launch_pp_rows(dim, nq, Uin, FS_, pos_, thNew, props_, wflag_, grid_, faceW_, ppList_, counter_ + 1,
+                            counter_, counter_ + 2, stream_);
+    });
+    th_ ^= 1;
+    timed("flux_rk", [&] {
+      return launch_flux_fused(dim, nq, Uin, faceWQ_, FS_, pos_, SB_, sb_, grid_, a, stream_);
+    });
+    nRhsEval++;
+    return;
+  }
   if (useSparseWeno) {
     // the learned selection does not read the KXRCF indicator
     double *wi = (learned_ && !forcePlain_ && hydro) ? nullptr : WI_;
@@ -938,6 +990,14 @@ void Solver::uploadWeights() {
   check(hipMemcpy(faceWT_, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(faceWS_, fs.data(), fs.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(derivWS_, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  if (faceWQ_) {  // full-stencil face weights per point: [Q][C][F]
+    const int Q = wts_.Q, F = wts_.F;
+    std::vector<double> wq(size_t(Q) * C * F);
+    for (int q = 0; q < Q; q++)
+      for (int n = 0; n < C; n++)
+        for (int f = 0; f < F; f++) wq[(size_t(q) * C + n) * F + f] = wts_.face[size_t(f * Q + q) * C + n];
+    check(hipMemcpy(faceWQ_, wq.data(), wq.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  }
   if (cellW_) {
     const int NI = wts_.nqCell;
     std::vector<double> cs(wts_.cell.size());

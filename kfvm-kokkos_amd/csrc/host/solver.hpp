@@ -123,12 +123,19 @@ class Solver {
   double *Uprev_ = nullptr, *Uhat_ = nullptr, *Utmp_ = nullptr, *K_ = nullptr;
   double *FS_ = nullptr, *WI_ = nullptr, *wflag_ = nullptr;
   double *WS_ = nullptr;  // compact WENO states of the first wsCap_ listed cells (Euler)
-  int *pos_ = nullptr;    // per cell: its row in WS_, or -1
+  int *pos_ = nullptr;    // per cell: its row in WS_ (fused path: in FS_), or -1
   int wsCap_ = 0;
+  // Euler sparse mode, fused path: the brick-boundary face states of the linear
+  // pass (surface planes) and the face weights per point [Q][C][F]; listed
+  // cells keep their states in rows of FS_
+  bool fused_ = false;
+  double *SB_ = nullptr, *faceWQ_ = nullptr;
+  Surf sb_{};
   double *props_ = nullptr;  // per-cell internal energy and sound speed [2][ncell]
   double *theta_[2] = {nullptr, nullptr};  // positivity theta: previous RHS / this RHS
   int th_ = 0;                             // theta_[th_] holds the latest values
-  int *list_ = nullptr, *counter_ = nullptr;  // WENO list; counter_[0] WENO, [1] limiter count
+  // WENO list; counter_[0] WENO cells, [1] limiter list, [2] limiter rows (fused path)
+  int *list_ = nullptr, *counter_ = nullptr;
   int *ppList_ = nullptr;                      // cells for the limiter's exact sequence
   // GLM-MHD: interior point states / sources, interior weights, differentiation
   // matrix, cleaning-speed maxima
