@@ -6,19 +6,23 @@ Riemann problem across x+y+z = 1.5 with (rho,u,v,w,p) = (1,0,0,0,1) /
 Gauss points per face, SSP(4,3,2) with the adaptive controller, to t = 0.2.
 It runs twice: plain (KXRCF selection of the WENO cells) and with the learned
 troubled-cell indicator (weights trained on the GPU by
-kfvm_amd.train_indicator on other planar problems at 64^3).  Each run warms
-up to t = 0.1 untimed, times K whole RK steps (take_step: reconstruction,
-selection, WENO, limiter, Riemann solves, divergence, register update, error
-norm, controller, and for the learned run the admissibility check), then
-finishes to t = 0.2 for the L1 density error against the exact (Sod)
-solution along the plane normal.  The state is resident in HBM throughout.
+kfvm_amd.train_indicator on other planar problems at 64^3).  Each run starts
+at t = 0, takes W untimed steps (default: until t = 0.1, the developed flow),
+times K whole RK steps (take_step: reconstruction, selection, WENO, limiter,
+Riemann solves, divergence, register update, error norm, controller, and for
+the learned run the admissibility check), then finishes to t = 0.2 for the L1
+density error against the exact (Sod) solution along the plane normal.  The
+state is resident in HBM throughout.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--retrain] [--plain-only]
 
-Prints one JSON line: value = zone-updates per RK stage per second of the
-learned run (plain and learned figures and the L1 ratio alongside), the
-roofline of the dominant kernel from HIP events, and the CPU baseline (the
-oracle on a bounded sample of the same problem).
+Prints one JSON line: value = zone-updates of the accepted RK stages per
+second in the timed window of the learned run (stages of rejected attempts
+cost time but add no zone-update); beside it the plain and learned figures,
+each run's wall time from t = 0 to t = 0.2 (time to solution) and L1 error,
+the roofline of the dominant kernel from HIP events taken during the timed
+window, and the CPU baseline (the oracle on a bounded sample of the same
+problem on every core this process may use).
 """
 import argparse
 import json
@@ -81,53 +85,78 @@ BYTES_STAGE = 40 * 5                           # read U_s, Uprev, Uhat; write U_
 PATH_F = {2: (12.4e3, 51.9e3), 3: (26.9e3, 87.9e3)}  # (linear path, full WENO) by NQ
 # HBM bytes per launch of a kernel at 256^3, NQ = 2, from rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes over this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
-# factors a plain copy of known size gives); None where not measured.
-TRAFFIC = {"flux_rk": 23.14e9}
+# factors a plain copy of known size gives), with the profile they come from;
+# None where not measured.
+TRAFFIC = {"flux_rk": (23.14e9, "profiles/flux_rk_traffic_riemann3d_256.md")}
 
 
-def cpu_baseline(seconds_target=15.0):
-    """The oracle (C++/OpenMP restatement, reference dataflow) on 48^3 of the
-    same problem, fixed dt, timed over whole RK steps."""
+def _host_cores():
+    """Cores this process may run on (the cpuset / affinity mask), and the CPU model."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
+def cpu_baseline(n=64, reps=5):
+    """The oracle (C++/OpenMP restatement of the reference's unfused dataflow)
+    on n^3 of the same problem with a fixed dt, on every core this process may
+    use: per method (KXRCF-sparse and full WENO) one warm-up step (4 stages),
+    then the median of `reps` timed steps, in zone-updates per stage per second."""
     import numpy as np
     import kfvm_amd
     from orc import Oracle
-    threads = min(os.cpu_count() or 1, 16)
-    n = 48
-    text = init_text(n, 1.0)
-    p = kfvm_amd.parse_init(text)
-    f, d = kfvm_amd.build_weights(3, 2, p["lfac"], 1.0 / n, 1.0 / n, 1.0 / n)
-    o = Oracle(dim=3, n=(n, n, n), face_w=f, deriv_w=d, nq=2, eq="hydro", rs="hllc",
-               rk="ssp4_3_2", bc="outflow", gamma=p["gamma"], weno_thresh=p["wenothresh"],
-               cfl=p["cfl"], final_time=1e9, atol=p["atol"], rtol=p["rtol"],
-               fixed_dt=0.002, threads=threads)
-    o.set_ic("riemann3d_planar", (1.5,))
-    o.prime()
-    o.take_step()  # warm-up step
-    e0 = o.rhs_evals
-    t0 = time.perf_counter()
-    steps = 0
-    while time.perf_counter() - t0 < seconds_target and steps < 50:
-        o.take_step()
-        steps += 1
-    el = time.perf_counter() - t0
-    stages = o.rhs_evals - e0
-    return {"value": n ** 3 * stages / el / 1e6, "unit": "Mzone-updates/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle (C++/OpenMP, reference unfused dataflow) on {n}^3 of the same "
-                      f"problem, {stages} RK stages ({steps} fixed-dt steps) in {el:.1f} s"}
+    threads, model = _host_cores()
+    out = {}
+    for label, wt in (("sparse", 1.0), ("full_weno", -1.0)):
+        text = init_text(n, wt)
+        p = kfvm_amd.parse_init(text)
+        f, d = kfvm_amd.build_weights(3, 2, p["lfac"], 1.0 / n, 1.0 / n, 1.0 / n)
+        o = Oracle(dim=3, n=(n, n, n), face_w=f, deriv_w=d, nq=2, eq="hydro", rs="hllc",
+                   rk="ssp4_3_2", bc="outflow", gamma=p["gamma"], weno_thresh=p["wenothresh"],
+                   cfl=p["cfl"], final_time=1e9, atol=p["atol"], rtol=p["rtol"],
+                   fixed_dt=0.0025, threads=threads)
+        o.set_ic("riemann3d_planar", (1.5,))
+        o.prime()
+        o.take_step()  # warm-up: 4 stages
+        times = []
+        for _ in range(reps):
+            e0 = o.rhs_evals
+            t0 = time.perf_counter()
+            o.take_step()
+            times.append((time.perf_counter() - t0) / (o.rhs_evals - e0))
+        out[label] = n ** 3 / float(np.median(times)) / 1e6
+    return {"value": out["sparse"], "unit": "Mzone-updates/s", "cores": threads, "kind": "port",
+            "full_weno_value": out["full_weno"], "cpu": model,
+            "sample": f"oracle (C++/OpenMP, the reference's unfused dataflow) on {n}^3 of the same problem, "
+                      f"fixed dt, {threads} threads: 1 warm-up step, then the median of {reps} steps per method; "
+                      f"value = KXRCF-sparse, full_weno_value = full WENO"}
 
 
 def _mode_run(n, weights, a, profile):
-    """One 256^3 run of the planar Riemann problem: untimed warm-up into the
-    developed flow, a timed window of whole RK steps (take_step: stages,
-    error norm, controller, admissibility check), then the rest of the run to
-    t = 0.2 for the L1 error."""
+    """One run of the planar Riemann problem from t = 0 to t = 0.2: untimed
+    warm-up steps, a timed window of K whole RK steps (take_step: stages,
+    error norm, controller, admissibility check), then the rest of the run.
+    Kernel times are HIP events on the solver's stream during the timed
+    window; the whole run's wall time is the time to solution."""
     import numpy as np
     import kfvm_amd
     from kfvm_amd.exact import planar_riemann_l1
     s = kfvm_amd.Solver(init_text=init_text(n, a.wenothresh, nq=a.nq))
     if weights is not None:
         s.set_indicator(weights, 0.0)
+    s.sync()
+    t_run = time.perf_counter()
     nwarm = 0
     if a.warmup >= 0:
         for _ in range(a.warmup):
@@ -139,35 +168,46 @@ def _mode_run(n, weights, a, profile):
             nwarm += 1
     s.sync()
     c0 = s.counters
+    ind0 = s.indicator_counters if weights is not None else None
+    if profile:
+        s.profile(True)
     t0 = time.perf_counter()
     nweno = 0
+    accepted = 0
     for _ in range(a.steps):
-        s.take_step()
+        st = s.take_step()
+        accepted += st == 0
         nweno += s.counters["last_nweno"]
     s.sync()
     el = time.perf_counter() - t0
-    stages = s.counters["rhs_evals"] - c0["rhs_evals"]
-    value = n ** 3 * stages / el / 1e6
+    c1 = s.counters
+    stages = c1["rhs_evals"] - c0["rhs_evals"]
+    rejected = (c1["rejects_tol"] - c0["rejects_tol"]) + (c1["rejects_unphys"] - c0["rejects_unphys"])
     kern = {}
     if profile:
-        s.profile(True)
-        for _ in range(profile):
-            s.take_step()
         s.profile(False)
         for name in ("cell_props", "linear", "select", "weno", "weno_post", "pp", "flux_rk", "stage_pre", "nad"):
             ms, cnt = s.profile_get(name)
             if cnt:
                 kern[name] = {"ms_total": ms, "launches": cnt, "ms_avg": ms / cnt}
+    ind1 = s.indicator_counters if weights is not None else None
     while s.time[0] < 0.2 - 1e-12 and not a.no_l1:
         if s.take_step() not in (0, 1, 2):
             break
+    s.sync()
+    t_solution = time.perf_counter() - t_run
     l1 = planar_riemann_l1(s.state[0], s.time[0])
-    out = {"value": value, "ms_per_step": el / a.steps * 1e3, "warmup_steps": nwarm, "stages": stages,
-           "seconds": el,
+    rk_stages = 4  # SSP(4,3,2)
+    out = {"value": n ** 3 * rk_stages * accepted / el / 1e6,
+           "value_all_stages": n ** 3 * stages / el / 1e6,
+           "ms_per_step": el / a.steps * 1e3, "warmup_steps": nwarm, "stages": stages,
+           "accepted_steps": accepted, "rejected_attempts": rejected, "seconds": el,
            "weno_fraction": nweno / a.steps / n ** 3, "l1_rho_t0.2": l1, "t_final": s.time[0],
-           "steps_total": s.time[2] - 1}
+           "steps_total": s.time[2] - 1, "rejects_total": s.counters["rejects_tol"] + s.counters["rejects_unphys"],
+           "seconds_to_solution": t_solution}
     if weights is not None:
         out.update(s.indicator_counters)
+        out["nad_cells_per_step_timed"] = (ind1["nad_cells"] - ind0["nad_cells"]) / a.steps
     s.close()
     return out, kern
 
@@ -189,7 +229,9 @@ def _roofline(kern, head, ncell, n, nq):
         roof = {"bound": "hbm", "achieved": BYTES_STAGE * ncell / (d["ms_avg"] * 1e-3) / 1e9,
                 "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = TRAFFIC.get(dom) if n == 256 and nq == 2 else None
+    tr = TRAFFIC.get(dom) if n == 256 and nq == 2 else None
+    roof["traffic"] = tr[0] if tr else None
+    roof["traffic_source"] = tr[1] if tr else None
     roof["kernel"] = dom
     # the whole path against its FP64 ceiling at the measured WENO fraction
     f_lin, f_weno = PATH_F[nq]
@@ -214,7 +256,6 @@ def main():
     ap.add_argument("--learned-only", action="store_true")
     ap.add_argument("--no-l1", action="store_true", help="stop after the timed window (profiling runs)")
     ap.add_argument("--retrain", action="store_true", help="train the indicator on the GPU first")
-    ap.add_argument("--profile-steps", type=int, default=1)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -250,14 +291,17 @@ def main():
             weights = train_indicator.load("euler3d")
     plain, kp = None, {}
     if not a.learned_only or weights is None:
-        plain, kp = _mode_run(n, None, a, a.profile_steps if weights is None else 0)
-        print(f"[bench] plain: {plain['value']:.1f} Mzone-updates/s, WENO {plain['weno_fraction']:.3f}, "
-              f"L1 {plain['l1_rho_t0.2']:.5e}", flush=True)
+        plain, kp = _mode_run(n, None, a, weights is None)
+        print(f"[bench] plain: {plain['value']:.1f} Mzone-updates/s (accepted stages), WENO "
+              f"{plain['weno_fraction']:.3f}, L1 {plain['l1_rho_t0.2']:.5e}, {plain['steps_total']} steps "
+              f"in {plain['seconds_to_solution']:.2f} s to t = 0.2", flush=True)
     learned, kl = (None, {})
     if weights is not None:
-        learned, kl = _mode_run(n, weights, a, a.profile_steps)
-        print(f"[bench] learned: {learned['value']:.1f} Mzone-updates/s, WENO {learned['weno_fraction']:.3f}, "
-              f"L1 {learned['l1_rho_t0.2']:.5e}, redone steps {learned['nad_fallbacks']}", flush=True)
+        learned, kl = _mode_run(n, weights, a, True)
+        print(f"[bench] learned: {learned['value']:.1f} Mzone-updates/s (accepted stages), WENO "
+              f"{learned['weno_fraction']:.3f}, L1 {learned['l1_rho_t0.2']:.5e}, {learned['steps_total']} steps "
+              f"in {learned['seconds_to_solution']:.2f} s to t = 0.2, redone steps {learned['nad_fallbacks']}",
+              flush=True)
     head = learned if learned is not None else plain
     kern = kl if learned is not None else kp
     ncell = n ** 3
@@ -283,6 +327,8 @@ def main():
         "plain": plain,
         "learned": learned,
         "speedup_learned_vs_plain": (learned["value"] / plain["value"]) if learned and plain else None,
+        "time_to_solution_ratio_plain_over_learned": (plain["seconds_to_solution"] / learned["seconds_to_solution"])
+        if learned and plain else None,
         "l1_ratio_learned_vs_plain": (learned["l1_rho_t0.2"] / plain["l1_rho_t0.2"]) if learned and plain else None,
         "roofline": roof,
         "kernels_ms_avg": {k: v["ms_avg"] for k, v in kern.items()},

@@ -38,7 +38,7 @@ def test_roofline_of_the_dominant_kernel():
     # 200 B per cell over 2 ms
     assert abs(r["achieved"] - 200 * 256 ** 3 / 2e-3 / 1e9) < 1e-9
     assert abs(r["frac"] - r["achieved"] / bench.PEAK_HBM_GBS) < 1e-15
-    assert r["traffic"] == bench.TRAFFIC["flux_rk"]
+    assert r["traffic"] == bench.TRAFFIC["flux_rk"][0] and r["traffic_source"].startswith("profiles/")
     lin, weno = bench.PATH_F[2]
     assert abs(r["path"]["flop_per_zone_update"] - (lin + 0.05 * (weno - lin))) < 1e-9
     # another size: no counter figure for the traffic
