@@ -154,7 +154,7 @@ def _mode_run(n, weights, a, profile):
     from kfvm_amd.exact import planar_riemann_l1
     s = kfvm_amd.Solver(init_text=init_text(n, a.wenothresh, nq=a.nq))
     if weights is not None:
-        s.set_indicator(weights, 0.0)
+        s.set_indicator(weights, a.threshold)
     s.sync()
     t_run = time.perf_counter()
     nwarm = 0
@@ -256,6 +256,10 @@ def main():
     ap.add_argument("--learned-only", action="store_true")
     ap.add_argument("--no-l1", action="store_true", help="stop after the timed window (profiling runs)")
     ap.add_argument("--retrain", action="store_true", help="train the indicator on the GPU first")
+    ap.add_argument("--threshold", type=float, default=0.0,
+                    help="logit above which the learned indicator selects WENO")
+    ap.add_argument("--fused", action="store_true",
+                    help="fused reconstruction + flux path (the flux kernel recomputes the linear states)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -271,6 +275,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": nn * 8, "copy_bytes_written": nn * 8, "launches": 5}))
         return
+    if a.fused:
+        os.environ["KFVM_FUSED"] = "1"
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the domain decomposition over GPUs is design-only)", file=sys.stderr)
