@@ -81,6 +81,19 @@ int kfvm_advance_stages(kfvm_solver *s, int64_t nstages, double dt);
  * sources; the compiled-in problems have none, so it is only checked to be
  * finite. */
 int kfvm_eval_rhs(kfvm_solver *s, int which, double t, double *max_vel);
+/* kfvm_eval_rhs without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Evaluates the RHS of the device state u_dev ([NV][nz][ny][nx] doubles, the
+ * layout of kfvm_get_state) into rhs_dev (same layout, device memory) at time
+ * t, with the solver's configuration, weights, previous positivity theta and
+ * WENO mode, and leaves the solver unchanged.  Every per-cell scratch buffer
+ * of the evaluation lives in one file-scope device array of 256 MiB per GPU
+ * (held until the process ends); grids whose scratch exceeds it are rejected
+ * with an error.  Runs on the solver's stream and returns after it; inputs
+ * must be complete before the call.  All calls on one GPU share that array:
+ * calls that overlap on one GPU race. */
+int kfvm_eval_rhs_noalloc(kfvm_solver *s, const double *u_dev, double *rhs_dev, double t, double *max_vel);
 
 /* Solution / RHS / WENO-flag access (U_halo interior, RHS, wenoFlagView). */
 int kfvm_get_state(kfvm_solver *s, double *host, int64_t n);

@@ -30,7 +30,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
 HOST_CXX = ["common.cpp", "weights.cpp", "problems.cpp", "writer.cpp"]
 HOST_HIP = ["solver.cpp", "indicator.cpp", "capi.cpp"]
-DEVICE = ["dispatch.hip", "inst_d2q2.hip", "inst_d2q3.hip", "inst_d3q2.hip", "inst_d3q3.hip"]
+DEVICE = ["dispatch.hip", "inst_d2q2.hip", "inst_d2q3.hip", "inst_d3q2.hip", "inst_d3q3.hip", "noalloc.hip"]
 
 
 def _headers():

@@ -73,6 +73,13 @@ __host__ __device__ inline size_t surf_index(const Surf &sb, const Grid &g, int 
   return size_t(sb.off[a]) + (size_t(p * 2 + (hi ? 0 : 1)) * (Q * 5) + q * 5 + v) * size_t(sb.T[a]) + t;
 }
 
+// Scratch memory of the entry point that allocates nothing: a file-scope
+// device array of this many bytes per GPU (noalloc.hip).  noalloc_scratch()
+// returns its device address on the current device, taken with
+// hipGetSymbolAddress on the first call per device and kept.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+void *noalloc_scratch();
+
 enum RKMode : int { RK_PRIME = 0, RK_STAGE = 1, RK_FSAL = 2 };
 
 // Register-update epilogue of the flux kernel.

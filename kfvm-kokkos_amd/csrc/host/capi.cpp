@@ -143,6 +143,40 @@ int kfvm_eval_rhs(kfvm_solver *s, int which, double t, double *max_vel) {
   });
 }
 
+namespace {
+// true if p lies in device memory with at least `bytes` bytes from p to the
+// end of its allocation (clears the error a host pointer leaves)
+bool device_span(const void *p, size_t bytes) {
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  const char *b = static_cast<const char *>(base), *q = static_cast<const char *>(p);
+  return q >= b && size_t(q - b) + bytes <= size;
+}
+}  // namespace
+
+int kfvm_eval_rhs_noalloc(kfvm_solver *s, const double *u_dev, double *rhs_dev, double t, double *max_vel) {
+  CHECK_SOLVER(s);
+  if (!std::isfinite(t)) return fail("time must be finite");
+  return guarded([&] {
+    const size_t bytes = size_t(state_size(*s->s)) * sizeof(double);
+    if (!u_dev || !device_span(u_dev, bytes))
+      return fail("u must point to device memory holding NV*ncell doubles");
+    if (!rhs_dev || !device_span(rhs_dev, bytes))
+      return fail("rhs must point to device memory holding NV*ncell doubles");
+    const size_t need = s->s->noAllocBytes();
+    if (need > kfvm::kNoAllocScratchBytes)
+      return fail("this grid needs " + std::to_string(need) + " bytes of scratch; the no-allocation entry point "
+                  "has " + std::to_string(kfvm::kNoAllocScratchBytes) + " (use kfvm_eval_rhs)");
+    const double v = s->s->evalRhsNoAlloc(u_dev, rhs_dev);
+    if (max_vel) *max_vel = v;
+    return 0;
+  });
+}
+
 int kfvm_get_state(kfvm_solver *s, double *host, int64_t n) {
   CHECK_SOLVER(s);
   if (!host || n != state_size(*s->s)) return fail("state buffer must hold NV*ncell doubles");

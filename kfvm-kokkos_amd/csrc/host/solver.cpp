@@ -189,6 +189,12 @@ void Solver::setup() {
   check(hipHostMalloc((void **)&hostRed_, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
 
   uploadWeights();
+  // device address of the no-allocation entry point's scratch array, taken
+  // once per device at set-up; a failure is reported by that entry point
+  try {
+    (void)noalloc_scratch();
+  } catch (const std::exception &) {
+  }
   check(launch_fill(theta_[0], 1.0, nc, stream_), "fill theta");
   check(launch_fill(theta_[1], 1.0, nc, stream_), "fill theta");
   check(launch_fill(wflag_, 0.0, nc, stream_), "fill wflag");
@@ -668,6 +674,108 @@ double Solver::evalRHS(int which) {
   a.vmaxBits = vmax_;
   const double *Uin = which == 1 ? Uprev_ : (stateInCur_ ? U_[cur_] : Uprev_);
   pipeline(Uin, a);
+  return readSpeed();
+}
+
+size_t Solver::carveEval(unsigned char *base, EvalBufs &b) const {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void * {
+    off = (off + 255) / 256 * 256;
+    void *p = base ? base + off : nullptr;
+    off += bytes;
+    return p;
+  };
+  const size_t nc = size_t(ncell_), FQ = size_t(wts_.F) * wts_.Q, D = sizeof(double), I = sizeof(int);
+  b.FS = static_cast<double *>(take(fs_elems(nc, int(FQ) * nv_) * D));
+  b.WS = WS_ ? static_cast<double *>(take(size_t(wsCap_) * FQ * nv_ * D)) : nullptr;
+  if (SB_) {
+    Surf sb{};
+    b.SB = static_cast<double *>(take(surf_elems(cfg_.dim, cfg_.nQuad, grid_, sb) * D));
+  }
+  b.WI = static_cast<double *>(take(fs_elems(nc, int(FQ)) * D));
+  b.props = static_cast<double *>(take(nc * 2 * D));
+  b.thNew = static_cast<double *>(take(nc * D));
+  b.wflag = static_cast<double *>(take(nc * D));
+  b.SRC = SRC_ ? static_cast<double *>(take(nc * size_t(wts_.nqCell) * nv_ * D)) : nullptr;
+  b.list = static_cast<int *>(take(nc * I));
+  b.counter = static_cast<int *>(take(4 * I));
+  b.ppList = static_cast<int *>(take(nc * I));
+  b.pos = pos_ ? static_cast<int *>(take(nc * I)) : nullptr;
+  b.chBits = chBits_ ? static_cast<unsigned long long *>(take(2 * sizeof(unsigned long long))) : nullptr;
+  b.vmax = static_cast<unsigned long long *>(take(size_t(slices_) * sizeof(unsigned long long)));
+  return off;
+}
+
+size_t Solver::noAllocBytes() const {
+  EvalBufs b;
+  return carveEval(nullptr, b);
+}
+
+double Solver::evalRhsNoAlloc(const double *U, double *rhs) {
+  const size_t need = noAllocBytes();
+  if (need > kNoAllocScratchBytes)
+    throw std::runtime_error("this grid needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                             std::to_string(kNoAllocScratchBytes) + " of the no-allocation entry point");
+  unsigned char *base = static_cast<unsigned char *>(noalloc_scratch());
+  EvalBufs b;
+  carveEval(base, b);
+  // the solver's own buffers and evaluation state, put back on every exit
+  struct Restore {
+    Solver &s;
+    EvalBufs keep;
+    int th;
+    bool sparse, lastFull, prof;
+    int64_t nRhs;
+    ~Restore() {
+      s.FS_ = keep.FS;
+      s.WS_ = keep.WS;
+      s.SB_ = keep.SB;
+      s.WI_ = keep.WI;
+      s.props_ = keep.props;
+      s.theta_[th ^ 1] = keep.thNew;
+      s.wflag_ = keep.wflag;
+      s.SRC_ = keep.SRC;
+      s.list_ = keep.list;
+      s.counter_ = keep.counter;
+      s.ppList_ = keep.ppList;
+      s.pos_ = keep.pos;
+      s.chBits_ = keep.chBits;
+      s.vmax_ = keep.vmax;
+      s.th_ = th;
+      s.useSparseWeno = sparse;
+      s.lastRhsFull_ = lastFull;
+      s.profOn_ = prof;
+      s.nRhsEval = nRhs;
+    }
+  } restore{*this,
+            {FS_, WS_, SB_, WI_, props_, theta_[th_ ^ 1], wflag_, SRC_, list_, counter_, ppList_, pos_, chBits_,
+             vmax_},
+            th_,
+            useSparseWeno,
+            lastRhsFull_,
+            profOn_,
+            nRhsEval};
+  FS_ = b.FS;
+  WS_ = b.WS;
+  SB_ = b.SB;
+  WI_ = b.WI;
+  props_ = b.props;
+  theta_[th_ ^ 1] = b.thNew;
+  wflag_ = b.wflag;
+  SRC_ = b.SRC;
+  list_ = b.list;
+  counter_ = b.counter;
+  ppList_ = b.ppList;
+  pos_ = b.pos;
+  chBits_ = b.chBits;
+  vmax_ = b.vmax;
+  profOn_ = false;  // event timing creates events
+  check(hipMemsetAsync(vmax_, 0, size_t(slices_) * sizeof(unsigned long long), stream_), "clear vmax");
+  RKArgs a{};
+  a.mode = RK_PRIME;
+  a.K = rhs;
+  a.vmaxBits = vmax_;
+  pipeline(U, a);
   return readSpeed();
 }
 

@@ -30,6 +30,15 @@ class Solver {
   // Solver::evalRHS on the current (which=0) or step-start (which=1) state;
   // stores the FSAL register and returns the max wave speed.
   double evalRHS(int which);
+  // evalRHS of the device state U (layout of the solution registers) into
+  // rhs, with every per-cell scratch buffer of the evaluation (face states,
+  // indicator planes, lists, counters, this evaluation's theta, speeds) carved
+  // from the library's file-scope scratch array instead of the solver's own:
+  // no device allocation.  Reads the solver's weights and previous theta and
+  // changes none of its state.  Throws if the scratch does not fit.
+  double evalRhsNoAlloc(const double *U, double *rhs);
+  // scratch bytes evalRhsNoAlloc needs for this solver
+  size_t noAllocBytes() const;
   int takeStep();
   // 2D batches: number of slices and their times / step sizes
   int slices() const { return slices_; }
@@ -91,6 +100,16 @@ class Solver {
   bool profileGet(const std::string &name, double &ms, int64_t &count);
 
  private:
+  // Per-evaluation scratch buffers of one RHS evaluation (pipeline)
+  struct EvalBufs {
+    double *FS = nullptr, *WS = nullptr, *SB = nullptr, *WI = nullptr, *props = nullptr, *thNew = nullptr,
+           *wflag = nullptr, *SRC = nullptr;
+    int *list = nullptr, *counter = nullptr, *ppList = nullptr, *pos = nullptr;
+    unsigned long long *chBits = nullptr, *vmax = nullptr;
+  };
+  // Lay the buffers of this solver's evaluation out from base (256-byte
+  // aligned; base null: sizes only) and return the bytes they span.
+  size_t carveEval(unsigned char *base, EvalBufs &b) const;
   struct Prof {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     double ms = 0.0;

@@ -36,7 +36,7 @@ EXPORTS = [
     "kfvm_indicator_get_weights", "kfvm_indicator_gradient", "kfvm_indicator_logits",
     "kfvm_indicator_sizes", "kfvm_set_nad", "kfvm_write_plot", "kfvm_write_checkpoint",
     "kfvm_write_output_file", "kfvm_read_checkpoint_file", "kfvm_create_batch", "kfvm_get_slice_times",
-    "kfvm_initial_condition",
+    "kfvm_initial_condition", "kfvm_eval_rhs_noalloc",
 ]
 
 
@@ -71,6 +71,7 @@ def load_library(path=None):
         "kfvm_solve": [P, ctypes.POINTER(I64)],
         "kfvm_take_step": [P, ctypes.POINTER(ctypes.c_int)],
         "kfvm_eval_rhs": [P, ctypes.c_int, ctypes.c_double, D],
+        "kfvm_eval_rhs_noalloc": [P, P, P, ctypes.c_double, D],
         "kfvm_advance_stages": [P, I64, ctypes.c_double],
         "kfvm_get_state": [P, D, I64],
         "kfvm_set_state": [P, D, I64],
@@ -399,6 +400,18 @@ class Solver:
         if t is None:
             t = self.time[0]
         self._call("kfvm_eval_rhs", self._h, int(which), float(t), ctypes.byref(v))
+        return v.value
+
+    def eval_rhs_noalloc(self, u_ptr, rhs_ptr, t=None):
+        """The RHS of the device state at address u_ptr ([NV][nz][ny][nx]
+        doubles) into the device buffer at rhs_ptr, with no device allocation
+        (scratch in the library's fixed device array); the solver is left
+        unchanged.  Returns the largest wave speed."""
+        v = ctypes.c_double()
+        if t is None:
+            t = self.time[0]
+        self._call("kfvm_eval_rhs_noalloc", self._h, ctypes.c_void_p(u_ptr), ctypes.c_void_p(rhs_ptr), float(t),
+                   ctypes.byref(v))
         return v.value
 
     def advance_stages(self, n, dt):
