@@ -103,6 +103,9 @@ int kfvm_get_rhs(kfvm_solver *s, double *host, int64_t n);
 int kfvm_get_flags(kfvm_solver *s, double *host, int64_t n);
 int kfvm_set_flags(kfvm_solver *s, const double *host, int64_t n);
 
+/* The solver's time, next step size and step number (Solver::time, dt, nTS;
+ * src/Solver.H:63-66); set_time sets time and next step size, on a batch
+ * those of every problem. */
 int kfvm_get_time(kfvm_solver *s, double *t, double *dt, int64_t *step);
 int kfvm_set_time(kfvm_solver *s, double t, double dt);
 int kfvm_get_info(kfvm_solver *s, kfvm_info *info);

@@ -232,9 +232,10 @@ int kfvm_get_time(kfvm_solver *s, double *t, double *dt, int64_t *step) {
 
 int kfvm_set_time(kfvm_solver *s, double t, double dt) {
   CHECK_SOLVER(s);
-  s->s->time = t;
-  s->s->dt = dt;
-  return 0;
+  return guarded([&] {
+    s->s->setTime(t, dt);
+    return 0;
+  });
 }
 
 int kfvm_get_info(kfvm_solver *s, kfvm_info *info) {

@@ -181,9 +181,14 @@ void Solver::setup() {
     sDtDev_ = dalloc<double>(size_t(slices_), "slice dt");
     sRed_ = dalloc<double>(2 * size_t(slices_), "slice reductions");
     sMask_ = dalloc<int>(size_t(slices_), "slice mask");
+    thBak_ = dalloc<double>(nc, "accepted slices' theta");
+    wfBak_ = dalloc<double>(nc, "accepted slices' WENO flags");
     check(hipHostMalloc((void **)&sHost_, 4 * size_t(slices_) * sizeof(double), hipHostMallocDefault),
           "hipHostMalloc");
-    check(hipHostMalloc((void **)&sMaskHost_, size_t(slices_) * sizeof(int), hipHostMallocDefault),
+    // three host mask regions (attempt's pending slices, attempt's accepted
+    // slices, step's accepted slices), so that no region is rewritten while
+    // its asynchronous copy may still be queued
+    check(hipHostMalloc((void **)&sMaskHost_, 3 * size_t(slices_) * sizeof(int), hipHostMallocDefault),
           "hipHostMalloc");
   }
   check(hipHostMalloc((void **)&hostRed_, 4 * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
@@ -267,6 +272,17 @@ void Solver::setup() {
   }
 }
 
+void Solver::setTime(double t, double dtNext) {
+  time = t;
+  dt = dtNext;
+  for (int sl = 0; sl < slices_ && slices_ > 1; sl++) {
+    sT_[size_t(sl)] = t;
+    sDt_[size_t(sl)] = dtNext;
+    sDone_[size_t(sl)] = t >= cfg_.finalTime;
+    sLast_[size_t(sl)] = 0;
+  }
+}
+
 void Solver::sliceTimes(double *t, double *dtOut) const {
   for (int sl = 0; sl < slices_; sl++) {
     if (t) t[sl] = slices_ > 1 ? sT_[size_t(sl)] : time;
@@ -298,6 +314,11 @@ int Solver::takeStepSlices() {
   stateInCur_ = false;
   stagePos_ = 0;
   const int blocksPerSlice = nPart_ / S;
+  const bool hydro = nv_ == 5;
+  // slices accepted in this step, and slices whose next evaluation runs WENO
+  // on every cell (after an unphysical attempt)
+  std::vector<char> acceptedNow(static_cast<size_t>(S), 0), forceFull(static_cast<size_t>(S), 0);
+  const std::vector<char> doneBefore(sDone_);
   double *hDt = sHost_, *hRed = sHost_ + S, *hVm = sHost_ + 3 * S;
   for (int nT = 0; nT < cfg_.rejectionLimit; nT++) {
     if (std::none_of(pending.begin(), pending.end(), [](char p) { return p != 0; })) break;
@@ -309,6 +330,15 @@ int Solver::takeStepSlices() {
     check(hipMemcpyAsync(sMask_, sMaskHost_, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_), "mask");
     check(hipMemsetAsync(vmax_, 0, size_t(S) * sizeof(unsigned long long), stream_), "clear vmax");
     if (nT > 0) {  // the retrying slices' FSAL register, from their step start state
+      // Euler: a slice whose last attempt was unphysical gets WENO on every
+      // cell in this evaluation (Solver.cpp:298, 437-443) by a previous
+      // theta of 0 on all its cells, which the selectors flag; the other
+      // slices keep their own selection
+      for (int sl = 0; sl < S; sl++)
+        if (forceFull[size_t(sl)]) {
+          check(launch_fill(theta_[th_] + size_t(sl) * cps, 0.0, size_t(cps), stream_), "force full WENO");
+          forceFull[size_t(sl)] = 0;
+        }
       RKArgs a{};
       a.mode = RK_PRIME;
       a.K = K_;
@@ -362,7 +392,7 @@ int Solver::takeStepSlices() {
     std::memcpy(&cnt, &hostRed_[3], sizeof(int));
     lastNWeno = lastRhsFull_ ? ncell_ : cnt;
     for (int sl = 0; sl < S; sl++) {
-      sMaskHost_[sl] = 0;
+      sMaskHost_[S + sl] = 0;
       if (!pending[size_t(sl)]) continue;
       const double errSum = hRed[2 * sl], posFlag = hRed[2 * sl + 1], maxVel = hVm[sl];
       double &d = sDt_[size_t(sl)];
@@ -372,14 +402,18 @@ int Solver::takeStepSlices() {
       if (posFlag < 0.0) {
         d = firstUnphys[size_t(sl)] ? std::fmin(0.25 * cfg_.cfl * dmin / maxVel, d / 4.0) : d / 2.0;
         firstUnphys[size_t(sl)] = 0;
-        useSparseWeno = false;
+        if (hydro)
+          forceFull[size_t(sl)] = 1;
+        else
+          useSparseWeno = false;  // linear advection: every slice's next evaluation
         nRejectUnphys++;
         sLast_[size_t(sl)] = 0;
       } else if (dtfac >= cfg_.rejectionThresh || fixed) {
         sT_[size_t(sl)] += d;
         d = fixed ? cfg_.fixedDt : std::fmin(d * dtfac, cfg_.cfl * dmin / maxVel);
         sErr_[size_t(sl)] = errNew;
-        sMaskHost_[sl] = 1;
+        sMaskHost_[S + sl] = 1;
+        acceptedNow[size_t(sl)] = 1;
         pending[size_t(sl)] = 0;
         if (sLast_[size_t(sl)]) sDone_[size_t(sl)] = 1;
       } else {
@@ -388,8 +422,35 @@ int Solver::takeStepSlices() {
         sLast_[size_t(sl)] = 0;
       }
     }
-    check(hipMemcpyAsync(sMask_, sMaskHost_, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_), "mask");
+    check(hipMemcpyAsync(sMask_, sMaskHost_ + S, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_),
+          "mask");
     check(launch_commit_slices(U_[cur_], Uprev_, sMask_, nv_, int(ncell_), cps, stream_), "commit");
+    // the accepted slices' theta and WENO flags of their last evaluation: the
+    // retries of other slices re-evaluate every slice and would overwrite them
+    check(launch_commit_slices(theta_[th_], thBak_, sMask_, 1, int(ncell_), cps, stream_), "keep theta");
+    check(launch_commit_slices(wflag_, wfBak_, sMask_, 1, int(ncell_), cps, stream_), "keep flags");
+  }
+  // slices still rejected after the limit stop with their last attempt's
+  // state, theta and flags, as a single run does (stateInCur_)
+  if (std::any_of(pending.begin(), pending.end(), [](char p) { return p != 0; })) {
+    for (int sl = 0; sl < S; sl++) sMaskHost_[S + sl] = pending[size_t(sl)];
+    check(hipMemcpyAsync(sMask_, sMaskHost_ + S, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_),
+          "mask");
+    check(launch_commit_slices(U_[cur_], Uprev_, sMask_, nv_, int(ncell_), cps, stream_), "commit");
+    check(launch_commit_slices(theta_[th_], thBak_, sMask_, 1, int(ncell_), cps, stream_), "keep theta");
+    check(launch_commit_slices(wflag_, wfBak_, sMask_, 1, int(ncell_), cps, stream_), "keep flags");
+  }
+  // every slice accepted in this step, stopped at the rejection limit or
+  // finished before it keeps the theta and flags of its own last evaluation
+  // (the finished ones are re-evaluated with a zero step while the others go on)
+  for (int sl = 0; sl < S; sl++)
+    acceptedNow[size_t(sl)] = acceptedNow[size_t(sl)] || doneBefore[size_t(sl)] || pending[size_t(sl)];
+  if (std::any_of(acceptedNow.begin(), acceptedNow.end(), [](char a) { return a != 0; })) {
+    for (int sl = 0; sl < S; sl++) sMaskHost_[2 * S + sl] = acceptedNow[size_t(sl)];
+    check(hipMemcpyAsync(sMask_, sMaskHost_ + 2 * S, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_),
+          "mask");
+    check(launch_commit_slices(thBak_, theta_[th_], sMask_, 1, int(ncell_), cps, stream_), "restore theta");
+    check(launch_commit_slices(wfBak_, wflag_, sMask_, 1, int(ncell_), cps, stream_), "restore flags");
   }
   // slices still rejected after the limit stop where they are
   for (int sl = 0; sl < S; sl++)
@@ -443,7 +504,7 @@ void Solver::release() noexcept {
   } catch (...) {
   }
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void *p : bufs)
     if (p) (void)hipFree(p);

@@ -43,6 +43,8 @@ class Solver {
   // 2D batches: number of slices and their times / step sizes
   int slices() const { return slices_; }
   void sliceTimes(double *t, double *dt) const;
+  // time and next step size (of every problem of a batch)
+  void setTime(double t, double dtNext);
   int64_t solve();
   // Advance exactly n RK stages of fixed-dt steps, queued on the device with
   // no host synchronisation (a step completes every rkStages() stages).
@@ -186,6 +188,8 @@ class Solver {
   std::vector<char> sLast_, sDone_;
   double *sDtDev_ = nullptr, *sRed_ = nullptr, *sHost_ = nullptr;  // host: [dt S][red 2S][vmax S]
   int *sMask_ = nullptr, *sMaskHost_ = nullptr;
+  // theta and WENO flags of each slice's accepted evaluation in this step
+  double *thBak_ = nullptr, *wfBak_ = nullptr;
 
   std::unique_ptr<Writer> writer_;
   bool writePlot_ = false;  // the last step ended on a plot time

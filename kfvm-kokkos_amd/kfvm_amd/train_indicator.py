@@ -137,7 +137,7 @@ EVALUATIONS = {
 }
 
 
-def evaluate(name, weights=None, n=None, log=print, modes=("full", "kxrcf", "learned")):
+def evaluate(name, weights=None, n=None, log=print, modes=("full", "kxrcf", "learned"), t_end=None):
     """Full WENO (the reference's plain WENO), KXRCF-sparse and learned runs of
     the recipe's evaluation problem to its end time: wall time, zone-updates
     per second, mean WENO fraction, and each run's L1 distance (mean over
@@ -148,10 +148,11 @@ def evaluate(name, weights=None, n=None, log=print, modes=("full", "kxrcf", "lea
     r = RECIPES[name]
     W = load(name) if weights is None else weights
     size = n or e["n"]
+    t_end = t_end or e["t_end"]
     res = {}
     states = {}
     for mode in modes:
-        text = init_text(e["dim"], size, e["problem"], e["user"], e["seed"], e["bc"], final=e["t_end"],
+        text = init_text(e["dim"], size, e["problem"], e["user"], e["seed"], e["bc"], final=t_end,
                          wenothresh=-1.0 if mode == "full" else 1.0, gamma=e.get("gamma", 1.4),
                          equation=e.get("equation", "hydro"), riemann=e.get("riemann", "hllc"))
         s = k.Solver(init_text=text)
@@ -163,7 +164,7 @@ def evaluate(name, weights=None, n=None, log=print, modes=("full", "kxrcf", "lea
         s.sync()
         t0 = time.time()
         t_log = t0
-        while s.time[0] < e["t_end"] - 1e-12:
+        while s.time[0] < t_end - 1e-12:
             st = s.take_step()
             if st not in (0, 1, 2):
                 break
@@ -206,7 +207,8 @@ def load(name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("recipe", choices=sorted(RECIPES))
-    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--out", default=WEIGHTS, help="directory of the trained weights")
+    ap.add_argument("--eval-out", default=".", help="directory of the evaluation's JSON result")
     ap.add_argument("--evaluate", action="store_true", help="also run the recipe's evaluation problem")
     ap.add_argument("--evaluate-only", action="store_true")
     ap.add_argument("--n", type=int, default=0, help="evaluation resolution per axis (default: the recipe's)")
@@ -220,7 +222,7 @@ def main():
         modes = tuple(a.modes.split(","))
         res = evaluate(a.recipe, W, n=(a.n,) * dim if a.n else None, modes=modes)
         tag = (f"_n{a.n}" if a.n else "") + ("" if "full" in modes else "_sparse")
-        with open(os.path.join(a.out, f"{a.recipe}_evaluation{tag}.json"), "w") as f:
+        with open(os.path.join(a.eval_out, f"{a.recipe}_evaluation{tag}.json"), "w") as f:
             json.dump(res, f, indent=1)
 
 

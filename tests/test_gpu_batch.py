@@ -1,6 +1,10 @@
 """Batches of independent 2D problems (kfvm_create_batch): each problem of a
-batch evolves exactly as the same problem run alone, with fixed and with
-adaptive step sizes, and an ensemble of uneven sizes reproduces its members."""
+batch evolves bit for bit as the same problem run alone, with fixed and with
+adaptive step sizes (rejected attempts of one problem re-evaluate the others
+with a zero step, which must leave their state, limiter thetas and WENO flags
+untouched, and an unphysical attempt forces full WENO on its own problem
+only), an ensemble of uneven sizes reproduces its members, and one member
+agrees with the CPU oracle run of the same problem."""
 import numpy as np
 import pytest
 
@@ -34,26 +38,42 @@ def test_fixed_dt_batch_matches_single_runs():
     assert np.all(t == singles[0].time[0])
 
 
-def test_adaptive_batch_matches_single_runs():
+@pytest.mark.parametrize("kind", ["tolerance", "unphysical"])
+def test_adaptive_batch_matches_single_runs(kind):
+    """Adaptive steps: tight tolerances make members reject attempts for
+    accuracy (some to the rejection limit, where a member stops with its last
+    attempt's state); a first step about 5 times the stable one makes members
+    reject attempts as unphysical (the retry evaluates that member with full
+    WENO only)."""
     import kfvm_amd
-    c = config(dim=2, n=(48, 48, 1), problem="quadrant2d", user=(), final=0.04)
+    if kind == "tolerance":
+        c = config(dim=2, n=(48, 48, 1), problem="quadrant2d", user=(), final=0.04, atol=1e-6, rtol=1e-6)
+    else:
+        c = config(dim=2, n=(48, 48, 1), problem="quadrant2d", user=(), final=0.04, cfl=50.0)
     seeds = [1, 5, 9, 13]
     b = kfvm_amd.Solver(init_text=init_text(c), seeds=seeds)
+    first_dt = 0.04 if kind == "unphysical" else None  # about 5 times the stable step
+    if first_dt:
+        b.set_time(0.0, first_dt)
     b.solve()
     t, _ = b.slice_times
     U = b.state
+    rej = 0
     for k, sd in enumerate(seeds):
         s = _single(c, sd)
+        if first_dt:
+            s.set_time(0.0, first_dt)
         s.solve()
         cnt = s.counters
         d = np.abs(U[:, k:k + 1] - s.state).max()
-        print(f"[batch] adaptive, problem {k}: t {t[k]:.6f} vs {s.time[0]:.6f}, max diff {d:.3e}, "
+        print(f"[batch] adaptive ({kind}), problem {k}: t {t[k]:.6f} vs {s.time[0]:.6f}, max diff {d:.3e}, "
               f"rejections {cnt['rejects_tol']}+{cnt['rejects_unphys']}", flush=True)
         assert t[k] == s.time[0]
-        if cnt["rejects_tol"] + cnt["rejects_unphys"] == 0:
-            assert d == 0.0
-        else:
-            assert d < 1e-2 * np.abs(s.state).max()
+        assert d == 0.0
+        assert np.array_equal(b.flags[:, k:k + 1], s.flags)
+        rej += cnt["rejects_tol"] if kind == "tolerance" else cnt["rejects_unphys"]
+    # the batch's retries are exercised: some member rejected an attempt
+    assert rej > 0
 
 
 def test_ensemble_of_uneven_sizes_reproduces_its_members():
@@ -68,5 +88,32 @@ def test_ensemble_of_uneven_sizes_reproduces_its_members():
         s.solve()
         d = np.abs(e.state(k) - s.state).max()
         print(f"[batch] ensemble member {k} ({n}^2): max diff {d:.3e}", flush=True)
-        assert d < 1e-2 * np.abs(s.state).max()
+        assert d == 0.0
     e.close()
+
+
+def test_batch_member_matches_the_oracle():
+    """Member k of a fixed-dt batch against the oracle run of problem k (same
+    seed), each from its own initial condition: 3 steps, per-cell relative
+    difference <= 1e-12."""
+    import kfvm_amd
+    from helpers import cell_rel_diff, oracle_for
+    c = config(dim=2, n=(40, 36, 1), problem="quadrant2d", user=(), fixeddt=0.002)
+    seeds = [3, 7, 11]
+    b = kfvm_amd.Solver(init_text=init_text(c), seeds=seeds)
+    k = 1
+    text = init_text(dict(c, seed=seeds[k]))
+    p = kfvm_amd.parse_init(text)
+    f, dw = b.weights
+    o = oracle_for(c, p, f, dw)
+    o.set_ic(c["problem"], p["user"], p["seed"])
+    assert np.array_equal(o.state, b.state[:, k:k + 1])
+    fl = np.zeros((2,) + o.state.shape[1:])
+    fl[1] = 1.0
+    o.flags = fl
+    o.prime()
+    for _ in range(3):
+        assert b.take_step() == 0 and o.take_step() == 0
+    r = cell_rel_diff(b.state[:, k:k + 1], o.state).max()
+    print(f"[batch] member {k} vs oracle after 3 steps: max cell rel diff {r:.3e}", flush=True)
+    assert r <= 1e-12

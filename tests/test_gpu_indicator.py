@@ -197,6 +197,22 @@ def test_trained_indicator_gains_over_kxrcf_at_equal_l1():
     assert res["learned"][1] < 0.5 * res["plain"][1]
 
 
+def test_shear_layer_network_tracks_full_weno_like_kxrcf():
+    """The committed 2D network (trained on the shear layer at 256^2 and 512^2,
+    seeds 0-4) on the shear layer at 768^2, seed 0, a size it never saw, to
+    t = 1: fewer WENO cells than KXRCF, and its distance from the full-WENO
+    solution within 10 % of KXRCF's (the 1024^2 evaluation gives 1.014)."""
+    from kfvm_amd import train_indicator
+    res = train_indicator.evaluate("shear2d", n=(768, 768), t_end=1.0, log=lambda m: None)
+    kx, lr = res["kxrcf"], res["learned"]
+    print(f"[indicator] shear 768^2 t=1: WENO {kx['weno_fraction']:.3f} (KXRCF) / {lr['weno_fraction']:.3f} "
+          f"(learned); L1 from full WENO {kx['l1_vs_full_weno']:.4e} / {lr['l1_vs_full_weno']:.4e}; wall "
+          f"{res['full']['seconds']:.2f} / {kx['seconds']:.2f} / {lr['seconds']:.2f} s "
+          f"(full / KXRCF / learned)", flush=True)
+    assert lr["weno_fraction"] < 0.75 * kx["weno_fraction"]
+    assert lr["l1_vs_full_weno"] <= 1.10 * kx["l1_vs_full_weno"]
+
+
 @pytest.mark.parametrize("name", ["riemann3d", "quadrant2d", "mhd_blast3d"])
 def test_selecting_every_cell_equals_full_weno(name):
     """A network whose logit is always large lists every cell: the WENO
