@@ -1,8 +1,9 @@
 """Build the stage library in-tree.
 
 Produces kfvm-kokkos_amd/build/libkfvm_amd.so (HIP kernels for gfx950 plus the
-C++17 host driver behind the C ABI of include/kfvm.h) and the command-line
-driver kfvm-kokkos_amd/build/kfvm.ex.
+C++17 host driver behind the C ABI of include/kfvm.h), the command-line
+driver kfvm-kokkos_amd/build/kfvm.ex and the FP64 throughput probe
+kfvm-kokkos_amd/build/fp64_peak.
 
     python kfvm-kokkos_amd/build.py [--force]
 """
@@ -17,6 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "build")
 LIB = os.path.join(BUILD, "libkfvm_amd.so")
 EXE = os.path.join(BUILD, "kfvm.ex")
+PROBE = os.path.join(BUILD, "fp64_peak")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -92,6 +94,10 @@ def build(force=False, verbose=False):
     main_src = os.path.join(CSRC, "host", "main.cpp")
     if relink or _stale(EXE, main_src, deps):
         _run([CXX, "-O2", "-std=c++17", "-o", EXE, main_src, LIB, "-Wl,-rpath,$ORIGIN"])
+    # FP64 vector-FMA vs matrix-FMA throughput probe (run it under rocprofv3)
+    probe_src = os.path.join(CSRC, "probe", "fp64_peak.hip")
+    if force or _stale(PROBE, probe_src, []):
+        _run([HIPCC] + [f for f in HIP_FLAGS if f not in ("-x", "hip", "-fPIC")] + ["-o", PROBE, probe_src])
     return LIB
 
 

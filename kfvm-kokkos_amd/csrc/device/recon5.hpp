@@ -268,65 +268,97 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
   const double gm1 = g.gamma - 1.0;
   double thD = 1.0;
   bool okE = true;
-  // one variable of one pass: P dot products of length C, accumulated with
-  // the stencil walk outermost (each accumulator still sums n = 0..C-1 in
-  // order), so only the P accumulators and one stencil value are live and
-  // the weights of one stencil cell arrive as one scalar load
-  auto axis_var = [&](int ax, int v, double (&acc)[P]) {
-    const double *p = lds + v * NP + corner;
-    const double *wt0 = WT + ax * C * P;
-    // opaque to loop-invariant code motion: hoisting the weight loads out of
-    // the variable loop would park 2*C*P scalars in VGPR lanes; the constant
-    // address space keeps them scalar loads
-    asm volatile("" : "+s"(wt0));
-    const cdouble *wt = (const cdouble *)wt0;
+  // One pass: the P points of all five variables.  The stencil walk is
+  // outermost (each accumulator still sums n = 0..C-1 in order), so the P
+  // weights of one stencil cell -- one scalar load, shared by the five
+  // variables -- feed 5P FMAs.  The next stencil cell's weights and its five
+  // LDS values are fetched before this cell's FMAs issue (software
+  // pipelining): the wait for them falls after 5P FMAs instead of before each
+  // group of FMAs.  The weight pointer is made opaque once per stencil cell,
+  // which keeps the scalar loads of later cells from being hoisted (they
+  // would park their values in VGPR lanes).
+  auto pass = [&](int ax, double (&acc)[5][P]) {
+    const double *pc = lds + corner;
+    const double *wbase = WT + ax * C * P;
 #pragma unroll
-    for (int q = 0; q < P; q++) acc[q] = 0.0;
+    for (int v = 0; v < 5; v++)
+#pragma unroll
+      for (int q = 0; q < P; q++) acc[v][q] = 0.0;
+    double w[P], x[5];
+    {
+      const double *w0 = wbase;
+      asm volatile("" : "+s"(w0));
+      const cdouble *wc = (const cdouble *)w0;
+#pragma unroll
+      for (int q = 0; q < P; q++) w[q] = wc[q];
+      constexpr int o = ST.off[0][0] + H + Bx::LX * (ST.off[0][1] + H) +
+                        (D == 3 ? Bx::LX * Bx::LY * (ST.off[0][2] + H) : 0);
+#pragma unroll
+      for (int v = 0; v < 5; v++) x[v] = pc[v * NP + o];
+    }
 #pragma unroll
     for (int n = 0; n < C; n++) {
-      const double x = p[(ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
-                         (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0)];
+      double wn[P], xn[5];
+      if (n + 1 < C) {
+        // The next cell's addresses pass through an empty asm that takes this
+        // cell's weights and values as inputs: the wait for this cell's loads
+        // (one counter serves scalar and LDS loads, and scalar loads may
+        // return out of order, so a wait with a scalar load outstanding is a
+        // full one) falls before the next cell's loads are issued, not after.
+        const double *w1 = wbase + (n + 1) * P;
+        int cn = corner;
+        asm volatile("" : "+s"(w1), "+v"(cn) : "s"(w[0]), "s"(w[P - 1]), "v"(x[0]), "v"(x[1]), "v"(x[2]),
+                     "v"(x[3]), "v"(x[4]));
+        const cdouble *wc = (const cdouble *)w1;
 #pragma unroll
-      for (int q = 0; q < P; q++) acc[q] = fma(wt[n * P + q], x, acc[q]);
-      // interleave: one LDS read, then this stencil cell's P FMAs
-      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x002, P, 0);
-    }
-    // FS is null on the fused path (SURF); the test stays a run-time one: with
-    // the stores compiled out the scheduler hoists the next variable's weight
-    // loads and runs out of scalar registers
-    if (FS) {
+        for (int q = 0; q < P; q++) wn[q] = wc[q];
+        const int o = (ST.off[n + 1][0] + H) + Bx::LX * (ST.off[n + 1][1] + H) +
+                      (D == 3 ? Bx::LX * Bx::LY * (ST.off[n + 1][2] + H) : 0);
+#pragma unroll
+        for (int v = 0; v < 5; v++) xn[v] = lds[v * NP + cn + o];
+      }
 #pragma unroll
       for (int q = 0; q < P; q++)
-        FS[fs_base(cell, Shape<D, NQ>::F * Shape<D, NQ>::Q * 5) + (size_t(ax * P + q) * 5 + v) * FSB] = acc[q];
+#pragma unroll
+        for (int v = 0; v < 5; v++) acc[v][q] = fma(w[q], x[v], acc[v][q]);
+      if (n + 1 < C) {
+        // the next cell's LDS reads, then this cell's FMAs
+        __builtin_amdgcn_sched_group_barrier(0x100, 5, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 5 * P, 0);
+#pragma unroll
+        for (int q = 0; q < P; q++) w[q] = wn[q];
+#pragma unroll
+        for (int v = 0; v < 5; v++) x[v] = xn[v];
+      }
     }
-    // keep the scheduler from hoisting the next variable's stencil loads
     __builtin_amdgcn_sched_barrier(0);
   };
 #pragma unroll 1
   for (int ax = 0; ax < NPASS; ax++) {
-    double rho[P], msq[P], t[P];
-    axis_var(ax, 0, rho);
+    double acc[5][P];
+    pass(ax, acc);
+    if (FS) {
+#pragma unroll
+      for (int q = 0; q < P; q++)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+          FS[fs_base(cell, Shape<D, NQ>::F * Shape<D, NQ>::Q * 5) + (size_t(ax * P + q) * 5 + v) * FSB] =
+              acc[v][q];
+    }
 #pragma unroll
     for (int q = 0; q < P; q++) {
-      thD = dens_theta(thD, rho[q], pb);
-      msq[q] = 0.0;  // 0 + m^2 == m^2 exactly: same sums as Hydro_EOS.H
-    }
-#pragma unroll 1
-    for (int v = 1; v < 4; v++) {
-      axis_var(ax, v, t);
-#pragma unroll
-      for (int q = 0; q < P; q++) msq[q] = msq[q] + t[q] * t[q];
-    }
-    axis_var(ax, 4, t);
-#pragma unroll
-    for (int q = 0; q < P; q++) {
+      thD = dens_theta(thD, acc[0][q], pb);
+      // 0 + m^2 == m^2 exactly: the same sums as Hydro_EOS.H
+      double msq = 0.0;
+      msq = msq + acc[1][q] * acc[1][q];
+      msq = msq + acc[2][q] * acc[2][q];
+      msq = msq + acc[3][q] * acc[3][q];
       // Hydro::eint / pres / weno of the point (Hydro_EOS.H)
-      const double ei = t[q] - 0.5 * msq[q] / rho[q];
+      const double ei = acc[4][q] - 0.5 * msq / acc[0][q];
       okE = okE && ei > pb.minEint;
       if (WI) {  // the KXRCF indicator's input; not needed by the learned selection
         WI[fs_base(cell, Shape<D, NQ>::F * Shape<D, NQ>::Q) + size_t(ax * P + q) * FSB] =
-            (gm1 * ei) / pow(rho[q], g.gamma);
+            (gm1 * ei) / pow(acc[0][q], g.gamma);
         __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
       }
     }
