@@ -80,6 +80,20 @@ __host__ __device__ inline size_t surf_index(const Surf &sb, const Grid &g, int 
 constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
 void *noalloc_scratch();
 
+// Learned WENO selection run by the Euler linear pass on its staged brick:
+// network parameters and the selection's inputs and outputs (those of
+// launch_learned_select).
+struct LearnArgs {
+  const double *eint = nullptr;     // per-cell internal energy (the limiter's plane)
+  const double *theta = nullptr;    // positivity theta of the previous RHS
+  const double *nadFlag = nullptr;  // cells that failed the last admissibility check
+  const float *P = nullptr;         // network parameters (bfloat16 values)
+  float thr = 0.0f;                 // logit threshold
+  double *wflag = nullptr;
+  int *list = nullptr, *counter = nullptr, *pos = nullptr;
+  int cap = 0;
+};
+
 enum RKMode : int { RK_PRIME = 0, RK_STAGE = 1, RK_FSAL = 2 };
 
 // Register-update epilogue of the flux kernel.
@@ -119,9 +133,12 @@ hipError_t launch_cell_props(int dim, const double *U, double *props, const Grid
 // of linear_pass_points(dim, nq) points as [F*Q/P][C][P].  Euler: cells
 // whose states violate a limiter bound are appended to ppList (count in
 // *ppCount, zeroed by the caller) and limited by launch_pp_lanes.
+// la (Euler, may be null): the learned WENO selection runs in the same kernel
+// on the staged brick (outputs those of launch_learned_select).
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
                          double *thetaNew, const double *props, const Grid &g, const double *W0,
-                         const double *W0T, int *ppList, int *ppCount, hipStream_t s);
+                         const double *W0T, int *ppList, int *ppCount, hipStream_t s,
+                         const LearnArgs *la = nullptr);
 // Euler: the limiter's exact sequence on the cells of launch_linear whose
 // states failed the fast test (list, size *count); those wflag marks for
 // WENO are skipped.

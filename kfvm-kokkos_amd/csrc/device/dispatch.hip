@@ -20,9 +20,10 @@ hipError_t launch_cell_props(int dim, const double *U, double *props, const Grid
 
 hipError_t launch_linear(int dim, int nq, int nv, const double *U, double *FS, double *WI,
                          double *thetaNew, const double *props, const Grid &g, const double *W0,
-                         const double *W0T, int *ppList, int *ppCount, hipStream_t s) {
+                         const double *W0T, int *ppList, int *ppCount, hipStream_t s, const LearnArgs *la) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, W0T, ppList, ppCount, s));
+  if (la && nv != 5) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::linear(nv, U, FS, WI, thetaNew, props, g, W0, W0T, ppList, ppCount, s, la));
 }
 
 hipError_t launch_linear_surf(int dim, int nq, const double *U, double *WI, double *thetaNew,

@@ -358,6 +358,177 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__
   list_append(live, flagged, c, list, counter, pos, cap);
 }
 
+// The learned selection on the brick the Euler linear pass has staged (its
+// five conserved planes in lds, radius-2 halo): the same features, network and
+// decisions as k_learned_select, with the density from plane 0 and the
+// pressure written over plane 1 (the momenta are no longer needed), and each
+// cell's centre values, inverse deviations and range features passed to the
+// network lanes by shuffles within the wave instead of through LDS.
+template <int D>
+__device__ void learned_tail(double *lds, const LearnArgs &la, const Grid &g, int i0, int j0, int k0, int tid) {
+  using Bx = RBox<D>;
+  using N = Net<D>;
+  constexpr auto &ST = StTab<D>::ST;
+  constexpr int C = SInfo<D>::C, O = centre_index<D>(), H = Bx::H, NP = Bx::N, NT = Bx::NT;
+  constexpr int NIN = N::NIN, KS = (NIN + 31) / 32;
+  static_assert(NT % 64 == 0 && N::H == 32, "waves of 64 cells, 32 hidden units");
+  const double *sr = lds;
+  double *sp = lds + NP;
+  const double gm1 = g.gamma - 1.0;
+  __syncthreads();  // every linear pass of the block has read plane 1
+  for (int idx = tid; idx < NP; idx += NT) {
+    const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, cz_ = r / Bx::LY;
+    const int ex = box_out<Bx::BX, H>(a), ey = box_out<Bx::BY, H>(bb);
+    const int ez = D == 3 ? box_out<Bx::BZ, H>(cz_) : 0;
+    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + cz_ - H : k0;
+    // the cells the linear pass staged (every stencil cell of a live cell)
+    if (ex * ex + ey * ey + ez * ez <= H * H && gi < g.nx + H && gj < g.ny + H && (D == 2 || gk < g.nz + H))
+      sp[idx] = gm1 * la.eint[map_cell<D, 5>(g, gi, gj, gk).idx];
+  }
+  __syncthreads();
+  auto corner_of = [&](int t) {
+    const int tx = t % Bx::BX, ty = (t / Bx::BX) % Bx::BY, tz = t / (Bx::BX * Bx::BY);
+    return (tz * Bx::LY + ty) * Bx::LX + tx;
+  };
+  const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
+  const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+  const bool live = i < g.nx && j < g.ny && (D == 2 || k < g.nz);
+  const int c = live ? (k * g.ny + j) * g.nx + i : 0;
+  // this thread's cell: centre values, inverse deviations, range features
+  double r0 = 0.0, p0 = 0.0, ir = 0.0, ip = 0.0;
+  float xr = 0.0f, xp = 0.0f;
+  if (live) {
+    const int corner = corner_of(tid);
+    auto at = [&](int n) {
+      return corner + (ST.off[n][0] + H) + Bx::LX * (ST.off[n][1] + H) +
+             (D == 3 ? Bx::LX * Bx::LY * (ST.off[n][2] + H) : 0);
+    };
+    r0 = sr[at(O)];
+    p0 = sp[at(O)];
+    double smr = 0.0, smp = 0.0;
+#pragma unroll
+    for (int n = 0; n < C; n++) {
+      if (n == O) continue;
+      smr = fmax(smr, fabs(sr[at(n)] - r0));
+      smp = fmax(smp, fabs(sp[at(n)] - p0));
+    }
+    ir = 1.0 / (smr + 1e-300);
+    ip = 1.0 / (smp + 1e-300);
+    xr = float(fmin(smr / fabs(r0), 4.0));
+    xp = float(fmin(smp / fabs(p0), 4.0));
+  }
+  // ---- network, per wave on its 64 cells
+  const float *P = la.P;
+  const int lane = tid & 63, lg = lane >> 4, col = lane & 15, wbase = tid & ~63;
+  bf16x8 A1[2][KS], A2[2];
+  float b1v[2][4], b2v[2][4], w3v[2][4];
+  int foff[KS][8];
+#pragma unroll
+  for (int mt = 0; mt < 2; mt++) {
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int kk = 32 * ks + 8 * lg + e;
+        A1[mt][ks][e] = (__bf16)(kk < NIN ? P[N::oW1 + (16 * mt + col) * NIN + kk] : 0.0f);
+      }
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int j1 = e < 4 ? 4 * lg + e : 16 + 4 * lg + (e - 4);
+      A2[mt][e] = (__bf16)P[N::oW2 + (16 * mt + col) * N::H + j1];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      b1v[mt][e] = P[N::oB1 + 16 * mt + 4 * lg + e];
+      b2v[mt][e] = P[N::oB2 + 16 * mt + 4 * lg + e];
+      w3v[mt][e] = P[N::oW3 + 16 * mt + 4 * lg + e];
+    }
+  }
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int k0f = 32 * ks + e;
+      foff[ks][e] = lg == 0 ? feature_lds<D>(k0f)
+                            : (lg == 1 ? feature_lds<D>(k0f + 8)
+                                       : (lg == 2 ? feature_lds<D>(k0f + 16) : feature_lds<D>(k0f + 24)));
+    }
+  const float b3 = P[N::oB3];
+  float myz = 0.0f;
+#pragma unroll 1
+  for (int grp = 0; grp < 4; grp++) {
+    const int src = 16 * grp + col;  // the lane (of this wave) owning this lane's column cell
+    const int cl = wbase + src;
+    const int corner = corner_of(cl);
+    const double cr0 = __shfl(r0, src), cp0 = __shfl(p0, src), cir = __shfl(ir, src), cip = __shfl(ip, src);
+    const float cxr = __shfl(xr, src), cxp = __shfl(xp, src);
+    f32x4 acc[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc[mt][e] = b1v[mt][e];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      bf16x8 bh, bm, bl;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int kk = 32 * ks + 8 * lg + e;
+        const double v = kk < C - 1 ? sr[corner + foff[ks][e]] : sp[corner + foff[ks][e]];
+        float x;
+        if (kk < C - 1) x = float((v - cr0) * cir);
+        else if (kk < 2 * C - 2) x = float((v - cp0) * cip);
+        else if (kk == 2 * C - 2) x = cxr;
+        else if (kk == 2 * C - 1) x = cxp;
+        else x = 0.0f;
+        __bf16 h, m, l;
+        split3(x, h, m, l);
+        bh[e] = h;
+        bm[e] = m;
+        bl[e] = l;
+      }
+#pragma unroll
+      for (int mt = 0; mt < 2; mt++) {
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[mt][ks], bh, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[mt][ks], bm, acc[mt], 0, 0, 0);
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[mt][ks], bl, acc[mt], 0, 0, 0);
+      }
+    }
+    bf16x8 hh, hm, hl;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const float h1 = fmaxf(e < 4 ? acc[0][e] : acc[1][e - 4], 0.0f);
+      __bf16 h, m, l;
+      split3(h1, h, m, l);
+      hh[e] = h;
+      hm[e] = m;
+      hl[e] = l;
+    }
+    f32x4 acc2[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) acc2[mt][e] = b2v[mt][e];
+      acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[mt], hh, acc2[mt], 0, 0, 0);
+      acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[mt], hm, acc2[mt], 0, 0, 0);
+      acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[mt], hl, acc2[mt], 0, 0, 0);
+    }
+    float z = 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) z = fmaf(w3v[mt][e], fmaxf(acc2[mt][e], 0.0f), z);
+    z += __shfl_xor(z, 16);
+    z += __shfl_xor(z, 32);
+    if (lg == grp) myz = z + b3;  // the owner lane of this group's cells
+  }
+  bool flagged = false;
+  if (live) {
+    flagged = myz > la.thr || la.theta[c] < 1.0 || la.nadFlag[c] > 0.0;
+    la.wflag[c] = flagged ? 1.0 : 0.0;
+  }
+  list_append(live, flagged, c, la.list, la.counter, la.pos, la.cap);
+}
+
 // Features of every cell, feature-major rows: X[c][NIN] (training data).
 template <int D, int NV>
 __global__ __launch_bounds__(BLK) void k_features(const double *__restrict__ U, const double *__restrict__ eint,

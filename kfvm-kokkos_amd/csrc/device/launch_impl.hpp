@@ -22,8 +22,11 @@ struct Launch {
   }
   static hipError_t linear(int nv, const double *U, double *FS, double *WI, double *thetaNew,
                            const double *props, const Grid &g, const double *W0, const double *W0T,
-                           int *ppList, int *ppCount, hipStream_t s) {
-    if (nv == 5)
+                           int *ppList, int *ppCount, hipStream_t s, const LearnArgs *la) {
+    if (nv == 5 && la)
+      hipLaunchKernelGGL((dev::k_linear_fused<D, NQ, false, true>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s,
+                         U, FS, WI, thetaNew, props, g, W0T, ppList, ppCount, nullptr, Surf{}, *la);
+    else if (nv == 5)
       hipLaunchKernelGGL((dev::k_linear_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, FS, WI,
                          thetaNew, props, g, W0T, ppList, ppCount);
     else

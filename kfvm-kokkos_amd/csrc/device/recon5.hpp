@@ -211,6 +211,11 @@ __device__ inline void surface_states(const double *__restrict__ lds, const Grid
   }
 }
 
+// The learned WENO selection on the linear pass' staged brick (Euler);
+// defined with the learned indicator (indicator.hpp).
+template <int D>
+__device__ void learned_tail(double *lds, const LearnArgs &la, const Grid &g, int i0, int j0, int k0, int tid);
+
 // Linear reconstruction + KXRCF indicator + limiter, one thread per cell of a
 // staged brick.  All five conserved variables of the brick and its halo sit
 // in LDS.  The face points are produced in passes of P points (both faces of
@@ -223,12 +228,16 @@ __device__ inline void surface_states(const double *__restrict__ lds, const Grid
 // SURF = true (the fused flux path): no face state is stored except those of
 // the brick's outer faces (surface_states), into the surface planes SB (see
 // Surf), from which the neighbouring bricks' flux kernel reads them.
-template <int D, int NQ, bool SURF = false>
+//
+// LEARN = true: the learned WENO selection follows on the same brick
+// (learned_tail), in place of a separate selection kernel that would stage
+// the brick's density and pressure again.
+template <int D, int NQ, bool SURF = false, bool LEARN = false>
 __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
     const double *__restrict__ U, double *__restrict__ FS, double *__restrict__ WI,
     double *__restrict__ thetaNew, const double *__restrict__ props, Grid g,
     const double *__restrict__ WT, int *__restrict__ ppList, int *__restrict__ ppCount,
-    double *__restrict__ SB = nullptr, Surf sb = Surf{}) {
+    double *__restrict__ SB = nullptr, Surf sb = Surf{}, LearnArgs la = LearnArgs{}) {
   using Bx = RBox<D>;
   constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = SInfo<D>::C, Q = Shape<D, NQ>::Q, H = Bx::H, NP = Bx::N;
@@ -380,6 +389,8 @@ __global__ __launch_bounds__(RBox<D>::NT, 3) void k_linear_fused(
   }
   // every thread of the block, live or not, takes surface items
   if (SURF) surface_states<D, NQ>(lds, g, WT, SB, sb, i0, j0, k0, tid);
+  // every thread of the block, live or not, runs the selection (barriers, MFMA)
+  if (LEARN) learned_tail<D>(lds, la, g, i0, j0, k0, tid);
 }
 
 template <int D>

@@ -538,6 +538,52 @@ __device__ inline double riemann(const Grid &g, double ch, const double *uL, con
   }
 }
 
+// RK register update of one cell from its RHS (Numeric_K.H:198-290): the
+// FSAL prime stores K; a stage forms U_next (and Utmp, Uhat); the FSAL stage
+// finishes the embedded solution, keeps K and adds the cell's term of the
+// error norm, with the positivity minimum (NaN: -1).  write = false (PRIME /
+// FSAL of a batch slice that keeps its accepted step) skips the stores.
+template <int NV>
+__device__ inline void rk_epilogue(const RKArgs &a, const double *__restrict__ U, size_t c, size_t nc,
+                                   const double *rhs, double betaDt, double bhatDt, bool write, double &errLoc,
+                                   double &posLoc) {
+  if (a.mode == RK_PRIME) {
+    if (write) {
+#pragma unroll
+      for (int v = 0; v < NV; v++) a.K[size_t(v) * nc + c] = rhs[v];
+    }
+  } else if (a.mode == RK_STAGE) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const size_t o = size_t(v) * nc + c;
+      const double u = U[o];
+      double tmp = 0.0;
+      if (a.Utmp) {
+        tmp = a.Utmp[o] + a.delta * u;
+        a.Utmp[o] = tmp;
+      }
+      a.Unext[o] = a.g1 * u + a.g2 * tmp + a.g3 * a.Uprev[o] + betaDt * rhs[v];
+      a.Uhat[o] = a.Uhat[o] + bhatDt * rhs[v];
+    }
+  } else if (write) {  // RK_FSAL: finish the embedded solution, keep K, error norm
+    bool hasNan = false;
+    double st[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const size_t o = size_t(v) * nc + c;
+      const double u = U[o];
+      const double uh = a.Uhat[o] + bhatDt * rhs[v];
+      a.Uhat[o] = uh;
+      a.K[o] = rhs[v];
+      st[v] = u;
+      hasNan = hasNan || isnan(u);
+      const double e = (u - uh) / (a.atol + a.rtol * fmax(fabs(u), fabs(uh)));
+      errLoc += e * e;  // the reference's pow(e, 2), as a product (rounding-level difference)
+    }
+    posLoc = hasNan ? -1.0 : (NV == 5 ? Hydro::positivity(st) : (NV == 9 ? fmin(st[0], Mhd::eint(st)) : 1.0));
+  }
+}
+
 // Block-wide reductions with a fixed combination order (deterministic).
 __device__ inline double block_sum(double x, double *sh, int tid) {
   sh[tid] = x;
@@ -812,44 +858,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
         }
   }
   const bool write = !a.sliceWrite || a.sliceWrite[slice] != 0;
-  if (live) {
-    if (a.mode == RK_PRIME) {
-      if (write) {
-#pragma unroll
-        for (int v = 0; v < NV; v++) a.K[size_t(v) * nc + c] = rhs[v];
-      }
-    } else if (a.mode == RK_STAGE) {
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const size_t o = size_t(v) * nc + c;
-        const double u = U[o];
-        double tmp = 0.0;
-        if (a.Utmp) {
-          tmp = a.Utmp[o] + a.delta * u;
-          a.Utmp[o] = tmp;
-        }
-        a.Unext[o] = a.g1 * u + a.g2 * tmp + a.g3 * a.Uprev[o] + betaDt * rhs[v];
-        a.Uhat[o] = a.Uhat[o] + bhatDt * rhs[v];
-      }
-    } else if (write) {  // RK_FSAL: finish the embedded solution, keep K, error norm
-      bool hasNan = false;
-      double st[NV];
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        const size_t o = size_t(v) * nc + c;
-        const double u = U[o];
-        const double uh = a.Uhat[o] + bhatDt * rhs[v];
-        a.Uhat[o] = uh;
-        a.K[o] = rhs[v];
-        st[v] = u;
-        hasNan = hasNan || isnan(u);
-        const double e = (u - uh) / (a.atol + a.rtol * fmax(fabs(u), fabs(uh)));
-        errLoc += e * e;  // the reference's pow(e, 2), as a product (rounding-level difference)
-      }
-      posLoc = hasNan ? -1.0
-                      : (NV == 5 ? Hydro::positivity(st) : (NV == 9 ? fmin(st[0], Mhd::eint(st)) : 1.0));
-    }
-  }
+  if (live) rk_epilogue<NV>(a, U, c, nc, rhs, betaDt, bhatDt, write, errLoc, posLoc);
   const int bid = b;  // slice-major: the bricks of a 2D slice are one contiguous range
   const double bmax = block_max(vmax, sh, tid);
   if (tid == 0 && bmax > 0.0) atomicMax(a.vmaxBits + slice, (unsigned long long)__double_as_longlong(bmax));

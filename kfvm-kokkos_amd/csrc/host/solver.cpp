@@ -667,19 +667,29 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     return;
   }
   if (useSparseWeno) {
-    // the learned selection does not read the KXRCF indicator
-    double *wi = (learned_ && !forcePlain_ && hydro) ? nullptr : WI_;
+    // the learned selection does not read the KXRCF indicator; for the Euler
+    // equations it runs inside the linear pass, on the brick that pass stages
+    const bool lrn = learned_ && !forcePlain_ && hydro;
+    double *wi = lrn ? nullptr : WI_;
+    LearnArgs la;
+    if (lrn) {
+      la.eint = props_;
+      la.theta = thPrev;
+      la.nadFlag = nadFlag_;
+      la.P = netP_;
+      la.thr = netThr_;
+      la.wflag = wflag_;
+      la.list = list_;
+      la.counter = counter_;
+      la.pos = pos_;
+      la.cap = wsCap_;
+    }
     timed("linear", [&] {
       return launch_linear(dim, nq, nv_, Uin, FS_, wi, thNew, props_, grid_, faceW_, faceWT_, ppList_, counter_ + 1,
-                           stream_);
+                           stream_, lrn ? &la : nullptr);
     });
     if (hydro) {
-      if (learned_ && !forcePlain_) {
-        timed("select", [&] {
-          return launch_learned_select(dim, 5, Uin, props_, thPrev, nadFlag_, netP_, netThr_, wflag_, list_, counter_,
-                                       grid_, pos_, wsCap_, stream_);
-        });
-      } else {
+      if (!lrn) {
         const double wThresh = cfg_.wenoThresh * std::pow(cfg_.dmin(), 1.5);
         timed("select", [&] {
           return launch_select(dim, nq, Uin, WI_, thPrev, wflag_, list_, counter_, wThresh, grid_, pos_, wsCap_,
