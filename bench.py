@@ -87,7 +87,23 @@ PATH_F = {2: (12.4e3, 51.9e3), 3: (26.9e3, 87.9e3)}  # (linear path, full WENO) 
 # WRITE_SIZE passes over this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
 # factors a plain copy of known size gives), with the profile they come from;
 # None where not measured.
-TRAFFIC = {"flux_rk": (23.14e9, "profiles/flux_rk_traffic_riemann3d_256.md")}
+TRAFFIC = {"flux_rk": (23.54e9, "profiles/flux_rk_traffic_riemann3d_256.md"),
+           "linear": (20.87e9, "profiles/flux_rk_traffic_riemann3d_256.md")}
+# Algorithmic FP64 work per cell of the Riemann / divergence / RK kernel: 3 NQ^2
+# face solves per cell at ~200 flop-equivalents each (HLLC: 8 square roots,
+# 11 divisions and one power counted as 10 flops each, plus ~70 arithmetic
+# operations), the divergence (3 flops per face point and variable) and the
+# register update (~8 per variable).
+def flop_flux(nq):  # 2,620 at NQ = 2, 5,845 at NQ = 3
+    return 3 * nq * nq * 200 + 3 * nq * nq * 5 * 3 + 40
+# FP64 operations per cell the counters measure for the two largest kernels at
+# 256^3, NQ = 2 (rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 + _TRANS x 64 lanes / cells;
+# divisions and square roots count their Newton steps), and the fraction of
+# time the VALU is busy (SQ_ACTIVE_INST_VALU x 4 / SIMDs / cycles at the
+# kernel's traced duration), from the counter profile.
+COUNTERS = {"flux_rk": {"fp64_flop_per_cell": 8281.0, "valu_busy": 0.71},
+            "linear": {"fp64_flop_per_cell": 8836.0, "valu_busy": 0.58}}
+COUNTERS_SOURCE = "profiles/fp64_counters_riemann3d_256.md"
 
 
 def _host_cores():
@@ -214,25 +230,37 @@ def _mode_run(n, weights, a, profile):
 
 def _roofline(kern, head, ncell, n, nq):
     """Roofline of the kernel with the largest profiled time, plus the whole
-    path against its FP64 ceiling at the measured WENO fraction."""
+    path against its FP64 ceiling at the measured WENO fraction.  The
+    reconstruction, WENO and flux kernels do more FP64 work per byte of
+    algorithmic traffic than the ridge point (78.6 TFLOP/s / 8 TB/s = 9.8
+    flop/B; the flux kernel ~13), so their roof is FP64 compute ("mfma": the
+    FP64 peak, which the vector FMA pipe shares); the HBM figures are given
+    beside it."""
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
     d = kern[dom]
+    t = d["ms_avg"] * 1e-3
     if dom == "weno":
-        cells = head["weno_fraction"] * ncell
-        roof = {"bound": "mfma", "achieved": flop_weno(nq) * cells / (d["ms_avg"] * 1e-3) / 1e12,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
+        alg = flop_weno(nq) * head["weno_fraction"] * ncell
     elif dom == "linear":
-        roof = {"bound": "mfma", "achieved": flop_linear(nq) * ncell / (d["ms_avg"] * 1e-3) / 1e12,
-                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
+        alg = flop_linear(nq) * ncell
     else:
-        # flux/RK kernel: HBM roof on its algorithmic register traffic
-        roof = {"bound": "hbm", "achieved": BYTES_STAGE * ncell / (d["ms_avg"] * 1e-3) / 1e9,
-                "peak": PEAK_HBM_GBS, "unit": "GB/s"}
+        alg = flop_flux(nq) * ncell
+    roof = {"bound": "mfma", "achieved": alg / t / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "algorithmic_flop_per_launch": alg}
     roof["frac"] = roof["achieved"] / roof["peak"]
     tr = TRAFFIC.get(dom) if n == 256 and nq == 2 else None
     roof["traffic"] = tr[0] if tr else None
     roof["traffic_source"] = tr[1] if tr else None
     roof["kernel"] = dom
+    # the algorithmic register traffic (200 B per cell) against HBM
+    roof["hbm"] = {"achieved": BYTES_STAGE * ncell / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": BYTES_STAGE * ncell / t / 1e9 / PEAK_HBM_GBS,
+                   "traffic_GBps": (tr[0] / t / 1e9) if tr else None}
+    cn = COUNTERS.get(dom) if n == 256 and nq == 2 else None
+    if cn:
+        tf = cn["fp64_flop_per_cell"] * ncell / t / 1e12
+        roof["fp64_counters"] = {"tflops": tf, "frac": tf / PEAK_FP64_TFLOPS, "valu_busy": cn["valu_busy"],
+                                 "source": COUNTERS_SOURCE}
     # the whole path against its FP64 ceiling at the measured WENO fraction
     f_lin, f_weno = PATH_F[nq]
     f_alg = f_lin + head["weno_fraction"] * (f_weno - f_lin)

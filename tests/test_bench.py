@@ -34,10 +34,15 @@ def test_roofline_of_the_dominant_kernel():
     kern = {"linear": {"ms_total": 10.0, "ms_avg": 1.0},
             "flux_rk": {"ms_total": 20.0, "ms_avg": 2.0}}
     r = bench._roofline(kern, head, 256 ** 3, 256, 2)
-    assert r["kernel"] == "flux_rk" and r["bound"] == "hbm"
-    # 200 B per cell over 2 ms
-    assert abs(r["achieved"] - 200 * 256 ** 3 / 2e-3 / 1e9) < 1e-9
-    assert abs(r["frac"] - r["achieved"] / bench.PEAK_HBM_GBS) < 1e-15
+    # the flux kernel's roof is FP64 compute: 12 face solves of ~200 flops, the
+    # divergence and the register update per cell, over 2 ms
+    assert r["kernel"] == "flux_rk" and r["bound"] == "mfma" and r["unit"] == "TFLOP/s"
+    assert bench.flop_flux(2) == 12 * 200 + 12 * 5 * 3 + 40
+    assert abs(r["achieved"] - bench.flop_flux(2) * 256 ** 3 / 2e-3 / 1e12) < 1e-12
+    assert abs(r["frac"] - r["achieved"] / bench.PEAK_FP64_TFLOPS) < 1e-15
+    # beside it: 200 B per cell of register traffic over 2 ms, and the counters
+    assert abs(r["hbm"]["achieved"] - 200 * 256 ** 3 / 2e-3 / 1e9) < 1e-9
+    assert abs(r["fp64_counters"]["tflops"] - 8281.0 * 256 ** 3 / 2e-3 / 1e12) < 1e-9
     assert r["traffic"] == bench.TRAFFIC["flux_rk"][0] and r["traffic_source"].startswith("profiles/")
     lin, weno = bench.PATH_F[2]
     assert abs(r["path"]["flop_per_zone_update"] - (lin + 0.05 * (weno - lin))) < 1e-9
