@@ -225,6 +225,8 @@ hipError_t launch_stage_pre(int nv, const double *Uprev, const double *K, double
 hipError_t launch_finalize(const double *errPartial, const double *posPartial, int nPart,
                            double *out, hipStream_t s);
 hipError_t launch_fill(double *p, double v, size_t n, hipStream_t s);
+// Zero `bytes` (a multiple of 4) with a kernel, not a memset (graph capture).
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 
 // ---- learned troubled-cell indicator (Euler nv 5, GLM-MHD nv 9); P = network parameters
 int net_inputs(int dim);  // features per cell

@@ -205,6 +205,14 @@ hipError_t launch_nad(int dim, int nv, const double *Uold, const double *Unew, d
   KFVM_SELECT(dim, 2, L::nad(nv, Uold, Unew, fr, fv, nadFlag, count, g, s));
 }
 
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s) {
+  if (bytes % 4) return hipErrorInvalidValue;
+  const size_t n = bytes / 4;
+  const unsigned nb = unsigned(std::min<size_t>((n + dev::BLK - 1) / dev::BLK, 1024));
+  hipLaunchKernelGGL(dev::k_zero32, dim3(nb ? nb : 1), dim3(dev::BLK), 0, s, static_cast<unsigned *>(p), n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(double *p, double v, size_t n, hipStream_t s) {
   const unsigned nb = unsigned(std::min<size_t>((n + dev::BLK - 1) / dev::BLK, 4096));
   hipLaunchKernelGGL(dev::k_fill, dim3(nb ? nb : 1), dim3(dev::BLK), 0, s, p, v, n);

@@ -995,6 +995,12 @@ __global__ void k_fill(double *__restrict__ p, double v, size_t n) {
   for (size_t t = size_t(blockIdx.x) * BLK + threadIdx.x; t < n; t += size_t(gridDim.x) * BLK) p[t] = v;
 }
 
+// Zero n 32-bit words (counters, speeds): a kernel, so that steps captured
+// into HIP graphs hold no memset nodes.
+__global__ void k_zero32(unsigned *__restrict__ p, size_t n) {
+  for (size_t t = size_t(blockIdx.x) * BLK + threadIdx.x; t < n; t += size_t(gridDim.x) * BLK) p[t] = 0u;
+}
+
 #endif  // KFVM_SMALL_KERNELS
 
 }  // namespace dev

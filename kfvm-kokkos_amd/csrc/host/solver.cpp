@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -181,6 +182,8 @@ void Solver::setup() {
     sDtDev_ = dalloc<double>(size_t(slices_), "slice dt");
     sRed_ = dalloc<double>(2 * size_t(slices_), "slice reductions");
     sMask_ = dalloc<int>(size_t(slices_), "slice mask");
+    const char *eg = std::getenv("KFVM_GRAPH");
+    graphsOn_ = !(eg && std::string(eg) == "0");
     thBak_ = dalloc<double>(nc, "accepted slices' theta");
     wfBak_ = dalloc<double>(nc, "accepted slices' WENO flags");
     check(hipHostMalloc((void **)&sHost_, 4 * size_t(slices_) * sizeof(double), hipHostMallocDefault),
@@ -203,7 +206,7 @@ void Solver::setup() {
   check(launch_fill(theta_[0], 1.0, nc, stream_), "fill theta");
   check(launch_fill(theta_[1], 1.0, nc, stream_), "fill theta");
   check(launch_fill(wflag_, 0.0, nc, stream_), "fill wflag");
-  check(hipMemsetAsync(counter_, 0, sizeof(int), stream_), "clear counter");
+  check(launch_zero(counter_, sizeof(int), stream_), "clear counter");
 
   const bool restart = !cfg_.restartFile.empty();
   if (cfg_.plotFreq > 0.0 || cfg_.ckptFreq > 0 || restart) writer_.reset(new Writer(cfg_));
@@ -290,6 +293,70 @@ void Solver::sliceTimes(double *t, double *dtOut) const {
   }
 }
 
+// One step attempt of a batch as a HIP graph: the launches of an attempt
+// depend only on the buffer parities (cur_, th_), on whether the attempt
+// re-primes the FSAL register, on the WENO mode and on the selection (KXRCF
+// or the learned network), so one graph is captured
+// per such key on its first use and replayed afterwards, with the host-side
+// state (parities, mode, counters) advanced as the captured run advanced it.
+// KFVM_GRAPH=0, event timing or a failed capture run the attempt eagerly.
+template <class F>
+void Solver::runAttempt(bool prime, F &&body) {
+  if (!graphsOn_ || profOn_) {
+    body();
+    return;
+  }
+  const int key = (cur_ & 1) | ((th_ & 1) << 1) | (prime ? 4 : 0) | (useSparseWeno ? 8 : 0) |
+                  (learned_ && !forcePlain_ ? 16 : 0);
+  AttemptGraph &gr = graphs_[key];
+  if (!gr.exec) {
+    const int cur0 = cur_, th0 = th_;
+    const bool sparse0 = useSparseWeno, full0 = lastRhsFull_;
+    const int64_t nRhs0 = nRhsEval;
+    hipGraph_t g = nullptr;
+    bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      try {
+        body();
+      } catch (...) {
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      ok = hipStreamEndCapture(stream_, &g) == hipSuccess && g;
+    }
+    hipGraphExec_t ex = nullptr;
+    if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    // the captured run advanced the host state without running anything
+    gr.cur = cur_;
+    gr.th = th_;
+    gr.sparse = useSparseWeno;
+    gr.full = lastRhsFull_;
+    gr.dRhs = nRhsEval - nRhs0;
+    cur_ = cur0;
+    th_ = th0;
+    useSparseWeno = sparse0;
+    lastRhsFull_ = full0;
+    nRhsEval = nRhs0;
+    if (!ok) {  // run eagerly from now on
+      std::fprintf(stderr, "warning: HIP graph capture of a batch step failed; running the steps eagerly\n");
+      graphsOn_ = false;
+      body();
+      return;
+    }
+    nGraphs++;
+    gr.exec = ex;
+  }
+  check(hipGraphLaunch(gr.exec, stream_), "hipGraphLaunch");
+  cur_ = gr.cur;
+  th_ = gr.th;
+  useSparseWeno = gr.sparse;
+  lastRhsFull_ = gr.full;
+  nRhsEval += gr.dRhs;
+}
+
 // One lockstep step of every unfinished slice, each with its own step size
 // and controller (the reference's TakeStep, Solver.cpp:188-343, per slice):
 // the stages of all slices run in the same launches; slices whose attempt is
@@ -328,65 +395,73 @@ int Solver::takeStepSlices() {
     }
     check(hipMemcpyAsync(sDtDev_, hDt, size_t(S) * sizeof(double), hipMemcpyHostToDevice, stream_), "slice dt");
     check(hipMemcpyAsync(sMask_, sMaskHost_, size_t(S) * sizeof(int), hipMemcpyHostToDevice, stream_), "mask");
-    check(hipMemsetAsync(vmax_, 0, size_t(S) * sizeof(unsigned long long), stream_), "clear vmax");
-    if (nT > 0) {  // the retrying slices' FSAL register, from their step start state
+    if (nT > 0) {
       // Euler: a slice whose last attempt was unphysical gets WENO on every
-      // cell in this evaluation (Solver.cpp:298, 437-443) by a previous
-      // theta of 0 on all its cells, which the selectors flag; the other
-      // slices keep their own selection
+      // cell in this attempt's first evaluation (Solver.cpp:298, 437-443) by
+      // a previous theta of 0 on all its cells, which the selectors flag; the
+      // other slices keep their own selection
       for (int sl = 0; sl < S; sl++)
         if (forceFull[size_t(sl)]) {
           check(launch_fill(theta_[th_] + size_t(sl) * cps, 0.0, size_t(cps), stream_), "force full WENO");
           forceFull[size_t(sl)] = 0;
         }
-      RKArgs a{};
-      a.mode = RK_PRIME;
-      a.K = K_;
-      a.vmaxBits = vmax_;
-      a.sliceWrite = sMask_;
-      pipeline(Uprev_, a);
     }
-    timed("stage_pre", [&] {
-      return launch_stage_pre_slices(nv_, Uprev_, K_, U_[cur_], Uhat_, Utmp_, sDtDev_, rk_.beta[0], rk_.bhat[0],
-                                     int(ncell_), cps, stream_);
-    });
-    for (int st = 1; st < rk_.nStages; st++) {
-      RKArgs a{};
-      a.mode = RK_STAGE;
-      a.g1 = rk_.gam1[st];
-      a.g2 = rk_.gam2[st];
-      a.g3 = rk_.gam3[st];
-      a.delta = rk_.delta[st];
-      a.sliceDt = sDtDev_;
-      a.beta = rk_.beta[st];
-      a.bhat = rk_.bhat[st];
-      a.Uprev = Uprev_;
-      a.Uhat = Uhat_;
-      a.Utmp = Utmp_;
-      a.Unext = U_[cur_ ^ 1];
-      a.vmaxBits = vmax_;
-      pipeline(U_[cur_], a);
-      cur_ ^= 1;
-    }
-    {
-      RKArgs a{};
-      a.mode = RK_FSAL;
-      a.sliceDt = sDtDev_;
-      a.bhat = rk_.bhatfsal;
-      a.atol = cfg_.atol;
-      a.rtol = cfg_.rtol;
-      a.K = K_;
-      a.Uhat = Uhat_;
-      a.errPartial = errPart_;
-      a.posPartial = posPart_;
-      a.vmaxBits = vmax_;
-      a.sliceWrite = sMask_;
-      pipeline(U_[cur_], a);
-    }
-    check(launch_finalize_slices(errPart_, posPart_, blocksPerSlice, S, sRed_, stream_), "finalize");
-    check(hipMemcpyAsync(hRed, sRed_, 2 * size_t(S) * sizeof(double), hipMemcpyDeviceToHost, stream_), "read");
-    check(hipMemcpyAsync(hVm, vmax_, size_t(S) * sizeof(double), hipMemcpyDeviceToHost, stream_), "read vmax");
-    check(hipMemcpyAsync(&hostRed_[3], counter_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read count");
+    // the attempt's device work, from clearing the speed maxima to the host
+    // copies of the per-slice reductions
+    const bool prime = nT > 0;
+    auto attempt = [&] {
+      check(launch_zero(vmax_, size_t(S) * sizeof(unsigned long long), stream_), "clear vmax");
+      if (prime) {  // the retrying slices' FSAL register, from their step start state
+        RKArgs a{};
+        a.mode = RK_PRIME;
+        a.K = K_;
+        a.vmaxBits = vmax_;
+        a.sliceWrite = sMask_;
+        pipeline(Uprev_, a);
+      }
+      timed("stage_pre", [&] {
+        return launch_stage_pre_slices(nv_, Uprev_, K_, U_[cur_], Uhat_, Utmp_, sDtDev_, rk_.beta[0], rk_.bhat[0],
+                                       int(ncell_), cps, stream_);
+      });
+      for (int st = 1; st < rk_.nStages; st++) {
+        RKArgs a{};
+        a.mode = RK_STAGE;
+        a.g1 = rk_.gam1[st];
+        a.g2 = rk_.gam2[st];
+        a.g3 = rk_.gam3[st];
+        a.delta = rk_.delta[st];
+        a.sliceDt = sDtDev_;
+        a.beta = rk_.beta[st];
+        a.bhat = rk_.bhat[st];
+        a.Uprev = Uprev_;
+        a.Uhat = Uhat_;
+        a.Utmp = Utmp_;
+        a.Unext = U_[cur_ ^ 1];
+        a.vmaxBits = vmax_;
+        pipeline(U_[cur_], a);
+        cur_ ^= 1;
+      }
+      {
+        RKArgs a{};
+        a.mode = RK_FSAL;
+        a.sliceDt = sDtDev_;
+        a.bhat = rk_.bhatfsal;
+        a.atol = cfg_.atol;
+        a.rtol = cfg_.rtol;
+        a.K = K_;
+        a.Uhat = Uhat_;
+        a.errPartial = errPart_;
+        a.posPartial = posPart_;
+        a.vmaxBits = vmax_;
+        a.sliceWrite = sMask_;
+        pipeline(U_[cur_], a);
+      }
+      check(launch_finalize_slices(errPart_, posPart_, blocksPerSlice, S, sRed_, stream_), "finalize");
+      check(hipMemcpyAsync(hRed, sRed_, 2 * size_t(S) * sizeof(double), hipMemcpyDeviceToHost, stream_), "read");
+      check(hipMemcpyAsync(hVm, vmax_, size_t(S) * sizeof(double), hipMemcpyDeviceToHost, stream_), "read vmax");
+      check(hipMemcpyAsync(&hostRed_[3], counter_, sizeof(int), hipMemcpyDeviceToHost, stream_), "read count");
+    };
+    runAttempt(prime, attempt);
     check(hipStreamSynchronize(stream_), "sync");
     int cnt;
     std::memcpy(&cnt, &hostRed_[3], sizeof(int));
@@ -506,6 +581,9 @@ void Solver::release() noexcept {
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
                   faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_};
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto &kv : graphs_)
+    if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+  graphs_.clear();
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (hostRed_) (void)hipHostFree(hostRed_);
@@ -573,7 +651,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
   const double *thPrev = theta_[th_];
   double *thNew = theta_[th_ ^ 1];
   lastRhsFull_ = !useSparseWeno;
-  check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
+  check(launch_zero(counter_, 2 * sizeof(int), stream_), "clear counters");
   if (useSparseWeno) {
     const bool lrn = learned_ && !forcePlain_;
     timed("linear", [&] {
@@ -601,7 +679,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
     });
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
-  check(hipMemsetAsync(chBits_, 0, 2 * sizeof(unsigned long long), stream_), "clear speeds");
+  check(launch_zero(chBits_, 2 * sizeof(unsigned long long), stream_), "clear speeds");
   timed("pp", [&] { return launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thNew, grid_, chBits_, props_, stream_); });
   th_ ^= 1;
   timed("sources", [&] { return launch_mhd_sources(dim, nq, Uin, FS_, SRC_, thNew, grid_, diffM_, stream_); });
@@ -622,7 +700,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
   // counter_: [0] WENO cells, [1] cells of the linear pass for the limiter's
   // exact sequence (ppList_)
   if (hydro) {
-    check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
+    check(launch_zero(counter_, 2 * sizeof(int), stream_), "clear counters");
     timed("cell_props", [&] { return launch_cell_props(dim, Uin, props_, grid_, stream_); });
   }
   lastRhsFull_ = !useSparseWeno;
@@ -631,7 +709,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     // -> selection -> WENO and the limiter's exact sequence into rows ->
     // fused reconstruction + Riemann + divergence + RK update
     const bool lrn = learned_ && !forcePlain_;
-    check(hipMemsetAsync(counter_ + 2, 0, sizeof(int), stream_), "clear row count");
+    check(launch_zero(counter_ + 2, sizeof(int), stream_), "clear row count");
     timed("linear", [&] {
       return launch_linear_surf(dim, nq, Uin, lrn ? nullptr : WI_, thNew, props_, grid_, faceWT_, ppList_,
                                 counter_ + 1, SB_, sb_, stream_);
@@ -738,7 +816,7 @@ double Solver::readSpeed() {
 }
 
 double Solver::evalRHS(int which) {
-  check(hipMemsetAsync(vmax_, 0, size_t(slices_) * sizeof(unsigned long long), stream_), "clear vmax");
+  check(launch_zero(vmax_, size_t(slices_) * sizeof(unsigned long long), stream_), "clear vmax");
   RKArgs a{};
   a.mode = RK_PRIME;
   a.K = K_;
@@ -841,7 +919,7 @@ double Solver::evalRhsNoAlloc(const double *U, double *rhs) {
   chBits_ = b.chBits;
   vmax_ = b.vmax;
   profOn_ = false;  // event timing creates events
-  check(hipMemsetAsync(vmax_, 0, size_t(slices_) * sizeof(unsigned long long), stream_), "clear vmax");
+  check(launch_zero(vmax_, size_t(slices_) * sizeof(unsigned long long), stream_), "clear vmax");
   RKArgs a{};
   a.mode = RK_PRIME;
   a.K = rhs;
@@ -870,7 +948,7 @@ int Solver::takeStep() {
   stateInCur_ = false;
   stagePos_ = 0;
   for (int nT = 0; nT < cfg_.rejectionLimit; nT++) {
-    check(hipMemsetAsync(vmax_, 0, sizeof(unsigned long long), stream_), "clear vmax");
+    check(launch_zero(vmax_, sizeof(unsigned long long), stream_), "clear vmax");
     if (nT > 0) {
       RKArgs a{};
       a.mode = RK_PRIME;
@@ -977,7 +1055,7 @@ int Solver::takeStep() {
 }
 
 bool Solver::nadFails() {
-  check(hipMemsetAsync(nadCount_, 0, sizeof(int), stream_), "clear NAD count");
+  check(launch_zero(nadCount_, sizeof(int), stream_), "clear NAD count");
   timed("nad", [&] {
     return launch_nad(cfg_.dim, nv_, Uprev_, U_[cur_], nadRange_, nadValue_, nadFlag_, nadCount_, grid_, stream_);
   });
@@ -1029,7 +1107,7 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
     check(hipStreamSynchronize(stream_), "sync");
     return;
   }
-  check(hipMemsetAsync(counter_, 0, 2 * sizeof(int), stream_), "clear counters");
+  check(launch_zero(counter_, 2 * sizeof(int), stream_), "clear counters");
   check(launch_cell_props(dim, Uin, props_, grid_, stream_), "cell_props");
   check(launch_linear(dim, nq, nv_, Uin, FS_, WI_, thScratch, props_, grid_, faceW_, faceWT_, ppList_, counter_ + 1,
                       stream_),
@@ -1050,7 +1128,7 @@ void Solver::advanceStages(int64_t n, double dtFixed) {
   stateInCur_ = false;
   for (int64_t it = 0; it < n; it++) {
     if (stagePos_ == 0) {
-      check(hipMemsetAsync(vmax_, 0, sizeof(unsigned long long), stream_), "clear vmax");
+      check(launch_zero(vmax_, sizeof(unsigned long long), stream_), "clear vmax");
       timed("stage_pre", [&] {
         return launch_stage_pre(nv_, Uprev_, K_, U_[cur_], Uhat_, Utmp_, rk_.beta[0] * dtFixed,
                                 rk_.bhat[0] * dtFixed, int(ncell_), stream_);

@@ -90,6 +90,7 @@ class Solver {
     nadMaxFrac_ = maxFrac;
   }
   int64_t nNadCells = 0;  // violating cells summed over checked steps
+  int64_t nGraphs = 0;    // batch step attempts captured as HIP graphs
   bool learned() const { return learned_; }
   int64_t nNadFallback = 0;
   // Training samples of the current state: X[ncell][nin] features and Y[ncell]
@@ -190,6 +191,18 @@ class Solver {
   int *sMask_ = nullptr, *sMaskHost_ = nullptr;
   // theta and WENO flags of each slice's accepted evaluation in this step
   double *thBak_ = nullptr, *wfBak_ = nullptr;
+  // step attempts of a batch captured as HIP graphs, by key (buffer parities,
+  // re-prime, WENO mode), with the host state each replay leaves
+  struct AttemptGraph {
+    hipGraphExec_t exec = nullptr;
+    int cur = 0, th = 0;
+    bool sparse = true, full = false;
+    int64_t dRhs = 0;
+  };
+  std::map<int, AttemptGraph> graphs_;
+  bool graphsOn_ = false;
+  template <class F>
+  void runAttempt(bool prime, F &&body);
 
   std::unique_ptr<Writer> writer_;
   bool writePlot_ = false;  // the last step ended on a plot time

@@ -117,3 +117,43 @@ def test_batch_member_matches_the_oracle():
     r = cell_rel_diff(b.state[:, k:k + 1], o.state).max()
     print(f"[batch] member {k} vs oracle after 3 steps: max cell rel diff {r:.3e}", flush=True)
     assert r <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["fixed", "tolerance"])
+def test_graph_replayed_steps_equal_eager_steps(kind, monkeypatch, capfd):
+    """Step attempts of a batch captured once as HIP graphs and replayed
+    (KFVM_GRAPH=1, the default) against the same launches issued one by one
+    (KFVM_GRAPH=0): after every step, with the host's own reads and
+    synchronisations between the replays, states, flags and times are bit for
+    bit equal; the tolerance case replays the re-priming attempts of
+    rejected members too."""
+    import kfvm_amd
+    if kind == "fixed":
+        c = config(dim=2, n=(40, 36, 1), problem="quadrant2d", user=(), fixeddt=0.002)
+    else:
+        c = config(dim=2, n=(48, 48, 1), problem="quadrant2d", user=(), final=0.02, atol=1e-6, rtol=1e-6)
+    seeds = [2, 4, 6, 8]
+    monkeypatch.setenv("KFVM_GRAPH", "0")
+    eager = kfvm_amd.Solver(init_text=init_text(c), seeds=seeds)
+    monkeypatch.setenv("KFVM_GRAPH", "1")
+    graph = kfvm_amd.Solver(init_text=init_text(c), seeds=seeds)
+    steps = 0
+    for _ in range(12):
+        se, sg = eager.take_step(), graph.take_step()
+        assert se == sg
+        steps += 1
+        assert np.array_equal(graph.state, eager.state)
+        assert np.array_equal(graph.flags, eager.flags)
+        te, dte = eager.slice_times
+        tg, dtg = graph.slice_times
+        assert np.array_equal(te, tg) and np.array_equal(dte, dtg)
+        if graph.time[0] >= c["final"]:
+            break
+    ce, cg = eager.counters, graph.counters
+    print(f"[batch] graph replay ({kind}): {steps} steps equal, RHS evaluations {cg['rhs_evals']} / "
+          f"{ce['rhs_evals']}, rejections {cg['rejects_tol']}", flush=True)
+    assert ce == cg
+    if kind == "tolerance":
+        assert cg["rejects_tol"] > 0
+    # the graphs were captured (a failed capture warns and runs eagerly)
+    assert "graph capture" not in capfd.readouterr().err
