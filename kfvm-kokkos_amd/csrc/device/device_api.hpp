@@ -175,9 +175,14 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
 // KFVM-WENO reconstruction of the listed cells (list == nullptr: all).  Euler
 // leaves the face states in linearised primitive variables for
 // launch_weno_post; linear advection completes here.
-// FW, DW: reference weight layouts; FWS [nSub][C][F*Q], DWS [nSub][C][I]:
-// stencil-major copies (Euler).  WS (Euler, may be null): compact staging
+// FW, DW: reference weight layouts (linear advection).  Euler: FWS is the
+// compact table of the listed substencil rows only, [SUB][F*Q] face weights
+// then [SUB][I] derivative weights (SUB = C + Cc + B*Cb, substencils back to
+// back; compact_weno_weights), DWS is not read.  WS (Euler, may be null): compact staging
 // [(F*Q*5)][cap] for the first cap listed cells, shared with launch_weno_post.
+// Compact Euler WENO weight table (see launch_weno) from the reference layouts
+// face [nSub][F*Q][C] and deriv [nSub][I][C]; out has SUB*(F*Q + I) doubles.
+void compact_weno_weights(int dim, int fq, int nI, const double *face, const double *deriv, double *out);
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
                        const Grid &g, const double *FW, const double *DW, const double *FWS,
                        const double *DWS, const int *list, const int *counter, double *WS, int cap,

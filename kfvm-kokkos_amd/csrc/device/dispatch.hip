@@ -58,14 +58,28 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
   KFVM_SELECT(dim, nq, L::select(U, WI, theta, wflag, list, counter, wThresh, g, pos, cap, s));
 }
 
+void compact_weno_weights(int dim, int fq, int nI, const double *face, const double *deriv, double *out) {
+  const int C = dim == 3 ? SInfo<3>::C : SInfo<2>::C, Cc = dim == 3 ? SInfo<3>::Cc : SInfo<2>::Cc;
+  const int Cb = dim == 3 ? SInfo<3>::Cb : SInfo<2>::Cb, nSub = dim == 3 ? SInfo<3>::nSub : SInfo<2>::nSub;
+  const int SUB = C + Cc + (nSub - 2) * Cb;
+  int row = 0;
+  for (int sb = 0; sb < nSub; sb++) {
+    const int ns = sb == 0 ? C : (sb == 1 ? Cc : Cb);
+    for (int m = 0; m < ns; m++, row++) {
+      for (int p = 0; p < fq; p++) out[size_t(row) * fq + p] = face[(size_t(sb) * fq + p) * C + m];
+      for (int i = 0; i < nI; i++) out[size_t(SUB) * fq + size_t(row) * nI + i] = deriv[(size_t(sb) * nI + i) * C + m];
+    }
+  }
+}
+
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
                        const Grid &g, const double *FW, const double *DW, const double *FWS,
                        const double *DWS, const int *list, const int *counter, double *WS, int cap,
                        hipStream_t s) {
   if (nv != 5 && nv != 1) return hipErrorInvalidValue;
   if (WS && cap < 0) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, g, nv == 5 ? FWS : FW, nv == 5 ? DWS : DW, list, counter, WS,
-                               cap, s));
+  (void)DWS;
+  KFVM_SELECT(dim, nq, L::weno(nv, U, FS, thetaNew, g, FW, DW, list, counter, WS, cap, nv == 5 ? FWS : nullptr, s));
 }
 
 hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double *thetaNew,

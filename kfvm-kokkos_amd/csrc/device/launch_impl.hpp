@@ -43,14 +43,17 @@ struct Launch {
   }
   static hipError_t weno(int nv, const double *U, double *FS, double *thetaNew, const Grid &g,
                          const double *FW, const double *DW, const int *list, const int *counter,
-                         double *WS, int cap, hipStream_t s) {
-    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 1u << 16);
-    if (nv == 5)
-      hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, FW, DW, list,
-                         counter, WS, WS ? cap : 0);
-    else
+                         double *WS, int cap, const double *WC, hipStream_t s) {
+    if (nv == 5) {
+      // persistent: every workgroup stages the weights once, then strides over the list
+      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 2048u);
+      if (!WC) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list, counter,
+                         WS, WS ? cap : 0);
+    } else {
       hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
                          dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
+    }
     return hipGetLastError();
   }
   // ---- fused path (Euler, sparse WENO)

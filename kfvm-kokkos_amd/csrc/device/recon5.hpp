@@ -430,10 +430,27 @@ struct SubSten {
   }
 };
 
-// beta_S = sum_i (D_S V_S)_i^2 (findSmoothness, Stencil_K.H:506-554)
+// First row of substencil S in a weight table: the padded layout gives every
+// substencil C rows ([nSub][C][...]); the compact one stores only its n_S
+// rows, substencils back to back.
 template <int D, int S>
-__device__ __attribute__((always_inline)) inline double weno_beta(const double (&vs)[SInfo<D>::C], const cdouble *dt) {
-  constexpr int C = SInfo<D>::C, I = SInfo<D>::I, ns = SubSten<D, S>::n;
+struct SubOff {
+  static constexpr int value = SubOff<D, S - 1>::value + SubSten<D, S - 1>::n;
+};
+template <int D>
+struct SubOff<D, 0> {
+  static constexpr int value = 0;
+};
+template <int D, int S, bool COMPACT>
+struct WRow {
+  static constexpr int value = COMPACT ? SubOff<D, S>::value : S * SInfo<D>::C;
+};
+
+// beta_S = sum_i (D_S V_S)_i^2 (findSmoothness, Stencil_K.H:506-554); dt is a
+// scalar-cache (cdouble) or LDS table, padded or compact
+template <int D, int S, bool COMPACT = false, class WP = const cdouble *>
+__device__ __attribute__((always_inline)) inline double weno_beta(const double (&vs)[SInfo<D>::C], WP dt) {
+  constexpr int I = SInfo<D>::I, ns = SubSten<D, S>::n, r0 = WRow<D, S, COMPACT>::value;
   double acc[I];
 #pragma unroll
   for (int ii = 0; ii < I; ii++) acc[ii] = 0.0;
@@ -441,7 +458,7 @@ __device__ __attribute__((always_inline)) inline double weno_beta(const double (
   for (int m = 0; m < ns; m++) {
     const double x = vs[SubSten<D, S>::idx(m)];
 #pragma unroll
-    for (int ii = 0; ii < I; ii++) acc[ii] = fma(dt[(S * C + m) * I + ii], x, acc[ii]);
+    for (int ii = 0; ii < I; ii++) acc[ii] = fma(dt[(r0 + m) * I + ii], x, acc[ii]);
   }
   double beta = 0.0;
 #pragma unroll
@@ -451,10 +468,10 @@ __device__ __attribute__((always_inline)) inline double weno_beta(const double (
 
 // out[h] += omega_S * (W_S V_S)[h] over HF face points (fillRiemannStates,
 // Stencil_K.H:416-454)
-template <int D, int S, int HF, int FQ>
-__device__ __attribute__((always_inline)) inline void weno_faces(const double (&vs)[SInfo<D>::C], const cdouble *fc, double om,
-                                  double (&out)[HF]) {
-  constexpr int C = SInfo<D>::C, ns = SubSten<D, S>::n;
+template <int D, int S, int HF, int FQ, bool COMPACT = false, class WP = const cdouble *>
+__device__ __attribute__((always_inline)) inline void weno_faces(const double (&vs)[SInfo<D>::C], WP fc, double om,
+                                                               double (&out)[HF]) {
+  constexpr int ns = SubSten<D, S>::n, r0 = WRow<D, S, COMPACT>::value;
   double acc[HF];
 #pragma unroll
   for (int h = 0; h < HF; h++) acc[h] = 0.0;
@@ -462,34 +479,35 @@ __device__ __attribute__((always_inline)) inline void weno_faces(const double (&
   for (int m = 0; m < ns; m++) {
     const double x = vs[SubSten<D, S>::idx(m)];
 #pragma unroll
-    for (int h = 0; h < HF; h++) acc[h] = fma(fc[(S * C + m) * FQ + h], x, acc[h]);
+    for (int h = 0; h < HF; h++) acc[h] = fma(fc[(r0 + m) * FQ + h], x, acc[h]);
   }
 #pragma unroll
   for (int h = 0; h < HF; h++) out[h] = fma(om, acc[h], out[h]);
 }
 
-template <int D, int S, int N>
+template <int D, int S, int N, bool COMPACT = false>
 struct WenoSubs {
-  __device__ __attribute__((always_inline)) static inline void beta(const double (&vs)[SInfo<D>::C], const cdouble *dt,
-                                     double *om) {
-    om[S] = weno_beta<D, S>(vs, dt);
+  template <class WP>
+  __device__ __attribute__((always_inline)) static inline void beta(const double (&vs)[SInfo<D>::C], WP dt,
+                                                                    double *om) {
+    om[S] = weno_beta<D, S, COMPACT>(vs, dt);
     __builtin_amdgcn_sched_barrier(0);
-    WenoSubs<D, S + 1, N>::beta(vs, dt, om);
+    WenoSubs<D, S + 1, N, COMPACT>::beta(vs, dt, om);
   }
-  template <int HF, int FQ>
-  __device__ __attribute__((always_inline)) static inline void faces(const double (&vs)[SInfo<D>::C], const cdouble *fc,
-                                      const double *om, double (&out)[HF]) {
-    weno_faces<D, S, HF, FQ>(vs, fc, om[S], out);
+  template <int HF, int FQ, class WP>
+  __device__ __attribute__((always_inline)) static inline void faces(const double (&vs)[SInfo<D>::C], WP fc,
+                                                                     const double *om, double (&out)[HF]) {
+    weno_faces<D, S, HF, FQ, COMPACT>(vs, fc, om[S], out);
     __builtin_amdgcn_sched_barrier(0);
-    WenoSubs<D, S + 1, N>::template faces<HF, FQ>(vs, fc, om, out);
+    WenoSubs<D, S + 1, N, COMPACT>::template faces<HF, FQ>(vs, fc, om, out);
   }
 };
-template <int D, int N>
-struct WenoSubs<D, N, N> {
-  __device__ static inline void beta(const double (&)[SInfo<D>::C], const cdouble *, double *) {}
-  template <int HF, int FQ>
-  __device__ static inline void faces(const double (&)[SInfo<D>::C], const cdouble *,
-                                      const double *, double (&)[HF]) {}
+template <int D, int N, bool COMPACT>
+struct WenoSubs<D, N, N, COMPACT> {
+  template <class WP>
+  __device__ static inline void beta(const double (&)[SInfo<D>::C], WP, double *) {}
+  template <int HF, int FQ, class WP>
+  __device__ static inline void faces(const double (&)[SInfo<D>::C], WP, const double *, double (&)[HF]) {}
 };
 
 template <int D, int NQ>
@@ -501,11 +519,24 @@ struct WenoChunk {  // face points per accumulation chunk
 // Listed cells at list positions t < cap write their states compactly to
 // WS[t][point * 5 + var] (whole cache lines, read back coalesced by
 // k_weno_post); the others, and the unlisted full pass, write FS in place.
+//
+// The weights (compact: WC = face rows [SUB][F*Q], then derivative rows
+// [SUB][I], SUB = C + Cc + B*Cb) are staged in LDS once per workgroup and read
+// from there: the scalar cache (16 KB) cannot hold them (36 KB in 3D at NQ=2),
+// so scalar loads of them kept missing to L2 with the next FMAs waiting on
+// them.  The grid is persistent (at most wenoBlocks workgroups striding over
+// the list), so the staging is paid once per workgroup.
+template <int D, int NQ>
+struct WenoLds {
+  static constexpr int SUB = SInfo<D>::C + SInfo<D>::Cc + (SInfo<D>::nSub - 2) * SInfo<D>::Cb;
+  static constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  static constexpr int NFACE = SUB * FQ, NW = SUB * (FQ + SInfo<D>::I);
+};
+
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
                                                   double *__restrict__ FS, Grid g,
-                                                  const double *__restrict__ FT,
-                                                  const double *__restrict__ DT,
+                                                  const double *__restrict__ WC,
                                                   const int *__restrict__ list,
                                                   const int *__restrict__ counter,
                                                   double *__restrict__ WS, int cap) {
@@ -515,8 +546,13 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
   constexpr int FQ = WenoChunk<D, NQ>::FQ, HF = WenoChunk<D, NQ>::H;
   constexpr double gHi = 0.7, gLo = 0.7;
   constexpr double g0 = gHi, g1 = (1.0 - gHi) * gLo, gb = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
+  using WL = WenoLds<D, NQ>;
+  static_assert(WL::SUB == SubOff<D, nSub>::value, "compact rows of all substencils");
+  __shared__ double wl[WL::NW];
+  for (int e = threadIdx.x; e < WL::NW; e += BLK) wl[e] = WC[e];
+  __syncthreads();
   const Group G;
-  if (G.slot >= CPW) return;
+  if (G.slot >= CPW) return;  // no barrier follows
   const int n = list ? *counter : g.ncell;
   const int waveId = blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
   const int nWaves = gridDim.x * (BLK / 64);
@@ -552,13 +588,10 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
         vs[m] = G.v == 0 ? s0 : (sv - uv * s0) / lp.rho;
       }
     }
-    // weight tables opaque to loop-invariant code motion (see k_linear_fused)
-    const double *dt0 = DT, *ft0 = FT;
-    asm volatile("" : "+s"(dt0), "+s"(ft0));
-    const cdouble *dt = (const cdouble *)dt0;
     // smoothness indicators (findSmoothness)
+    const double *dt = wl + WL::NFACE;
     double om[nSub];
-    WenoSubs<D, 0, nSub>::beta(vs, dt, om);
+    WenoSubs<D, 0, nSub, true>::beta(vs, dt, om);
     // createOmegas
     double tot = 0.0;
 #pragma unroll
@@ -575,13 +608,11 @@ __global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
     // fillRiemannStates, HF face points at a time
 #pragma unroll 1
     for (int h0 = 0; h0 < FQ; h0 += HF) {
-      const double *fc0 = ft0 + h0;
-      asm volatile("" : "+s"(fc0));
-      const cdouble *fc = (const cdouble *)fc0;
+      const double *fc = wl + h0;
       double out[HF];
 #pragma unroll
       for (int h = 0; h < HF; h++) out[h] = 0.0;
-      WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, fc, om, out);
+      WenoSubs<D, 0, nSub, true>::template faces<HF, FQ>(vs, fc, om, out);
 #pragma unroll
       for (int h = 0; h < HF; h++) P[(size_t(h0 + h) * 5 + G.v) * ps] = out[h];
     }

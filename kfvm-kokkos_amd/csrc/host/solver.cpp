@@ -173,6 +173,11 @@ void Solver::setup() {
   faceWT_ = dalloc<double>(size_t(wts_.F) * wts_.Q * wts_.C, "face weights (pass-major)");
   faceWS_ = dalloc<double>(wts_.face.size(), "face weights (stencil-major)");
   derivWS_ = dalloc<double>(wts_.deriv.size(), "deriv weights (stencil-major)");
+  if (nv_ == 5) {
+    const int SUB = cfg_.dim == 3 ? SInfo<3>::C + SInfo<3>::Cc + SInfo<3>::B * SInfo<3>::Cb
+                                  : SInfo<2>::C + SInfo<2>::Cc + SInfo<2>::B * SInfo<2>::Cb;
+    faceWC_ = dalloc<double>(size_t(SUB) * (FQ + wts_.I), "WENO weights (compact)");
+  }
   nPart_ = flux_rk_partials(cfg_.dim, grid_);
   errPart_ = dalloc<double>(nPart_, "err partials");
   posPart_ = dalloc<double>(nPart_, "pos partials");
@@ -579,7 +584,7 @@ void Solver::release() noexcept {
   } catch (...) {
   }
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_, faceWC_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto &kv : graphs_)
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -727,7 +732,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
       });
     }
     timed("weno", [&] {
-      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, list_, counter_,
+      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWC_, derivWS_, list_, counter_,
                          FS_, wsCap_, stream_);
     });
     timed("weno_post", [&] {
@@ -775,7 +780,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
         });
       }
       timed("weno", [&] {
-        return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, list_,
+        return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWC_, derivWS_, list_,
                            counter_, WS_, wsCap_, stream_);
       });
       timed("weno_post", [&] {
@@ -787,7 +792,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     }
   } else {
     timed("weno", [&] {
-      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr,
+      return launch_weno(dim, nq, nv_, Uin, FS_, thNew, grid_, faceW_, derivW_, faceWC_, derivWS_, nullptr,
                          nullptr, nullptr, 0, stream_);
     });
     if (hydro)
@@ -1113,7 +1118,7 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
                       stream_),
         "linear");
   check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
-  check(launch_weno(dim, nq, nv_, Uin, FS_, thScratch, grid_, faceW_, derivW_, faceWS_, derivWS_, nullptr, nullptr,
+  check(launch_weno(dim, nq, nv_, Uin, FS_, thScratch, grid_, faceW_, derivW_, faceWC_, derivWS_, nullptr, nullptr,
                     nullptr, 0, stream_),
         "weno");
   check(launch_weno_post(dim, nq, Uin, FS_, thScratch, props_, grid_, nullptr, nullptr, nullptr, 0, stream_),
@@ -1251,6 +1256,13 @@ void Solver::uploadWeights() {
   check(hipMemcpy(faceWT_, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(faceWS_, fs.data(), fs.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   check(hipMemcpy(derivWS_, ds.data(), ds.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  if (faceWC_) {
+    const int SUB = wts_.dim == 3 ? SInfo<3>::C + SInfo<3>::Cc + SInfo<3>::B * SInfo<3>::Cb
+                                  : SInfo<2>::C + SInfo<2>::Cc + SInfo<2>::B * SInfo<2>::Cb;
+    std::vector<double> wc(size_t(SUB) * (FQ + I));
+    compact_weno_weights(wts_.dim, FQ, I, wts_.face.data(), wts_.deriv.data(), wc.data());
+    check(hipMemcpy(faceWC_, wc.data(), wc.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  }
   if (faceWQ_) {  // full-stencil face weights per point: [Q][C][F]
     const int Q = wts_.Q, F = wts_.F;
     std::vector<double> wq(size_t(Q) * C * F);

@@ -164,6 +164,7 @@ class Solver {
   double *SRC_ = nullptr, *cellW_ = nullptr, *cellWS_ = nullptr, *diffM_ = nullptr;
   unsigned long long *chBits_ = nullptr;
   double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr, *faceWS_ = nullptr, *derivWS_ = nullptr;
+  double *faceWC_ = nullptr;  // Euler WENO: compact face + derivative weights (compact_weno_weights)
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;
   unsigned long long *vmax_ = nullptr;
   double *hostRed_ = nullptr;  // pinned: [err, pos, vmax bits, counter]
