@@ -193,25 +193,37 @@ hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double
                             const double *props, const Grid &g, const int *list, const int *counter,
                             double *WS, int cap, hipStream_t s);
 // GLM-MHD (9 variables).  SRC [(p * 9 + v)][ncell]: states, then source
-// terms, at the NQ^D interior points; CW: interior weights [nSub][NQ^D][C];
-// Dm: differentiation matrix [NQ][NQ + 2]; chBits: the two maxima of the
-// cleaning speed over the limited face states (zeroed by the caller before
-// launch_mhd_pp, which limits the states and then takes the maxima).
+// terms, at the NQ^D interior points; props [2][ncell]: per-cell internal
+// energy and sound speed (launch_mhd_props, before the linear pass or WENO);
+// spd [2][ncell]: per-cell maxima of c_f + |u| and |u| over the cell's limited
+// face states, which launch_mhd_sources folds into chBits (zeroed by the
+// caller) for the flux kernel's cleaning speed.
+hipError_t launch_mhd_props(int dim, const double *U, double *props, const Grid &g, hipStream_t s);
+// WT: full-stencil weights per pass, the face passes [F*Q/P][C][P]
+// (P = linear_pass_points) then the interior passes [NQ^D/PI][C][PI]
+// (PI = mhd_interior_pass_points).  Cells whose states violate a limiter
+// bound are appended to ppList (count in *ppCount, zeroed by the caller) for
+// launch_mhd_pp; the others get theta = 1.  WI may be null (learned selection).
 hipError_t launch_mhd_linear(int dim, int nq, const double *U, double *FS, double *SRC, double *WI,
-                             const Grid &g, const double *W0, const double *CW, hipStream_t s);
+                             double *thetaNew, double *spd, const double *props, const Grid &g, const double *WT,
+                             int *ppList, int *ppCount, hipStream_t s);
 hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                              double *wflag, int *list, int *counter, double wThresh, const Grid &g,
                              hipStream_t s);
 // FT, DT, CT: stencil-major face [nSub][C][F*Q], smoothness [nSub][C][I] and
-// interior [nSub][C][NQ^D] weights.
+// interior [nSub][C][NQ^D] weights.  The listed cells (list null: every cell)
+// are converted back and limited (theta, spd).
 hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
                            const double *FT, const double *DT, const double *CT, const int *list,
-                           const int *counter, hipStream_t s);
-// props: [2][ncell] scratch for the per-cell internal energy and sound speed
+                           const int *counter, double *thetaNew, double *spd, const double *props,
+                           hipStream_t s);
+// The limiter's exact sequence on the listed cells whose WENO flag is 0.
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
-                         const Grid &g, unsigned long long *chBits, double *props, hipStream_t s);
+                         double *spd, const double *props, const double *wflag, const Grid &g, const int *list,
+                         const int *count, hipStream_t s);
 hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
-                              const double *theta, const Grid &g, const double *Dm, hipStream_t s);
+                              const double *theta, const Grid &g, const double *Dm, const double *spd,
+                              unsigned long long *chBits, hipStream_t s);
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
                           const Grid &g, const RKArgs &a, hipStream_t s);
 int flux_rk_partials(int dim, const Grid &g);

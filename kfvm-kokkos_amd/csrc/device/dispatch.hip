@@ -89,9 +89,13 @@ hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double
   KFVM_SELECT(dim, nq, L::weno_post(U, FS, thetaNew, props, g, list, counter, WS, cap, s));
 }
 
+hipError_t launch_mhd_props(int dim, const double *U, double *props, const Grid &g, hipStream_t s) {
+  KFVM_SELECT(dim, 2, L::mhd_props(U, props, g, s));
+}
 hipError_t launch_mhd_linear(int dim, int nq, const double *U, double *FS, double *SRC, double *WI,
-                             const Grid &g, const double *W0, const double *CW, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_linear(U, FS, SRC, WI, g, W0, CW, s));
+                             double *thetaNew, double *spd, const double *props, const Grid &g, const double *WT,
+                             int *ppList, int *ppCount, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_linear(U, FS, SRC, WI, thetaNew, spd, props, g, WT, ppList, ppCount, s));
 }
 hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI, const double *theta,
                              double *wflag, int *list, int *counter, double wThresh, const Grid &g,
@@ -100,16 +104,19 @@ hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI,
 }
 hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
                            const double *FT, const double *DT, const double *CT, const int *list,
-                           const int *counter, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FT, DT, CT, list, counter, s));
+                           const int *counter, double *thetaNew, double *spd, const double *props,
+                           hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FT, DT, CT, list, counter, thetaNew, spd, props, s));
 }
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
-                         const Grid &g, unsigned long long *chBits, double *props, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_pp(U, FS, SRC, thetaNew, g, chBits, props, s));
+                         double *spd, const double *props, const double *wflag, const Grid &g, const int *list,
+                         const int *count, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_pp(U, FS, SRC, thetaNew, spd, props, wflag, g, list, count, s));
 }
 hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
-                              const double *theta, const Grid &g, const double *Dm, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_sources(U, FS, SRC, theta, g, Dm, s));
+                              const double *theta, const Grid &g, const double *Dm, const double *spd,
+                              unsigned long long *chBits, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_sources(U, FS, SRC, theta, g, Dm, spd, chBits, s));
 }
 
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,

@@ -98,11 +98,21 @@ struct Launch {
     return hipGetLastError();
   }
   // ---- GLM-MHD
-  static hipError_t mhd_linear(const double *U, double *FS, double *SRC, double *WI, const Grid &g,
-                               const double *W0, const double *CW, hipStream_t s) {
+  static unsigned mhd_bricks(const Grid &g) {
+    using Bx = dev::MBox<D>;
+    return unsigned(((g.nx + Bx::BX - 1) / Bx::BX) * ((g.ny + Bx::BY - 1) / Bx::BY) *
+                    ((g.nz + Bx::BZ - 1) / Bx::BZ));
+  }
+  static hipError_t mhd_props(const double *U, double *props, const Grid &g, hipStream_t s) {
+    hipLaunchKernelGGL((dev::k_mhd_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
+    return hipGetLastError();
+  }
+  static hipError_t mhd_linear(const double *U, double *FS, double *SRC, double *WI, double *thetaNew, double *spd,
+                               const double *props, const Grid &g, const double *WT, int *ppList, int *ppCount,
+                               hipStream_t s) {
     // WI null: the learned selection does not read the KXRCF indicator
-    hipLaunchKernelGGL((dev::k_mhd_linear<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
-                       g, W0, CW, WI);
+    hipLaunchKernelGGL((dev::k_mhd_linear<D, NQ>), dim3(mhd_bricks(g)), dim3(dev::MBox<D>::NT), 0, s, U, FS, SRC,
+                       WI, thetaNew, spd, props, g, WT, ppList, ppCount);
     return hipGetLastError();
   }
   static hipError_t mhd_select(const double *U, const double *WI, const double *theta, double *wflag,
@@ -113,26 +123,28 @@ struct Launch {
   }
   static hipError_t mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FT,
                              const double *DT, const double *CT, const int *list, const int *counter,
-                             hipStream_t s) {
+                             double *thetaNew, double *spd, const double *props, hipStream_t s) {
     const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::MCPB - 1) / dev::MCPB), 1u << 16);
     hipLaunchKernelGGL((dev::k_mhd_wenorec<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, SRC, g, FT, DT, CT,
                        list, counter);
     const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
     hipLaunchKernelGGL((dev::k_mhd_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, g, list,
-                       counter);
+                       counter, thetaNew, spd, props);
     return hipGetLastError();
   }
-  static hipError_t mhd_pp(const double *U, double *FS, double *SRC, double *thetaNew, const Grid &g,
-                           unsigned long long *chBits, double *props, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_mhd_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
-    hipLaunchKernelGGL((dev::k_mhd_pp<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
-                       thetaNew, g, chBits, props);
+  static hipError_t mhd_pp(const double *U, double *FS, double *SRC, double *thetaNew, double *spd,
+                           const double *props, const double *wflag, const Grid &g, const int *list,
+                           const int *count, hipStream_t s) {
+    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1024u);
+    hipLaunchKernelGGL((dev::k_mhd_pp<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, thetaNew, spd, props,
+                       wflag, g, list, count);
     return hipGetLastError();
   }
   static hipError_t mhd_sources(const double *U, const double *FS, double *SRC, const double *theta,
-                                const Grid &g, const double *Dm, hipStream_t s) {
+                                const Grid &g, const double *Dm, const double *spd, unsigned long long *chBits,
+                                hipStream_t s) {
     hipLaunchKernelGGL((dev::k_mhd_sources<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
-                       theta, g, Dm);
+                       theta, g, Dm, spd, chBits);
     return hipGetLastError();
   }
   // ---- learned troubled-cell indicator

@@ -1,22 +1,24 @@
 // Stage kernels of the GLM-MHD equations (9 variables: rho, mx, my, mz, E,
-// Bx, By, Bz, psi).  One thread per cell (the WENO pass: one lane per cell
-// and variable); the face states (FS) and the interior quadrature-point
+// Bx, By, Bz, psi).  The face states (FS) and the interior quadrature-point
 // states (SRC, needed by the GLM source terms) are planes
-// [(point * 9 + var)][cell].
+// [(point * 9 + var)][cell] (FS in blocks of FSB cells).
 //
 //   k_mhd_linear     KernelLinearRecon_K incl. the source branch
-//                    (Stencil_K.H:32-118), then p/rho^gamma of every
-//                    unlimited face state (the KXRCF input,
-//                    Numeric_K.H:315-379)
-//   k_mhd_wenorec    KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504)
-//   k_mhd_weno_post  with VarConvert<MHD_GLM> (MHD_GLM_K.H:24-69)
+//                    (Stencil_K.H:32-118) on a staged brick, with the KXRCF
+//                    input p/rho^gamma of every face state (Numeric_K.H:315-379),
+//                    the limiter's fast test and the cleaning-speed maxima
+//   k_mhd_wenorec    KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504),
+//                    one lane per (cell, variable)
+//   k_mhd_weno_post  VarConvert<MHD_GLM> back-conversion (MHD_GLM_K.H:24-69),
+//                    then the limiter of the WENO cells
 //   k_mhd_pp         PositivityPreserve_K with interior points
-//                    (Common_K.H:299-643), then SpeedEstimate_K
-//                    (MHD_GLM_K.H:213-310) on the limited face states: the
-//                    cleaning speed c_h = max(c_f + |u|) - max(|u|),
-//                    x-components on every face
+//                    (Common_K.H:299-643) for the cells that failed the fast
+//                    test, then SpeedEstimate_K (MHD_GLM_K.H:213-310) on the
+//                    limited face states
 //   k_mhd_sources    SourceTerms_K (MHD_GLM_K.H:316-547): div B and psi-gradient
-//                    terms at the interior points
+//                    terms at the interior points; the cleaning speed
+//                    c_h = max(c_f + |u|) - max(|u|) over all face states,
+//                    x-components on every face
 #pragma once
 
 namespace kfvm {
@@ -39,52 +41,197 @@ struct MhdLin {
   }
 };
 
-// Linear face and interior states of every cell; with WI, then the KXRCF
-// indicator p/rho^gamma of each face state, read back from the lines this
-// thread has just written (still in cache).
-template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_linear(const double *__restrict__ U,
-                                                    double *__restrict__ FS,
-                                                    double *__restrict__ SRC, Grid g,
-                                                    const double *__restrict__ W,
-                                                    const double *__restrict__ CW,
-                                                    double *__restrict__ WI) {
+// ------------------------------------------------------------ linear pass
+// Brick of the linear pass: the nine variables of 16x8x4 cells (3D) and their
+// radius-2 halo in LDS (138 KB: one workgroup of 8 waves per CU).
+template <int D>
+struct MBox {
+  static constexpr int H = 2;
+  static constexpr int BX = 16, BY = D == 3 ? 8 : 16, BZ = D == 3 ? 4 : 1;
+  static constexpr int LX = BX + 2 * H, LY = BY + 2 * H, LZ = D == 3 ? BZ + 2 * H : 1;
+  static constexpr int N = LX * LY * LZ;   // staged cells per variable
+  static constexpr int NT = BX * BY * BZ;  // threads per block
+};
+
+// PP points of all nine variables from the staged brick.  The stencil walk is
+// outermost, so each accumulator sums n = 0..C-1 in order (the reference's
+// dot product), and the PP weights of one stencil cell -- one scalar load --
+// feed 9 PP FMAs.  The next stencil cell's weights and LDS values are fetched
+// before this cell's FMAs issue; their addresses pass through an empty asm
+// that takes this cell's operands as inputs, so the wait for this cell's
+// loads is not merged with the next cell's (one counter serves scalar and LDS
+// loads, and a wait with a scalar load outstanding is a full one).
+template <int D, class Bx, int PP>
+__device__ __forceinline__ void mhd_pass(const double *__restrict__ lds, int corner, const double *wbase,
+                                         double (&acc)[9][PP]) {
   constexpr auto ST = make_stencil_tables<D>();
-  constexpr int C = SInfo<D>::C, FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
-  int i, j, k;
-  cell_ijk<D>(c, g, i, j, k);
-  const size_t nc = g.ncell;
-#pragma unroll 1
-  for (int v = 0; v < 9; v++) {
-    double vs[C];
+  constexpr int C = SInfo<D>::C, H = Bx::H, NP = Bx::N;
 #pragma unroll
-    for (int n = 0; n < C; n++)
-      vs[n] = load_u<D, 9>(U, g, i + ST.off[n][0], j + ST.off[n][1], k + ST.off[n][2], v);
-#pragma unroll 2
-    for (int fq = 0; fq < FQ; fq++) {
-      double acc = 0.0;
+  for (int v = 0; v < 9; v++)
 #pragma unroll
-      for (int n = 0; n < C; n++) acc = fma(W[fq * C + n], vs[n], acc);
-      FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB] = acc;
+    for (int q = 0; q < PP; q++) acc[v][q] = 0.0;
+  double w[PP], x[9];
+  {
+    const double *w0 = wbase;
+    asm volatile("" : "+s"(w0));
+    const cdouble *wc = (const cdouble *)w0;
+#pragma unroll
+    for (int q = 0; q < PP; q++) w[q] = wc[q];
+    constexpr int o = ST.off[0][0] + H + Bx::LX * (ST.off[0][1] + H) +
+                      (D == 3 ? Bx::LX * Bx::LY * (ST.off[0][2] + H) : 0);
+#pragma unroll
+    for (int v = 0; v < 9; v++) x[v] = lds[v * NP + corner + o];
+  }
+#pragma unroll
+  for (int n = 0; n < C; n++) {
+    double wn[PP], xn[9];
+    if (n + 1 < C) {
+      const double *w1 = wbase + (n + 1) * PP;
+      int cn = corner;
+      asm volatile("" : "+s"(w1), "+v"(cn) : "s"(w[0]), "s"(w[PP - 1]), "v"(x[0]), "v"(x[1]), "v"(x[2]),
+                   "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]));
+      const cdouble *wc = (const cdouble *)w1;
+#pragma unroll
+      for (int q = 0; q < PP; q++) wn[q] = wc[q];
+      const int o = (ST.off[n + 1][0] + H) + Bx::LX * (ST.off[n + 1][1] + H) +
+                    (D == 3 ? Bx::LX * Bx::LY * (ST.off[n + 1][2] + H) : 0);
+#pragma unroll
+      for (int v = 0; v < 9; v++) xn[v] = lds[v * NP + cn + o];
     }
-#pragma unroll 2
-    for (int p = 0; p < NI; p++) {
-      double acc = 0.0;
 #pragma unroll
-      for (int n = 0; n < C; n++) acc = fma(CW[p * C + n], vs[n], acc);
-      SRC[(size_t(p) * 9 + v) * nc + c] = acc;
+    for (int q = 0; q < PP; q++)
+#pragma unroll
+      for (int v = 0; v < 9; v++) acc[v][q] = fma(w[q], x[v], acc[v][q]);
+    if (n + 1 < C) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 9 * PP, 0);
+#pragma unroll
+      for (int q = 0; q < PP; q++) w[q] = wn[q];
+#pragma unroll
+      for (int v = 0; v < 9; v++) x[v] = xn[v];
     }
   }
-  if (WI) {
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Fastest magnetosonic speed + |u| and |u| of one face state, x-components on
+// every face (SpeedEstimate_K, MHD_GLM_K.H:213-310).
+__device__ inline void mhd_face_speed(double gamma, const double *s, double &maxLam, double &maxVel) {
+  const double vel = fabs(s[1] / s[0]);
+  const double lam = Mhd::fast(gamma, s, s[5]) + vel;
+  maxVel = vel > maxVel ? vel : maxVel;
+  maxLam = lam > maxLam ? lam : maxLam;
+}
+
+// Linear face and interior states (KernelLinearRecon_K incl. the source
+// branch, Stencil_K.H:32-118), one thread per cell of a staged brick, in
+// passes of P face points / PI interior points of all nine variables.  While
+// a pass' states are in registers the kernel also forms
+//   - the KXRCF indicator p/rho^gamma of each face state (WI, when given;
+//     Numeric_K.H:315-379),
+//   - the limiter's fast test: every density inside the widened bounds and
+//     every internal energy above the bound (Common_K.H:299-643) -- a cell
+//     that passes keeps its states and theta = 1, one that fails goes to
+//     ppList for the exact sequence (k_mhd_pp),
+//   - the cleaning-speed maxima of the face states (SpeedEstimate_K) into the
+//     per-cell plane spd [maxLam | maxVel]; cells that the limiter or WENO
+//     change later overwrite theirs.
+template <int D, int NQ>
+__global__ __launch_bounds__(MBox<D>::NT, 2) void k_mhd_linear(
+    const double *__restrict__ U, double *__restrict__ FS, double *__restrict__ SRC, double *__restrict__ WI,
+    double *__restrict__ thetaNew, double *__restrict__ spd, const double *__restrict__ props, Grid g,
+    const double *__restrict__ WT, int *__restrict__ ppList, int *__restrict__ ppCount) {
+  using Bx = MBox<D>;
+  constexpr int C = SInfo<D>::C, H = Bx::H, NP = Bx::N;
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+  constexpr int P = linear_pass_points(D, NQ), PI = mhd_interior_pass_points(D, NQ);
+  static_assert(FQ % P == 0 && NI % PI == 0, "passes must tile the points");
+  __shared__ double lds[9 * NP];
+  const int nbx = (g.nx + Bx::BX - 1) / Bx::BX, nby = (g.ny + Bx::BY - 1) / Bx::BY;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (b % nbx) * Bx::BX, j0 = ((b / nbx) % nby) * Bx::BY, k0 = (b / (nbx * nby)) * Bx::BZ;
+  const int tid = threadIdx.x;
+  const size_t nc = g.ncell;
+  for (int idx = tid; idx < NP; idx += Bx::NT) {
+    const int a = idx % Bx::LX, r = idx / Bx::LX, bb = r % Bx::LY, c = r / Bx::LY;
+    const int ex = box_out<Bx::BX, H>(a), ey = box_out<Bx::BY, H>(bb);
+    const int ez = D == 3 ? box_out<Bx::BZ, H>(c) : 0;
+    const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + c - H : k0;
+    // cells no stencil of a live cell reads (box corners, beyond the halo)
+    const bool need = ex * ex + ey * ey + ez * ez <= H * H && gi < g.nx + H && gj < g.ny + H &&
+                      (D == 2 || gk < g.nz + H);
+    if (need) {
+      const CellRef cr = map_cell<D, 9>(g, gi, gj, gk);
+#pragma unroll
+      for (int w = 0; w < 9; w++) lds[w * NP + idx] = ld(U, nc, cr, w);
+    }
+  }
+  __syncthreads();
+  const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
+  const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
+  if (!(i < g.nx && j < g.ny && (D == 2 || k < g.nz))) return;  // no barrier below
+  const int cell = (k * g.ny + j) * g.nx + i;
+  const int corner = (tz * Bx::LY + ty) * Bx::LX + tx;  // stencil box corner (-H,-H,-H)
+  const int c0 = corner + H + Bx::LX * H + (D == 3 ? Bx::LX * Bx::LY * H : 0);
+  double ref[9];
+#pragma unroll
+  for (int w = 0; w < 9; w++) ref[w] = lds[w * NP + c0];
+  const PPBounds pb = pp_bounds_lds<D, Bx>(lds, c0, props, g, i, j, k, ref);
+  double thD = 1.0, maxLam = 0.0, maxVel = 0.0;
+  bool okE = true;
+  const size_t fb = fs_base(cell, FQ * 9);
 #pragma unroll 1
-    for (int fq = 0; fq < FQ; fq++) {
+  for (int ps = 0; ps < FQ / P; ps++) {
+    double acc[9][P];
+    mhd_pass<D, Bx, P>(lds, corner, WT + size_t(ps) * C * P, acc);
+#pragma unroll
+    for (int q = 0; q < P; q++)
+#pragma unroll
+      for (int v = 0; v < 9; v++) FS[fb + (size_t(ps * P + q) * 9 + v) * FSB] = acc[v][q];
+#pragma unroll
+    for (int q = 0; q < P; q++) {
       double s[9];
 #pragma unroll
-      for (int v = 0; v < 9; v++) s[v] = FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB];
-      WI[fs_base(c, FQ) + size_t(fq) * FSB] = Mhd::weno(g.gamma, s);
+      for (int v = 0; v < 9; v++) s[v] = acc[v][q];
+      thD = dens_theta(thD, s[0], pb);
+      okE = okE && Mhd::eint(s) > pb.minEint;
+      mhd_face_speed(g.gamma, s, maxLam, maxVel);
+      if (WI) {
+        WI[fs_base(cell, FQ) + size_t(ps * P + q) * FSB] = Mhd::weno(g.gamma, s);
+        __builtin_amdgcn_sched_barrier(0);  // one pow at a time: bounded registers
+      }
     }
+  }
+  const double *wi = WT + size_t(FQ) * C;
+#pragma unroll 1
+  for (int ps = 0; ps < NI / PI; ps++) {
+    double acc[9][PI];
+    mhd_pass<D, Bx, PI>(lds, corner, wi + size_t(ps) * C * PI, acc);
+#pragma unroll
+    for (int q = 0; q < PI; q++)
+#pragma unroll
+      for (int v = 0; v < 9; v++) SRC[(size_t(ps * PI + q) * 9 + v) * nc + cell] = acc[v][q];
+#pragma unroll
+    for (int q = 0; q < PI; q++) {
+      double s[9];
+#pragma unroll
+      for (int v = 0; v < 9; v++) s[v] = acc[v][q];
+      thD = dens_theta(thD, s[0], pb);
+      okE = okE && Mhd::eint(s) > pb.minEint;
+    }
+  }
+  spd[cell] = maxLam;
+  spd[nc + cell] = maxVel;
+  const bool slow = !(thD == 1.0 && okE);
+  if (!slow) thetaNew[cell] = 1.0;
+  const unsigned long long mask = __ballot(slow);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(ppCount, __popcll(mask));
+    base = __shfl(base, leader);
+    if (slow) ppList[base + __popcll(mask & ((1ull << lane) - 1ull))] = cell;
   }
 }
 
@@ -207,8 +354,147 @@ __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ 
   }
 }
 
+// ------------------------------------------------------------------ limiter
+__device__ inline double mhd_eint_pos(double emin, const double *ref, const double *s) {
+  double a = 0.0, b = 1.0, c;
+  double st[9];
+  for (int it = 0; it < 10; it++) {
+    c = 0.5 * (a + b);
+#pragma unroll
+    for (int v = 0; v < 9; v++) st[v] = ref[v] + c * (s[v] - ref[v]);
+    const double dec = Mhd::eint(st) - emin;
+    if (dec > 0.0) a = c;
+    else b = c;
+  }
+  return a;
+}
+
+template <int D, int NQ>
+__device__ inline void mhd_scale(double *__restrict__ FS, double *__restrict__ SRC, size_t nc,
+                                 int c, const double *ref, double th, bool replace) {
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+#pragma unroll 1
+  for (int p = 0; p < FQ + NI; p++) {
+    double *b = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
+    const size_t bs = p < FQ ? size_t(FSB) : nc;
+#pragma unroll
+    for (int v = 0; v < 9; v++) b[v * bs] = replace ? ref[v] : ref[v] + th * (b[v * bs] - ref[v]);
+  }
+}
+
+// Per-cell internal energy and sound speed (MHD_GLM_EOS.H), the two
+// neighbourhood quantities of the limiter bounds; both are even in the
+// momenta and the field, so mirrored wall cells need no sign.
+template <int D>
+__global__ __launch_bounds__(BLK) void k_mhd_props(const double *__restrict__ U, double *__restrict__ props,
+                                                   Grid g) {
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  if (c >= g.ncell) return;
+  const size_t nc = g.ncell;
+  double s[9];
+#pragma unroll
+  for (int v = 0; v < 9; v++) s[v] = U[size_t(v) * nc + c];
+  props[c] = Mhd::eint(s);
+  props[nc + c] = Mhd::sound(g.gamma, s);
+}
+
+// The limiter's exact sequence on one cell (PositivityPreserve_K with the
+// interior points, Common_K.H:299-643), then the cleaning-speed maxima of its
+// limited face states into spd: the energy pass already holds each face
+// state, so the speeds come from those registers unless that pass rescales
+// the states again.
+template <int D, int NQ>
+__device__ inline void mhd_pp_cell(const double *__restrict__ U, double *__restrict__ FS, double *__restrict__ SRC,
+                                   double *__restrict__ thetaNew, double *__restrict__ spd,
+                                   const double *__restrict__ props, const Grid &g, int c) {
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
+  int i, j, k;
+  cell_ijk<D>(c, g, i, j, k);
+  const size_t nc = g.ncell;
+  double ref[9];
+#pragma unroll
+  for (int v = 0; v < 9; v++) ref[v] = U[size_t(v) * nc + c];
+  // ranges over the 3^D neighbourhood (Common_K.H:329-387)
+  double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
+  double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
+  constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
+#pragma unroll 1
+  for (int xo = -1; xo <= 1; xo++)
+#pragma unroll 1
+    for (int yo = -1; yo <= 1; yo++)
+#pragma unroll 1
+      for (int zo = zl; zo <= zh; zo++) {
+        const size_t q = map_cell<D, 9>(g, i + xo, j + yo, k + zo).idx;
+        const double rho = U[q];
+        maxDens = fmax(maxDens, rho);
+        minDens = fmin(minDens, rho);
+        minEint = fmin(minEint, props[q]);
+        minSdSp = fmin(minSdSp, props[nc + q]);
+      }
+  const double dxU = load_u<D, 9>(U, g, i + 1, j, k, 1) / load_u<D, 9>(U, g, i + 1, j, k, 0) -
+                     load_u<D, 9>(U, g, i - 1, j, k, 1) / load_u<D, 9>(U, g, i - 1, j, k, 0);
+  const double dyV = load_u<D, 9>(U, g, i, j + 1, k, 2) / load_u<D, 9>(U, g, i, j + 1, k, 0) -
+                     load_u<D, 9>(U, g, i, j - 1, k, 2) / load_u<D, 9>(U, g, i, j - 1, k, 0);
+  double dzW;
+  if (D == 3)
+    dzW = load_u<D, 9>(U, g, i, j, k + 1, 3) / load_u<D, 9>(U, g, i, j, k + 1, 0) -
+          load_u<D, 9>(U, g, i, j, k - 1, 3) / load_u<D, 9>(U, g, i, j, k - 1, 0);
+  else
+    dzW = ref[3] / ref[0] - ref[3] / ref[0];
+  const double divV = (dxU + dyV + dzW) / 2.0;
+  double kappa = 0.2;
+  const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
+  kappa *= (1.0 - eta);
+  maxDens *= (1.0 + kappa);
+  minDens *= (1.0 - kappa);
+  minEint *= (1.0 - kappa);
+  PPBounds b;
+  b.rDens = ref[0];
+  b.maxDens = maxDens;
+  b.minDens = minDens;
+  b.minEint = minEint;
+  // density: faces then interior points
+  double thD = 1.0;
+#pragma unroll 1
+  for (int p = 0; p < FQ + NI; p++) {
+    const double rho = p < FQ ? FS[fs_base(c, FQ * 9) + size_t(p) * 9 * FSB] : SRC[size_t(p - FQ) * 9 * nc + c];
+    thD = dens_theta(thD, rho, b);
+  }
+  if (thD > 0.0 && thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, false);
+  else if (thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, true);
+  // internal energy (and the speeds of the face states as they stand)
+  double thP = 1.0, maxLam = 0.0, maxVel = 0.0;
+#pragma unroll 1
+  for (int p = 0; p < FQ + NI; p++) {
+    const double *q = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
+    const size_t qs = p < FQ ? size_t(FSB) : nc;
+    double s[9];
+#pragma unroll
+    for (int v = 0; v < 9; v++) s[v] = q[v * qs];
+    const double ei = Mhd::eint(s);
+    thP = ei > minEint ? thP : fmin(thP, mhd_eint_pos(minEint, ref, s));
+    if (p < FQ) mhd_face_speed(g.gamma, s, maxLam, maxVel);
+  }
+  if (thP < 1.0) {
+    mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thP, !(thP > 0.0));
+    maxLam = 0.0;
+    maxVel = 0.0;
+#pragma unroll 1
+    for (int fq = 0; fq < FQ; fq++) {
+      double s[9];
+#pragma unroll
+      for (int v = 0; v < 9; v++) s[v] = FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB];
+      mhd_face_speed(g.gamma, s, maxLam, maxVel);
+    }
+  }
+  thetaNew[c] = fmin(thD, thP);
+  spd[c] = maxLam;
+  spd[nc + c] = maxVel;
+}
+
 // Back to conserved variables (linPrim2Cons, MHD_GLM_K.H:24-69) at every face
-// and interior point of the listed cells.  The energy sums its terms in the
+// and interior point of the listed cells (list == nullptr: every cell), then
+// the limiter's exact sequence on the cell.  The energy sums its terms in the
 // reference's operand order
 //   E = vsq d + rho u mx + rho v my + rho w mz + bx Bx + by By + bz Bz
 //       + e / (gamma - 1) + psi0 psi.
@@ -217,7 +503,9 @@ __global__ __launch_bounds__(BLK) void k_mhd_weno_post(const double *__restrict_
                                                        double *__restrict__ FS,
                                                        double *__restrict__ SRC, Grid g,
                                                        const int *__restrict__ list,
-                                                       const int *__restrict__ counter) {
+                                                       const int *__restrict__ counter,
+                                                       double *__restrict__ thetaNew, double *__restrict__ spd,
+                                                       const double *__restrict__ props) {
   constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
   const int n = list ? *counter : g.ncell;
   const size_t nc = g.ncell;
@@ -252,164 +540,24 @@ __global__ __launch_bounds__(BLK) void k_mhd_weno_post(const double *__restrict_
       P[3 * ps] = mz;
       P[4 * ps] = e;
     }
+    mhd_pp_cell<D, NQ>(U, FS, SRC, thetaNew, spd, props, g, c);
   }
 }
 
-// ------------------------------------------------------------------ limiter
-__device__ inline double mhd_eint_pos(double emin, const double *ref, const double *s) {
-  double a = 0.0, b = 1.0, c;
-  double st[9];
-  for (int it = 0; it < 10; it++) {
-    c = 0.5 * (a + b);
-#pragma unroll
-    for (int v = 0; v < 9; v++) st[v] = ref[v] + c * (s[v] - ref[v]);
-    const double dec = Mhd::eint(st) - emin;
-    if (dec > 0.0) a = c;
-    else b = c;
-  }
-  return a;
-}
-
+// The limiter's exact sequence for the cells of the linear pass that failed
+// its fast test (list, *count of them); WENO cells are skipped, their
+// post-WENO pass limits them.
 template <int D, int NQ>
-__device__ inline void mhd_scale(double *__restrict__ FS, double *__restrict__ SRC, size_t nc,
-                                 int c, const double *ref, double th, bool replace) {
-  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
-#pragma unroll 1
-  for (int p = 0; p < FQ + NI; p++) {
-    double *b = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
-    const size_t bs = p < FQ ? size_t(FSB) : nc;
-#pragma unroll
-    for (int v = 0; v < 9; v++) b[v * bs] = replace ? ref[v] : ref[v] + th * (b[v * bs] - ref[v]);
-  }
-}
-
-// Fastest magnetosonic speed + |u| and |u| of one face state, x-components on
-// every face (SpeedEstimate_K, MHD_GLM_K.H:213-310).
-__device__ inline void mhd_face_speed(double gamma, const double *s, double &maxLam, double &maxVel) {
-  const double vel = fabs(s[1] / s[0]);
-  const double lam = Mhd::fast(gamma, s, s[5]) + vel;
-  maxVel = vel > maxVel ? vel : maxVel;
-  maxLam = lam > maxLam ? lam : maxLam;
-}
-
-// Per-cell internal energy and sound speed (MHD_GLM_EOS.H), the two
-// neighbourhood quantities of the limiter bounds; both are even in the
-// momenta and the field, so mirrored wall cells need no sign.
-template <int D>
-__global__ __launch_bounds__(BLK) void k_mhd_props(const double *__restrict__ U, double *__restrict__ props,
-                                                   Grid g) {
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
-  const size_t nc = g.ncell;
-  double s[9];
-#pragma unroll
-  for (int v = 0; v < 9; v++) s[v] = U[size_t(v) * nc + c];
-  props[c] = Mhd::eint(s);
-  props[nc + c] = Mhd::sound(g.gamma, s);
-}
-
-// Limiter of every cell, then the cleaning-speed maxima of its limited face
-// states: the energy pass already holds each face state, so the speeds come
-// from those registers unless that pass rescales the states again.  A block
-// folds its maxima into chBits (zeroed by the caller) with one atomic each.
-template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U,
-                                                double *__restrict__ FS,
-                                                double *__restrict__ SRC,
-                                                double *__restrict__ thetaNew, Grid g,
-                                                unsigned long long *__restrict__ chBits,
-                                                const double *__restrict__ props) {
-  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
-  __shared__ double sh[BLK];
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  double maxLam = 0.0, maxVel = 0.0;
-  if (c < g.ncell) {
-    int i, j, k;
-    cell_ijk<D>(c, g, i, j, k);
-    const size_t nc = g.ncell;
-    double ref[9];
-#pragma unroll
-    for (int v = 0; v < 9; v++) ref[v] = U[size_t(v) * nc + c];
-    // ranges over the 3^D neighbourhood (Common_K.H:329-387)
-    double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;
-    double minEint = 1.7976931348623157e308, minSdSp = 1.7976931348623157e308;
-    constexpr int zl = D == 3 ? -1 : 0, zh = D == 3 ? 1 : 0;
-#pragma unroll 1
-    for (int xo = -1; xo <= 1; xo++)
-#pragma unroll 1
-      for (int yo = -1; yo <= 1; yo++)
-#pragma unroll 1
-        for (int zo = zl; zo <= zh; zo++) {
-          const size_t q = map_cell<D, 9>(g, i + xo, j + yo, k + zo).idx;
-          const double rho = U[q];
-          maxDens = fmax(maxDens, rho);
-          minDens = fmin(minDens, rho);
-          minEint = fmin(minEint, props[q]);
-          minSdSp = fmin(minSdSp, props[nc + q]);
-        }
-    const double dxU = load_u<D, 9>(U, g, i + 1, j, k, 1) / load_u<D, 9>(U, g, i + 1, j, k, 0) -
-                       load_u<D, 9>(U, g, i - 1, j, k, 1) / load_u<D, 9>(U, g, i - 1, j, k, 0);
-    const double dyV = load_u<D, 9>(U, g, i, j + 1, k, 2) / load_u<D, 9>(U, g, i, j + 1, k, 0) -
-                       load_u<D, 9>(U, g, i, j - 1, k, 2) / load_u<D, 9>(U, g, i, j - 1, k, 0);
-    double dzW;
-    if (D == 3)
-      dzW = load_u<D, 9>(U, g, i, j, k + 1, 3) / load_u<D, 9>(U, g, i, j, k + 1, 0) -
-            load_u<D, 9>(U, g, i, j, k - 1, 3) / load_u<D, 9>(U, g, i, j, k - 1, 0);
-    else
-      dzW = ref[3] / ref[0] - ref[3] / ref[0];
-    const double divV = (dxU + dyV + dzW) / 2.0;
-    double kappa = 0.2;
-    const double eta = fmin(1.0, fmax(0.0, -(kappa * minSdSp + divV) / (kappa * minSdSp)));
-    kappa *= (1.0 - eta);
-    maxDens *= (1.0 + kappa);
-    minDens *= (1.0 - kappa);
-    minEint *= (1.0 - kappa);
-    PPBounds b;
-    b.rDens = ref[0];
-    b.maxDens = maxDens;
-    b.minDens = minDens;
-    b.minEint = minEint;
-    // density: faces then interior points
-    double thD = 1.0;
-#pragma unroll 1
-    for (int p = 0; p < FQ + NI; p++) {
-      const double rho = p < FQ ? FS[fs_base(c, FQ * 9) + size_t(p) * 9 * FSB] : SRC[size_t(p - FQ) * 9 * nc + c];
-      thD = dens_theta(thD, rho, b);
-    }
-    if (thD > 0.0 && thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, false);
-    else if (thD < 1.0) mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thD, true);
-    // internal energy (and the speeds of the face states as they stand)
-    double thP = 1.0;
-#pragma unroll 1
-    for (int p = 0; p < FQ + NI; p++) {
-      const double *q = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
-      const size_t qs = p < FQ ? size_t(FSB) : nc;
-      double s[9];
-#pragma unroll
-      for (int v = 0; v < 9; v++) s[v] = q[v * qs];
-      const double ei = Mhd::eint(s);
-      thP = ei > minEint ? thP : fmin(thP, mhd_eint_pos(minEint, ref, s));
-      if (p < FQ) mhd_face_speed(g.gamma, s, maxLam, maxVel);
-    }
-    if (thP < 1.0) {
-      mhd_scale<D, NQ>(FS, SRC, nc, c, ref, thP, !(thP > 0.0));
-      maxLam = 0.0;
-      maxVel = 0.0;
-#pragma unroll 1
-      for (int fq = 0; fq < FQ; fq++) {
-        double s[9];
-#pragma unroll
-        for (int v = 0; v < 9; v++) s[v] = FS[fs_base(c, FQ * 9) + (size_t(fq) * 9 + v) * FSB];
-        mhd_face_speed(g.gamma, s, maxLam, maxVel);
-      }
-    }
-    thetaNew[c] = fmin(thD, thP);
-  }
-  const double bl = block_max(maxLam, sh, threadIdx.x);
-  const double bv = block_max(maxVel, sh, threadIdx.x);
-  if (threadIdx.x == 0) {
-    if (bl > 0.0) atomicMax(&chBits[0], (unsigned long long)__double_as_longlong(bl));
-    if (bv > 0.0) atomicMax(&chBits[1], (unsigned long long)__double_as_longlong(bv));
+__global__ __launch_bounds__(BLK) void k_mhd_pp(const double *__restrict__ U, double *__restrict__ FS,
+                                                double *__restrict__ SRC, double *__restrict__ thetaNew,
+                                                double *__restrict__ spd, const double *__restrict__ props,
+                                                const double *__restrict__ wflag, Grid g,
+                                                const int *__restrict__ list, const int *__restrict__ count) {
+  const int n = *count;
+  for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
+    const int c = list[t];
+    if (wflag[c] != 0.0) continue;
+    mhd_pp_cell<D, NQ>(U, FS, SRC, thetaNew, spd, props, g, c);
   }
 }
 
@@ -422,14 +570,10 @@ __device__ inline double glm_ch(const unsigned long long *chBits) {
 // Dm: differentiation matrix [NQ][NQ+2] over the points (-1/2, interior, +1/2)
 // of one cell (DiffMatLUT, Numeric.cpp:47-99).
 template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_sources(const double *__restrict__ U,
-                                                     const double *__restrict__ FS,
-                                                     double *__restrict__ SRC,
-                                                     const double *__restrict__ theta, Grid g,
-                                                     const double *__restrict__ Dm) {
+__device__ inline void mhd_sources_cell(const double *__restrict__ U, const double *__restrict__ FS,
+                                        double *__restrict__ SRC, const double *__restrict__ theta,
+                                        const Grid &g, const double *__restrict__ Dm, int c) {
   constexpr int Q = MShape<D, NQ>::Q, NI = MShape<D, NQ>::NI, NZ = MShape<D, NQ>::NZ, FQ = MShape<D, NQ>::FQ;
-  const int c = blockIdx.x * BLK + threadIdx.x;
-  if (c >= g.ncell) return;
   int i, j, k;
   cell_ijk<D>(c, g, i, j, k);
   const size_t nc = g.ncell;
@@ -514,6 +658,33 @@ __global__ __launch_bounds__(BLK) void k_mhd_sources(const double *__restrict__ 
     src(p, 6) = f0 - divB * vy;
     src(p, 7) = f0 - divB * vz;
     src(p, 8) = -gpsiDu[p];
+  }
+}
+
+// Source terms of every cell; each block also folds the per-cell
+// cleaning-speed maxima spd [maxLam | maxVel] into chBits (zeroed by the
+// caller) with one atomic each, for the flux kernel that follows.
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_sources(const double *__restrict__ U,
+                                                     const double *__restrict__ FS,
+                                                     double *__restrict__ SRC,
+                                                     const double *__restrict__ theta, Grid g,
+                                                     const double *__restrict__ Dm,
+                                                     const double *__restrict__ spd,
+                                                     unsigned long long *__restrict__ chBits) {
+  __shared__ double sh[BLK];
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  double lam = 0.0, vel = 0.0;
+  if (c < g.ncell) {
+    mhd_sources_cell<D, NQ>(U, FS, SRC, theta, g, Dm, c);
+    lam = spd[c];
+    vel = spd[size_t(g.ncell) + c];
+  }
+  const double bl = block_max(lam, sh, threadIdx.x);
+  const double bv = block_max(vel, sh, threadIdx.x);
+  if (threadIdx.x == 0) {
+    if (bl > 0.0) atomicMax(&chBits[0], (unsigned long long)__double_as_longlong(bl));
+    if (bv > 0.0) atomicMax(&chBits[1], (unsigned long long)__double_as_longlong(bv));
   }
 }
 

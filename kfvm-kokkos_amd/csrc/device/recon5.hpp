@@ -66,12 +66,12 @@ __device__ inline int box_out(int t) {
 // Limiter bounds (Common_K.H:329-387) inside a staged brick: densities and
 // velocities from LDS, the neighbours' internal energy and sound speed from
 // the per-cell property planes (both are even in the momenta, so the mirror
-// images of wall boundaries need no sign handling).
-template <int D>
+// images of wall boundaries need no sign handling).  Bx: the brick shape
+// (the LDS holds each variable's box as one plane of Bx::N cells).
+template <int D, class Bx = RBox<D>>
 __device__ inline PPBounds pp_bounds_lds(const double *__restrict__ lds, int c0,
                                          const double *__restrict__ props, const Grid &g, int i,
                                          int j, int k, const double *ref) {
-  using Bx = RBox<D>;
   constexpr int SX = 1, SY = Bx::LX, SZ = Bx::LX * Bx::LY, NP = Bx::N;
   const size_t nc = g.ncell;
   double maxDens = -1.7976931348623157e308, minDens = 1.7976931348623157e308;

@@ -159,6 +159,8 @@ void Solver::setup() {
     cellWS_ = dalloc<double>(wts_.cell.size(), "interior weights (stencil-major)");
     diffM_ = dalloc<double>(size_t(qr2.n) * (qr2.n + 2), "differentiation matrix");
     chBits_ = dalloc<unsigned long long>(2, "cleaning speed");
+    spd_ = dalloc<double>(nc * 2, "cleaning-speed maxima per cell");
+    mhdWT_ = dalloc<double>(size_t(FQ + NI) * wts_.C, "linear weights (pass-major)");
     const std::vector<double> Dm = diff_matrix(cfg_.nQuad);
     check(hipMemcpy(diffM_, Dm.data(), Dm.size() * sizeof(double), hipMemcpyHostToDevice), "copy diff matrix");
   }
@@ -584,7 +586,7 @@ void Solver::release() noexcept {
   } catch (...) {
   }
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_, faceWC_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_, faceWC_, spd_, mhdWT_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto &kv : graphs_)
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -649,18 +651,23 @@ bool Solver::profileGet(const std::string &name, double &ms, int64_t &count) {
 }
 
 // GLM-MHD evaluation (Solver.cpp:410-530 with sources): reconstruction of
-// face and interior states, selection, WENO, limiter on every cell, cleaning
-// speed, source terms, then the Riemann / divergence / RK kernel.
+// face and interior states with the limiter's fast test, selection, WENO
+// (converted back and limited), the limiter's exact sequence for the other
+// cells that failed the test, source terms with the cleaning speed, then the
+// Riemann / divergence / RK kernel.
 void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
   const int dim = cfg_.dim, nq = cfg_.nQuad;
   const double *thPrev = theta_[th_];
   double *thNew = theta_[th_ ^ 1];
   lastRhsFull_ = !useSparseWeno;
+  // counter_: [0] WENO cells, [1] cells of the linear pass for the limiter's exact sequence
   check(launch_zero(counter_, 2 * sizeof(int), stream_), "clear counters");
+  timed("cell_props", [&] { return launch_mhd_props(dim, Uin, props_, grid_, stream_); });
   if (useSparseWeno) {
     const bool lrn = learned_ && !forcePlain_;
     timed("linear", [&] {
-      return launch_mhd_linear(dim, nq, Uin, FS_, SRC_, lrn ? nullptr : WI_, grid_, faceW_, cellW_, stream_);
+      return launch_mhd_linear(dim, nq, Uin, FS_, SRC_, lrn ? nullptr : WI_, thNew, spd_, props_, grid_, mhdWT_,
+                               ppList_, counter_ + 1, stream_);
     });
     if (lrn) {
       timed("select", [&] {
@@ -674,20 +681,25 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
       });
     }
     timed("weno", [&] {
-      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, list_, counter_,
-                             stream_);
+      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, list_, counter_, thNew,
+                             spd_, props_, stream_);
+    });
+    timed("pp", [&] {
+      return launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thNew, spd_, props_, wflag_, grid_, ppList_, counter_ + 1,
+                           stream_);
     });
   } else {
     timed("weno", [&] {
-      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr,
-                             stream_);
+      return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr, thNew,
+                             spd_, props_, stream_);
     });
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
-  check(launch_zero(chBits_, 2 * sizeof(unsigned long long), stream_), "clear speeds");
-  timed("pp", [&] { return launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thNew, grid_, chBits_, props_, stream_); });
   th_ ^= 1;
-  timed("sources", [&] { return launch_mhd_sources(dim, nq, Uin, FS_, SRC_, thNew, grid_, diffM_, stream_); });
+  check(launch_zero(chBits_, 2 * sizeof(unsigned long long), stream_), "clear speeds");
+  timed("sources", [&] {
+    return launch_mhd_sources(dim, nq, Uin, FS_, SRC_, thNew, grid_, diffM_, spd_, chBits_, stream_);
+  });
   RKArgs a = a0;
   a.chBits = chBits_;
   a.src = SRC_;
@@ -851,6 +863,7 @@ size_t Solver::carveEval(unsigned char *base, EvalBufs &b) const {
   b.thNew = static_cast<double *>(take(nc * D));
   b.wflag = static_cast<double *>(take(nc * D));
   b.SRC = SRC_ ? static_cast<double *>(take(nc * size_t(wts_.nqCell) * nv_ * D)) : nullptr;
+  b.spd = spd_ ? static_cast<double *>(take(nc * 2 * D)) : nullptr;
   b.list = static_cast<int *>(take(nc * I));
   b.counter = static_cast<int *>(take(4 * I));
   b.ppList = static_cast<int *>(take(nc * I));
@@ -889,6 +902,7 @@ double Solver::evalRhsNoAlloc(const double *U, double *rhs) {
       s.theta_[th ^ 1] = keep.thNew;
       s.wflag_ = keep.wflag;
       s.SRC_ = keep.SRC;
+      s.spd_ = keep.spd;
       s.list_ = keep.list;
       s.counter_ = keep.counter;
       s.ppList_ = keep.ppList;
@@ -902,8 +916,8 @@ double Solver::evalRhsNoAlloc(const double *U, double *rhs) {
       s.nRhsEval = nRhs;
     }
   } restore{*this,
-            {FS_, WS_, SB_, WI_, props_, theta_[th_ ^ 1], wflag_, SRC_, list_, counter_, ppList_, pos_, chBits_,
-             vmax_},
+            {FS_, WS_, SB_, WI_, props_, theta_[th_ ^ 1], wflag_, SRC_, spd_, list_, counter_, ppList_, pos_,
+             chBits_, vmax_},
             th_,
             useSparseWeno,
             lastRhsFull_,
@@ -917,6 +931,7 @@ double Solver::evalRhsNoAlloc(const double *U, double *rhs) {
   theta_[th_ ^ 1] = b.thNew;
   wflag_ = b.wflag;
   SRC_ = b.SRC;
+  spd_ = b.spd;
   list_ = b.list;
   counter_ = b.counter;
   ppList_ = b.ppList;
@@ -1102,11 +1117,15 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
   if (!FSlin_) FSlin_ = dalloc<double>(fsBytes / sizeof(double), "linear face states");
   if (nv_ == 9) {
     // labels compare the linear face states with the limited WENO ones
-    check(launch_mhd_linear(dim, nq, Uin, FS_, SRC_, nullptr, grid_, faceW_, cellW_, stream_), "mhd_linear");
+    check(launch_zero(counter_, 2 * sizeof(int), stream_), "clear counters");
+    check(launch_mhd_props(dim, Uin, props_, grid_, stream_), "mhd_props");
+    check(launch_mhd_linear(dim, nq, Uin, FS_, SRC_, nullptr, thScratch, spd_, props_, grid_, mhdWT_, ppList_,
+                            counter_ + 1, stream_),
+          "mhd_linear");
     check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
-    check(launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr, stream_),
+    check(launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr, thScratch,
+                          spd_, props_, stream_),
           "mhd_weno");
-    check(launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thScratch, grid_, chBits_, props_, stream_), "mhd_pp");
     check(launch_labels(dim, nq, 9, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
     check(launch_features(dim, 9, Uin, nullptr, X, grid_, stream_), "features");
     check(hipStreamSynchronize(stream_), "sync");
@@ -1280,6 +1299,16 @@ void Solver::uploadWeights() {
     check(hipMemcpy(cellW_, wts_.cell.data(), wts_.cell.size() * sizeof(double), hipMemcpyHostToDevice),
           "copy weights");
     check(hipMemcpy(cellWS_, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
+  }
+  if (mhdWT_) {  // full-stencil face passes [FQ/P][C][P], then interior passes [NI/PI][C][PI]
+    const int NI = wts_.nqCell, PI = mhd_interior_pass_points(wts_.dim, wts_.nq);
+    std::vector<double> mw(size_t(FQ + NI) * C);
+    std::copy(wt.begin(), wt.end(), mw.begin());
+    double *wi = mw.data() + size_t(FQ) * C;
+    for (int ps = 0; ps < NI / PI; ps++)
+      for (int n = 0; n < C; n++)
+        for (int q = 0; q < PI; q++) wi[(size_t(ps) * C + n) * PI + q] = wts_.cell[size_t(ps * PI + q) * C + n];
+    check(hipMemcpy(mhdWT_, mw.data(), mw.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   }
 }
 void Solver::sync() const { check(hipStreamSynchronize(stream_), "sync"); }

@@ -106,7 +106,7 @@ class Solver {
   // Per-evaluation scratch buffers of one RHS evaluation (pipeline)
   struct EvalBufs {
     double *FS = nullptr, *WS = nullptr, *SB = nullptr, *WI = nullptr, *props = nullptr, *thNew = nullptr,
-           *wflag = nullptr, *SRC = nullptr;
+           *wflag = nullptr, *SRC = nullptr, *spd = nullptr;
     int *list = nullptr, *counter = nullptr, *ppList = nullptr, *pos = nullptr;
     unsigned long long *chBits = nullptr, *vmax = nullptr;
   };
@@ -163,6 +163,8 @@ class Solver {
   // matrix, cleaning-speed maxima
   double *SRC_ = nullptr, *cellW_ = nullptr, *cellWS_ = nullptr, *diffM_ = nullptr;
   unsigned long long *chBits_ = nullptr;
+  // GLM-MHD: per-cell cleaning-speed maxima [2][ncell]; linear-pass weights (face, then interior passes)
+  double *spd_ = nullptr, *mhdWT_ = nullptr;
   double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr, *faceWS_ = nullptr, *derivWS_ = nullptr;
   double *faceWC_ = nullptr;  // Euler WENO: compact face + derivative weights (compact_weno_weights)
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;

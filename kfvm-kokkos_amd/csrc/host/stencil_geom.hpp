@@ -164,6 +164,11 @@ KFVM_HD constexpr int linear_pass_points(int dim, int nq) {
   return dim == 3 ? (nq == 3 ? nq * nq : 2 * nq * nq) : 2 * nq;
 }
 
+// Interior quadrature points per pass of the GLM-MHD linear kernel (all
+// NQ^D points, or one NQ^2 layer of the 3D 3x3x3 set).  The weight copy
+// of that kernel holds the face passes, then the interior passes.
+KFVM_HD constexpr int mhd_interior_pass_points(int dim, int nq) { return dim == 3 && nq == 2 ? 8 : nq * nq; }
+
 KFVM_HD constexpr int num_monos(int D, int t) {
   return D == 2 ? (t + 1) * (t + 2) / 2 : (t + 1) * (t + 2) * (t + 3) / 6;
 }
