@@ -110,7 +110,10 @@ struct RKArgs {
   double *posPartial;   // per-block positivity minima (FSAL mode)
   unsigned long long *vmaxBits;  // running max wave speed (bit pattern of a non-negative double)
   const unsigned long long *chBits;  // GLM-MHD: {max(c_f + |u|), max |u|} of this evaluation
-  const double *src;                 // GLM-MHD: source terms at the interior points
+  // GLM-MHD: limited states at the interior points [(p * 9 + v)][ncell], this
+  // evaluation's theta and the differentiation matrix [NQ][NQ + 2]: the flux
+  // kernel forms and integrates the source terms
+  const double *src, *theta, *Dm;
   // 2D batch of independent problems (slices): per-slice step sizes; the
   // epilogue then uses beta * sliceDt[s] and bhat * sliceDt[s] and keeps one
   // running max wave speed per slice (vmaxBits[s])
@@ -196,8 +199,8 @@ hipError_t launch_weno_post(int dim, int nq, const double *U, double *FS, double
 // terms, at the NQ^D interior points; props [2][ncell]: per-cell internal
 // energy and sound speed (launch_mhd_props, before the linear pass or WENO);
 // spd [2][ncell]: per-cell maxima of c_f + |u| and |u| over the cell's limited
-// face states, which launch_mhd_sources folds into chBits (zeroed by the
-// caller) for the flux kernel's cleaning speed.
+// face states, which launch_mhd_chmax folds into chBits for the flux kernel's
+// cleaning speed.
 hipError_t launch_mhd_props(int dim, const double *U, double *props, const Grid &g, hipStream_t s);
 // WT: full-stencil weights per pass, the face passes [F*Q/P][C][P]
 // (P = linear_pass_points) then the interior passes [NQ^D/PI][C][PI]
@@ -221,9 +224,10 @@ hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double 
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
                          double *spd, const double *props, const double *wflag, const Grid &g, const int *list,
                          const int *count, hipStream_t s);
-hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
-                              const double *theta, const Grid &g, const double *Dm, const double *spd,
-                              unsigned long long *chBits, hipStream_t s);
+// Cleaning speed: the per-cell maxima spd folded into chBits (zeroed by the
+// caller).  The source terms are formed inside launch_flux_rk (RKArgs src,
+// theta, Dm).
+hipError_t launch_mhd_chmax(int dim, const double *spd, const Grid &g, unsigned long long *chBits, hipStream_t s);
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
                           const Grid &g, const RKArgs &a, hipStream_t s);
 int flux_rk_partials(int dim, const Grid &g);

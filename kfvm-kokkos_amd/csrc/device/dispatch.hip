@@ -113,10 +113,11 @@ hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *S
                          const int *count, hipStream_t s) {
   KFVM_SELECT(dim, nq, L::mhd_pp(U, FS, SRC, thetaNew, spd, props, wflag, g, list, count, s));
 }
-hipError_t launch_mhd_sources(int dim, int nq, const double *U, const double *FS, double *SRC,
-                              const double *theta, const Grid &g, const double *Dm, const double *spd,
-                              unsigned long long *chBits, hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_sources(U, FS, SRC, theta, g, Dm, spd, chBits, s));
+hipError_t launch_mhd_chmax(int dim, const double *spd, const Grid &g, unsigned long long *chBits, hipStream_t s) {
+  (void)dim;
+  const unsigned nb = std::min<unsigned>(stage_blocks(g.ncell), 2048u);
+  hipLaunchKernelGGL(dev::k_mhd_chmax, dim3(nb), dim3(dev::BLK), 0, s, spd, int64_t(g.ncell), chBits);
+  return hipGetLastError();
 }
 
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,

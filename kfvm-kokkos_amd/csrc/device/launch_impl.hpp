@@ -140,13 +140,6 @@ struct Launch {
                        wflag, g, list, count);
     return hipGetLastError();
   }
-  static hipError_t mhd_sources(const double *U, const double *FS, double *SRC, const double *theta,
-                                const Grid &g, const double *Dm, const double *spd, unsigned long long *chBits,
-                                hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_mhd_sources<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, SRC,
-                       theta, g, Dm, spd, chBits);
-    return hipGetLastError();
-  }
   // ---- learned troubled-cell indicator
   // nv 5 (Euler; eint = the limiter's internal-energy plane) or 9 (GLM-MHD;
   // the internal energy comes from the conserved variables)

@@ -15,10 +15,11 @@
 //                    (Common_K.H:299-643) for the cells that failed the fast
 //                    test, then SpeedEstimate_K (MHD_GLM_K.H:213-310) on the
 //                    limited face states
-//   k_mhd_sources    SourceTerms_K (MHD_GLM_K.H:316-547): div B and psi-gradient
-//                    terms at the interior points; the cleaning speed
-//                    c_h = max(c_f + |u|) - max(|u|) over all face states,
-//                    x-components on every face
+//   mhd_source_rhs   SourceTerms_K (MHD_GLM_K.H:316-547): div B and psi-gradient
+//                    terms at the interior points, integrated inside the flux
+//                    kernel (k_flux_rk)
+//   k_mhd_chmax      the cleaning speed c_h = max(c_f + |u|) - max(|u|) over
+//                    all face states, x-components on every face
 #pragma once
 
 namespace kfvm {
@@ -567,48 +568,33 @@ __device__ inline double glm_ch(const unsigned long long *chBits) {
 }
 
 // -------------------------------------------------------------- GLM sources
-// Dm: differentiation matrix [NQ][NQ+2] over the points (-1/2, interior, +1/2)
-// of one cell (DiffMatLUT, Numeric.cpp:47-99).
+// Source terms (SourceTerms_K, MHD_GLM_K.H:316-547; no user forcing for the
+// registry problems) of cell c at its NQ^D interior points, integrated into
+// rhs with the interior quadrature weights (IntegrateRHS_K), point by point
+// in ascending order.  SRC holds the limited interior-point states; divB is
+// the cell's divergence from the face-averaged normal field (the flux kernel
+// forms it from the two states of every face point).  u . grad(psi): the psi
+// gradient from the cell's own face and interior values with the
+// differentiation matrix Dm [NQ][NQ+2] over the points (-1/2, interior, +1/2)
+// (DiffMatLUT, Numeric.cpp:47-99), or from central differences of the cell
+// averages where the limiter took theta below 0.75.
 template <int D, int NQ>
-__device__ inline void mhd_sources_cell(const double *__restrict__ U, const double *__restrict__ FS,
-                                        double *__restrict__ SRC, const double *__restrict__ theta,
-                                        const Grid &g, const double *__restrict__ Dm, int c) {
+__device__ void mhd_source_rhs(const double *__restrict__ U, const double *__restrict__ FS,
+                               const double *__restrict__ SRC, const double *__restrict__ theta, const Grid &g,
+                               const double *__restrict__ Dm, size_t c, int i, int j, int k, double divB,
+                               double *rhs) {
   constexpr int Q = MShape<D, NQ>::Q, NI = MShape<D, NQ>::NI, NZ = MShape<D, NQ>::NZ, FQ = MShape<D, NQ>::FQ;
-  int i, j, k;
-  cell_ijk<D>(c, g, i, j, k);
   const size_t nc = g.ncell;
-  auto own = [&](int f, int q, int v) { return FS[fs_base(c, FQ * 9) + (size_t(f * Q + q) * 9 + v) * FSB]; };
-  // normal field averaged over the two states of each face point
-  auto bav = [&](int f, int q) {
-    const int ax = f >> 1, hi = f & 1;
-    double in[9], out[9];
-#pragma unroll
-    for (int v = 0; v < 9; v++) in[v] = own(f, q, v);
-    const int ni = i + (ax == 0 ? (hi ? 1 : -1) : 0), nj = j + (ax == 1 ? (hi ? 1 : -1) : 0);
-    const int nk = k + (ax == 2 ? (hi ? 1 : -1) : 0);
-    face_state<D, NQ, 9>(FS, g, ni, nj, nk, f ^ 1, q, in, out);
-    return 0.5 * (in[5 + ax] + out[5 + ax]);
-  };
-  double divB = 0.0;
-#pragma unroll 1
-  for (int nQ = 0; nQ < NQ; nQ++)
-#pragma unroll 1
-    for (int nS = 0; nS < NZ; nS++) {
-      const int q = nS * NQ + nQ;
-      const double twt = D == 2 ? g.wt[nQ] : g.wt[nQ] * g.wt[nS];
-      divB += twt * (bav(1, q) - bav(0, q)) / g.dx;
-      divB += twt * (bav(3, q) - bav(2, q)) / g.dy;
-      if (D == 3) divB += twt * (bav(5, q) - bav(4, q)) / g.dz;
-    }
-  auto src = [&](int p, int v) -> double & { return SRC[(size_t(p) * 9 + v) * nc + c]; };
+  auto own = [&](int f, int q) { return FS[fs_base(c, FQ * 9) + (size_t(f * Q + q) * 9 + 8) * FSB]; };
+  auto src = [&](int p, int v) { return SRC[(size_t(p) * 9 + v) * nc + c]; };
   // u . grad(psi) at the interior points
   double gpsiDu[NI];
   const bool fd = theta[c] < 0.75;
-#pragma unroll 1
+#pragma unroll
   for (int nQ = 0; nQ < NQ; nQ++)
-#pragma unroll 1
+#pragma unroll
     for (int nR = 0; nR < NQ; nR++)
-#pragma unroll 1
+#pragma unroll
       for (int nS = 0; nS < NZ; nS++) {
         const int p = (nQ * NQ + nR) * NZ + nS;
         const double rho = src(p, 0);
@@ -624,61 +610,68 @@ __device__ inline void mhd_sources_cell(const double *__restrict__ U, const doub
           }
         } else {
           const int qx = D == 2 ? nR : nR * NQ + nS, qy = D == 2 ? nQ : nQ * NQ + nS;
-          dpx = Dm[nQ * (NQ + 2)] * own(0, qx, 8) + Dm[nQ * (NQ + 2) + NQ + 1] * own(1, qx, 8);
+          dpx = Dm[nQ * (NQ + 2)] * own(0, qx) + Dm[nQ * (NQ + 2) + NQ + 1] * own(1, qx);
+#pragma unroll
           for (int n = 1; n <= NQ; n++) dpx += Dm[nQ * (NQ + 2) + n] * src(((n - 1) * NQ + nR) * NZ + nS, 8);
           dpx /= g.dx;
-          dpy = Dm[nR * (NQ + 2)] * own(2, qy, 8) + Dm[nR * (NQ + 2) + NQ + 1] * own(3, qy, 8);
+          dpy = Dm[nR * (NQ + 2)] * own(2, qy) + Dm[nR * (NQ + 2) + NQ + 1] * own(3, qy);
+#pragma unroll
           for (int n = 1; n <= NQ; n++) dpy += Dm[nR * (NQ + 2) + n] * src((nQ * NQ + n - 1) * NZ + nS, 8);
           dpy /= g.dy;
           if (D == 3) {
             const int qz = nQ * NQ + nR;
-            dpz = Dm[nS * (NQ + 2)] * own(4, qz, 8) + Dm[nS * (NQ + 2) + NQ + 1] * own(5, qz, 8);
+            dpz = Dm[nS * (NQ + 2)] * own(4, qz) + Dm[nS * (NQ + 2) + NQ + 1] * own(5, qz);
+#pragma unroll
             for (int n = 1; n <= NQ; n++) dpz += Dm[nS * (NQ + 2) + n] * src((nQ * NQ + nR) * NZ + n - 1, 8);
             dpz /= g.dz;
           }
         }
         gpsiDu[p] = vx * dpx + vy * dpy + vz * dpz;
       }
-  // source terms (no user forcing for the registry problems)
+  // the source terms of each point, then their integral
 #pragma unroll
-  for (int p = 0; p < NI; p++) {
-    double st[9];
+  for (int nQ = 0; nQ < NQ; nQ++)
 #pragma unroll
-    for (int v = 0; v < 9; v++) st[v] = src(p, v);
-    const double vx = st[1] / st[0], vy = st[2] / st[0], vz = st[3] / st[0];
-    const double uDb = vx * st[5] + vy * st[6] + vz * st[7];
-    const double f0 = 0.0;
-    src(p, 0) = f0;
-    src(p, 1) = f0 - divB * st[5];
-    src(p, 2) = f0 - divB * st[6];
-    src(p, 3) = f0 - divB * st[7];
-    const double eForce = divB * uDb + gpsiDu[p] * st[8];
-    src(p, 4) = f0 - eForce;
-    src(p, 5) = f0 - divB * vx;
-    src(p, 6) = f0 - divB * vy;
-    src(p, 7) = f0 - divB * vz;
-    src(p, 8) = -gpsiDu[p];
-  }
+    for (int nR = 0; nR < NQ; nR++)
+#pragma unroll
+      for (int nS = 0; nS < NZ; nS++) {
+        const int p = (nQ * NQ + nR) * NZ + nS;
+        const double w = D == 3 ? g.wt[nQ] * g.wt[nR] * g.wt[nS] : g.wt[nQ] * g.wt[nR];
+        double st[9];
+#pragma unroll
+        for (int v = 0; v < 9; v++) st[v] = src(p, v);
+        const double vx = st[1] / st[0], vy = st[2] / st[0], vz = st[3] / st[0];
+        const double uDb = vx * st[5] + vy * st[6] + vz * st[7];
+        const double f0 = 0.0;
+        double sv[9];
+        sv[0] = f0;
+        sv[1] = f0 - divB * st[5];
+        sv[2] = f0 - divB * st[6];
+        sv[3] = f0 - divB * st[7];
+        const double eForce = divB * uDb + gpsiDu[p] * st[8];
+        sv[4] = f0 - eForce;
+        sv[5] = f0 - divB * vx;
+        sv[6] = f0 - divB * vy;
+        sv[7] = f0 - divB * vz;
+        sv[8] = -gpsiDu[p];
+#pragma unroll
+        for (int v = 0; v < 9; v++) rhs[v] += w * sv[v];
+      }
 }
 
-// Source terms of every cell; each block also folds the per-cell
-// cleaning-speed maxima spd [maxLam | maxVel] into chBits (zeroed by the
-// caller) with one atomic each, for the flux kernel that follows.
-template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_sources(const double *__restrict__ U,
-                                                     const double *__restrict__ FS,
-                                                     double *__restrict__ SRC,
-                                                     const double *__restrict__ theta, Grid g,
-                                                     const double *__restrict__ Dm,
-                                                     const double *__restrict__ spd,
-                                                     unsigned long long *__restrict__ chBits) {
+// ------------------------------------------------------------ cleaning speed
+#ifdef KFVM_SMALL_KERNELS
+// The per-cell maxima spd [maxLam | maxVel] folded into chBits (zeroed by the
+// caller) with one atomic per block each; the flux kernel reads
+// c_h = max(c_f + |u|) - max(|u|) from them.
+__global__ __launch_bounds__(BLK) void k_mhd_chmax(const double *__restrict__ spd, int64_t ncell,
+                                                   unsigned long long *__restrict__ chBits) {
   __shared__ double sh[BLK];
-  const int c = blockIdx.x * BLK + threadIdx.x;
   double lam = 0.0, vel = 0.0;
-  if (c < g.ncell) {
-    mhd_sources_cell<D, NQ>(U, FS, SRC, theta, g, Dm, c);
-    lam = spd[c];
-    vel = spd[size_t(g.ncell) + c];
+  for (int64_t c = int64_t(blockIdx.x) * BLK + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * BLK) {
+    const double l = spd[c], v = spd[ncell + c];
+    lam = l > lam ? l : lam;
+    vel = v > vel ? v : vel;
   }
   const double bl = block_max(lam, sh, threadIdx.x);
   const double bv = block_max(vel, sh, threadIdx.x);
@@ -687,6 +680,7 @@ __global__ __launch_bounds__(BLK) void k_mhd_sources(const double *__restrict__ 
     if (bv > 0.0) atomicMax(&chBits[1], (unsigned long long)__double_as_longlong(bv));
   }
 }
+#endif  // KFVM_SMALL_KERNELS
 
 }  // namespace dev
 }  // namespace kfvm

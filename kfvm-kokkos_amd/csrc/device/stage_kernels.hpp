@@ -690,6 +690,14 @@ __device__ inline void face_state(const double *__restrict__ FS, const Grid &g, 
     out[v] = FS[fs_base(cell, Shape<D, NQ>::F * Q * NV) + (size_t(f * Q + q) * NV + v) * FSB];
 }
 
+// GLM-MHD source terms of one cell integrated into rhs (defined with the
+// GLM-MHD kernels).
+template <int D, int NQ>
+__device__ void mhd_source_rhs(const double *__restrict__ U, const double *__restrict__ FS,
+                               const double *__restrict__ SRC, const double *__restrict__ theta, const Grid &g,
+                               const double *__restrict__ Dm, size_t c, int i, int j, int k, double divB,
+                               double *rhs);
+
 template <int D, int NQ, int NV>
 __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
                                                  const double *__restrict__ FS, Grid g,
@@ -697,6 +705,14 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
   using Bk = Brick<D>;
   constexpr int Q = Shape<D, NQ>::Q;
   __shared__ double fl[NV][Bk::NF];
+  // GLM-MHD: normal field averaged over the two states of each face (div B)
+  __shared__ double bvf[NV == 9 ? Bk::NF : 1];
+  // GLM-MHD: this cell's div B terms of each face point (x, y, z), summed
+  // after the face loop in the source kernel's point order
+  constexpr int QT = NV == 9 ? Q : 1;
+  double divT[QT][3];
+#pragma unroll
+  for (int qq = 0; qq < QT; qq++) divT[qq][0] = divT[qq][1] = divT[qq][2] = 0.0;
   // compact WENO rows of the brick's cells and their face neighbours (Euler
   // sparse WENO; -1: states in FS)
   constexpr int RX = Bk::X + 2, RY = Bk::Y + 2, RZ = D == 3 ? Bk::Z + 2 : 1;
@@ -792,6 +808,7 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
       double Fl[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) Fl[v] = 0.0;
+      double bv = 0.0;
       // a face is needed when one of its cells is a live cell of this brick
       const bool rBrick = fi < Bk::X && fj < Bk::Y && fk < Bk::Z && rIn;
       const bool lBrick = (dir == 0 ? fi > 0 : (dir == 1 ? fj > 0 : fk > 0)) && lIn;
@@ -806,13 +823,21 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
           face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, uL, uR, row_of(ri, rj, rk), a.ws);
         }
         double sp;
-        if (dir == 0) sp = riemann<NV, 0>(g, ch, uL, uR, Fl);
-        else if (dir == 1) sp = riemann<NV, 1>(g, ch, uL, uR, Fl);
-        else sp = riemann<NV, 2>(g, ch, uL, uR, Fl);
+        if (dir == 0) {
+          sp = riemann<NV, 0>(g, ch, uL, uR, Fl);
+          if (NV == 9) bv = 0.5 * (uL[5] + uR[5]);
+        } else if (dir == 1) {
+          sp = riemann<NV, 1>(g, ch, uL, uR, Fl);
+          if (NV == 9) bv = 0.5 * (uL[6] + uR[6]);
+        } else {
+          sp = riemann<NV, 2>(g, ch, uL, uR, Fl);
+          if (NV == 9) bv = 0.5 * (uL[7] + uR[7]);
+        }
         vmax = sp > vmax ? sp : vmax;
       }
 #pragma unroll
       for (int v = 0; v < NV; v++) fl[v][t] = Fl[v];
+      if (NV == 9) bvf[t] = bv;
     }
     __syncthreads();
     if (live) {
@@ -838,24 +863,41 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
           rhs[v] += w * (fl[v][zB] - fl[v][zT]) * rdz;
         }
       }
+      if constexpr (NV == 9) {
+        // div B terms of point q as SourceTerms_K forms them (the face's
+        // average is the same from either side: the sum commutes)
+        const double twt = D == 2 ? g.wt[q] : g.wt[q % NQ] * g.wt[q / NQ];
+        const double tx = twt * (bvf[xE] - bvf[xW]) / g.dx;
+        const double ty = twt * (bvf[yN] - bvf[yS]) / g.dy;
+        const double tz = D == 3 ? twt * (bvf[zT] - bvf[zB]) / g.dz : 0.0;
+#pragma unroll
+        for (int qq = 0; qq < QT; qq++) {
+          divT[qq][0] = q == qq ? tx : divT[qq][0];
+          divT[qq][1] = q == qq ? ty : divT[qq][1];
+          divT[qq][2] = q == qq ? tz : divT[qq][2];
+        }
+      }
     }
     __syncthreads();
   }
 
-  if (live && NV == 9) {
-    // source terms at the interior points (IntegrateRHS_K)
+  if constexpr (NV == 9) {
+    if (live) {
+    // div B in the order of SourceTerms_K's point loop (MHD_GLM_K.H:316-547),
+    // then the source terms at the interior points, integrated (IntegrateRHS_K)
     constexpr int NZ = D == 3 ? NQ : 1;
-#pragma unroll 1
-    for (int nQ = 0; nQ < NQ; nQ++)
-#pragma unroll 1
-      for (int nR = 0; nR < NQ; nR++)
-#pragma unroll 1
-        for (int nS = 0; nS < NZ; nS++) {
-          const int p = (nQ * NQ + nR) * NZ + nS;
-          const double w = D == 3 ? g.wt[nQ] * g.wt[nR] * g.wt[nS] : g.wt[nQ] * g.wt[nR];
+    double divB = 0.0;
 #pragma unroll
-          for (int v = 0; v < NV; v++) rhs[v] += w * a.src[(size_t(p) * NV + v) * nc + c];
-        }
+    for (int nQ = 0; nQ < NQ; nQ++)
+#pragma unroll
+      for (int nS = 0; nS < NZ; nS++) {
+        const int q = nS * NQ + nQ;
+        divB += divT[q][0];
+        divB += divT[q][1];
+        if (D == 3) divB += divT[q][2];
+      }
+    mhd_source_rhs<D, NQ>(U, FS, a.src, a.theta, g, a.Dm, c, ci, cj, ck, divB, rhs);
+    }
   }
   const bool write = !a.sliceWrite || a.sliceWrite[slice] != 0;
   if (live) rk_epilogue<NV>(a, U, c, nc, rhs, betaDt, bhatDt, write, errLoc, posLoc);

@@ -653,8 +653,8 @@ bool Solver::profileGet(const std::string &name, double &ms, int64_t &count) {
 // GLM-MHD evaluation (Solver.cpp:410-530 with sources): reconstruction of
 // face and interior states with the limiter's fast test, selection, WENO
 // (converted back and limited), the limiter's exact sequence for the other
-// cells that failed the test, source terms with the cleaning speed, then the
-// Riemann / divergence / RK kernel.
+// cells that failed the test, the cleaning speed, then the Riemann /
+// divergence / source-term / RK kernel.
 void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
   const int dim = cfg_.dim, nq = cfg_.nQuad;
   const double *thPrev = theta_[th_];
@@ -697,12 +697,12 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
   }
   th_ ^= 1;
   check(launch_zero(chBits_, 2 * sizeof(unsigned long long), stream_), "clear speeds");
-  timed("sources", [&] {
-    return launch_mhd_sources(dim, nq, Uin, FS_, SRC_, thNew, grid_, diffM_, spd_, chBits_, stream_);
-  });
+  timed("chmax", [&] { return launch_mhd_chmax(dim, spd_, grid_, chBits_, stream_); });
   RKArgs a = a0;
   a.chBits = chBits_;
   a.src = SRC_;
+  a.theta = thNew;
+  a.Dm = diffM_;
   timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, a, stream_); });
   nRhsEval++;
 }
