@@ -89,7 +89,8 @@ def build(force=False, verbose=False):
     relink = force or jobs or not os.path.exists(LIB)
     if relink:
         tmp = LIB + ".tmp"
-        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lquadmath", "-ldl"])
+        _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
+             ["-lquadmath", "-ldl", "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
         os.replace(tmp, LIB)
     main_src = os.path.join(CSRC, "host", "main.cpp")
     if relink or _stale(EXE, main_src, deps):

@@ -17,12 +17,24 @@
 #include <stdexcept>
 #include <string>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "problems.hpp"
 #include "stencil_geom.hpp"
 
 namespace kfvm {
 
 namespace {
+// A named host range for rocprofv3 --marker-trace, one per region of the
+// reference's Kokkos profiling hooks (Solver.cpp:85 Solver::Solve, 192
+// Solver::TakeStep, 411 Solver::evalRHS, 1949 Solver::setIC).
+struct Region {
+  explicit Region(const char *name) { roctxRangePushA(name); }
+  ~Region() { roctxRangePop(); }
+  Region(const Region &) = delete;
+  Region &operator=(const Region &) = delete;
+};
+
 void check(hipError_t e, const char *what) {
   if (e != hipSuccess)
     throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -217,6 +229,8 @@ void Solver::setup() {
 
   const bool restart = !cfg_.restartFile.empty();
   if (cfg_.plotFreq > 0.0 || cfg_.ckptFreq > 0 || restart) writer_.reset(new Writer(cfg_));
+  {
+  Region icRegion("Solver::setIC");
   if (!restart && slices_ > 1) {
     // one initial condition per slice, each with its own seed
     std::vector<double> U0(size_t(ncell_) * nv_), one;
@@ -246,6 +260,7 @@ void Solver::setup() {
     if (!d.theta.empty())
       check(hipMemcpy(theta_[th_], d.theta.data(), nc * sizeof(double), hipMemcpyHostToDevice), "restore theta");
     plotNum = cfg_.plotFreq > 0.0 ? int(std::ceil(time / cfg_.plotFreq)) : 0;
+  }
   }
   useSparseWeno = cfg_.wenoThresh > 0.0;
   if (!restart) {
@@ -708,6 +723,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
 }
 
 void Solver::pipeline(const double *Uin, const RKArgs &a) {
+  Region region("Solver::evalRHS");
   if (nv_ == 9) return pipelineMhd(Uin, a);
   const int dim = cfg_.dim, nq = cfg_.nQuad;
   const bool hydro = nv_ == 5;
@@ -949,6 +965,7 @@ double Solver::evalRhsNoAlloc(const double *U, double *rhs) {
 }
 
 int Solver::takeStep() {
+  Region region("Solver::TakeStep");
   if (slices_ > 1) return takeStepSlices();
   const bool fixed = cfg_.fixedDt > 0.0;
   if (fixed) dt = cfg_.fixedDt;
@@ -1198,6 +1215,7 @@ void Solver::advanceStages(int64_t n, double dtFixed) {
 
 int64_t Solver::solve() {
   // Solver::Solve (Solver.cpp:84-140): step, then the outputs that are due
+  Region region("Solver::Solve");
   int64_t n = 0;
   for (; nTS < cfg_.maxSteps && !lastTimeStep; n++) {
     const int64_t step = nTS;  // the reference advances its counter after the outputs

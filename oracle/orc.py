@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "liborc.so")
+# KFVM_ORACLE_LIB: another build of the same source (the sanitizer build of
+# tests/test_oracle_sanitized.py)
+LIB_PATH = os.environ.get("KFVM_ORACLE_LIB") or os.path.join(HERE, "build", "liborc.so")
 
 EQ = {"hydro": 0, "linadv": 1, "mhd_glm": 2}
 RS = {"llf": 0, "hll": 1, "hllc": 2, "exact": 3, "kepes": 4}
