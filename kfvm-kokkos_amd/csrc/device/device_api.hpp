@@ -183,9 +183,12 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
 // then [SUB][I] derivative weights (SUB = C + Cc + B*Cb, substencils back to
 // back; compact_weno_weights), DWS is not read.  WS (Euler, may be null): compact staging
 // [(F*Q*5)][cap] for the first cap listed cells, shared with launch_weno_post.
-// Compact Euler WENO weight table (see launch_weno) from the reference layouts
-// face [nSub][F*Q][C] and deriv [nSub][I][C]; out has SUB*(F*Q + I) doubles.
-void compact_weno_weights(int dim, int fq, int nI, const double *face, const double *deriv, double *out);
+// Compact WENO weight table (see launch_weno) from the reference layouts
+// face [nSub][F*Q][C] and deriv [nSub][I][C]; out has SUB*(F*Q + I) doubles,
+// then, with nCell > 0 (GLM-MHD), SUB*nCell rows of the interior-point
+// weights cell [nSub][nCell][C].
+void compact_weno_weights(int dim, int fq, int nI, const double *face, const double *deriv, double *out,
+                          int nCell = 0, const double *cell = nullptr);
 hipError_t launch_weno(int dim, int nq, int nv, const double *U, double *FS, double *thetaNew,
                        const Grid &g, const double *FW, const double *DW, const double *FWS,
                        const double *DWS, const int *list, const int *counter, double *WS, int cap,
@@ -214,12 +217,14 @@ hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI,
                              double *wflag, int *list, int *counter, double wThresh, const Grid &g,
                              hipStream_t s);
 // FT, DT, CT: stencil-major face [nSub][C][F*Q], smoothness [nSub][C][I] and
-// interior [nSub][C][NQ^D] weights.  The listed cells (list null: every cell)
-// are converted back and limited (theta, spd).
+// interior [nSub][C][NQ^D] weights; WC: the compact table of the same weights
+// (compact_weno_weights with the interior rows), read instead of them where it
+// fits in LDS.  The listed cells (list null: every cell) are converted back
+// and limited (theta, spd).
 hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
                            const double *FT, const double *DT, const double *CT, const int *list,
                            const int *counter, double *thetaNew, double *spd, const double *props,
-                           hipStream_t s);
+                           const double *WC, hipStream_t s);
 // The limiter's exact sequence on the listed cells whose WENO flag is 0.
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
                          double *spd, const double *props, const double *wflag, const Grid &g, const int *list,

@@ -58,7 +58,8 @@ hipError_t launch_select(int dim, int nq, const double *U, const double *WI, con
   KFVM_SELECT(dim, nq, L::select(U, WI, theta, wflag, list, counter, wThresh, g, pos, cap, s));
 }
 
-void compact_weno_weights(int dim, int fq, int nI, const double *face, const double *deriv, double *out) {
+void compact_weno_weights(int dim, int fq, int nI, const double *face, const double *deriv, double *out,
+                          int nCell, const double *cell) {
   const int C = dim == 3 ? SInfo<3>::C : SInfo<2>::C, Cc = dim == 3 ? SInfo<3>::Cc : SInfo<2>::Cc;
   const int Cb = dim == 3 ? SInfo<3>::Cb : SInfo<2>::Cb, nSub = dim == 3 ? SInfo<3>::nSub : SInfo<2>::nSub;
   const int SUB = C + Cc + (nSub - 2) * Cb;
@@ -68,6 +69,8 @@ void compact_weno_weights(int dim, int fq, int nI, const double *face, const dou
     for (int m = 0; m < ns; m++, row++) {
       for (int p = 0; p < fq; p++) out[size_t(row) * fq + p] = face[(size_t(sb) * fq + p) * C + m];
       for (int i = 0; i < nI; i++) out[size_t(SUB) * fq + size_t(row) * nI + i] = deriv[(size_t(sb) * nI + i) * C + m];
+      for (int p = 0; p < nCell; p++)
+        out[size_t(SUB) * (fq + nI) + size_t(row) * nCell + p] = cell[(size_t(sb) * nCell + p) * C + m];
     }
   }
 }
@@ -105,8 +108,8 @@ hipError_t launch_mhd_select(int dim, int nq, const double *U, const double *WI,
 hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double *SRC, const Grid &g,
                            const double *FT, const double *DT, const double *CT, const int *list,
                            const int *counter, double *thetaNew, double *spd, const double *props,
-                           hipStream_t s) {
-  KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FT, DT, CT, list, counter, thetaNew, spd, props, s));
+                           const double *WC, hipStream_t s) {
+  KFVM_SELECT(dim, nq, L::mhd_weno(U, FS, SRC, g, FT, DT, CT, list, counter, thetaNew, spd, props, WC, s));
 }
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
                          double *spd, const double *props, const double *wflag, const Grid &g, const int *list,

@@ -123,10 +123,13 @@ struct Launch {
   }
   static hipError_t mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FT,
                              const double *DT, const double *CT, const int *list, const int *counter,
-                             double *thetaNew, double *spd, const double *props, hipStream_t s) {
-    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::MCPB - 1) / dev::MCPB), 1u << 16);
+                             double *thetaNew, double *spd, const double *props, const double *WC, hipStream_t s) {
+    // weights in LDS: persistent (every workgroup stages the table once, then strides over the list)
+    constexpr bool lds = dev::MWenoLds<D, NQ>::USE;
+    if (lds && !WC) return hipErrorInvalidValue;
+    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::MCPB - 1) / dev::MCPB), lds ? 2048u : 1u << 16);
     hipLaunchKernelGGL((dev::k_mhd_wenorec<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, SRC, g, FT, DT, CT,
-                       list, counter);
+                       list, counter, WC);
     const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
     hipLaunchKernelGGL((dev::k_mhd_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, g, list,
                        counter, thetaNew, spd, props);

@@ -137,6 +137,10 @@ __device__ inline void mhd_face_speed(double gamma, const double *s, double &max
 //   - the cleaning-speed maxima of the face states (SpeedEstimate_K) into the
 //     per-cell plane spd [maxLam | maxVel]; cells that the limiter or WENO
 //     change later overwrite theirs.
+// The learned selection stays a kernel of its own here (k_learned_select):
+// run on this 138 KB brick, which leaves 8 waves per CU, the network's
+// matrix-core phase cost more than the separate kernel's re-staging of
+// density and pressure.
 template <int D, int NQ>
 __global__ __launch_bounds__(MBox<D>::NT, 2) void k_mhd_linear(
     const double *__restrict__ U, double *__restrict__ FS, double *__restrict__ SRC, double *__restrict__ WI,
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(MBox<D>::NT, 2) void k_mhd_linear(
   __syncthreads();
   const int tx = tid % Bx::BX, ty = (tid / Bx::BX) % Bx::BY, tz = tid / (Bx::BX * Bx::BY);
   const int i = i0 + tx, j = j0 + ty, k = k0 + tz;
-  if (!(i < g.nx && j < g.ny && (D == 2 || k < g.nz))) return;  // no barrier below
+  if (i < g.nx && j < g.ny && (D == 2 || k < g.nz)) {  // no barrier inside
   const int cell = (k * g.ny + j) * g.nx + i;
   const int corner = (tz * Bx::LY + ty) * Bx::LX + tx;  // stencil box corner (-H,-H,-H)
   const int c0 = corner + H + Bx::LX * H + (D == 3 ? Bx::LX * Bx::LY * H : 0);
@@ -234,6 +238,7 @@ __global__ __launch_bounds__(MBox<D>::NT, 2) void k_mhd_linear(
     base = __shfl(base, leader);
     if (slow) ppList[base + __popcll(mask & ((1ull << lane) - 1ull))] = cell;
   }
+  }
 }
 
 // KFVM-WENO of the listed cells (list == nullptr: all), one lane per (cell,
@@ -242,9 +247,12 @@ __global__ __launch_bounds__(MBox<D>::NT, 2) void k_mhd_linear(
 // the WENO-AO weights, and writes the weighted face and interior values, still
 // linearised primitive (k_mhd_weno_post converts them back).  The products
 // walk the stencil in their outer loop with one accumulator per output (the
-// Euler kernel's pieces: WenoSubs, stencil-major weights FT [nSub][C][F*Q],
-// DT [nSub][C][I], CT [nSub][C][NQ^D]), so each output still sums the stencil
-// cells in ascending order.
+// Euler kernel's pieces: WenoSubs over the compact table WC in LDS, or the
+// stencil-major tables FT [nSub][C][F*Q], DT [nSub][C][I], CT [nSub][C][NQ^D]
+// by scalar loads; see MWenoLds), so each output still sums the stencil cells
+// in ascending order.  The grid is persistent when the table is in LDS (at
+// most 2048 workgroups striding over the list), so the staging is paid once
+// per workgroup.
 constexpr int MLPC = 9;                  // lanes per cell
 constexpr int MCPW = 7;                  // cells per wave
 constexpr int MCPB = MCPW * (BLK / 64);  // cells per block
@@ -255,6 +263,20 @@ struct MhdChunk {  // interior points per accumulation chunk
   static constexpr int H = NI > 12 ? 9 : NI;
 };
 
+// The compact weight table of the GLM-MHD WENO kernel: the Euler kernel's
+// face rows [SUB][F*Q] and derivative rows [SUB][I], then interior-point rows
+// [SUB][NQ^D] (compact_weno_weights).  Staged in LDS once per workgroup
+// where it fits in 64 KB (the scalar cache cannot hold it: 43 KB in 3D at
+// NQ = 2); the 3D NQ = 3 table (85 KB) stays with the stencil-major tables
+// read by scalar loads.
+template <int D, int NQ>
+struct MWenoLds {
+  static constexpr int SUB = WenoLds<D, NQ>::SUB, FQ = MShape<D, NQ>::FQ, I = SInfo<D>::I;
+  static constexpr int NI = MShape<D, NQ>::NI;
+  static constexpr int NDER = SUB * FQ, NCELL = SUB * (FQ + I), NW = SUB * (FQ + I + NI);
+  static constexpr bool USE = NW * int(sizeof(double)) <= 65536;
+};
+
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ U,
                                                      double *__restrict__ FS,
@@ -263,7 +285,8 @@ __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ 
                                                      const double *__restrict__ DT,
                                                      const double *__restrict__ CT,
                                                      const int *__restrict__ list,
-                                                     const int *__restrict__ counter) {
+                                                     const int *__restrict__ counter,
+                                                     const double *__restrict__ WC) {
   constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = SInfo<D>::C, nSub = SInfo<D>::nSub;
   constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
@@ -271,9 +294,16 @@ __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ 
   static_assert(FQ % HF == 0 && NI % HI == 0, "chunks must tile the points");
   constexpr double gHi = 0.7, gLo = 0.7;
   constexpr double g0 = gHi, g1 = (1.0 - gHi) * gLo, gb = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
+  using WL = MWenoLds<D, NQ>;
+  __shared__ double wl[WL::USE ? WL::NW : 1];
+  if constexpr (WL::USE) {
+    static_assert(WL::SUB == SubOff<D, nSub>::value, "compact rows of all substencils");
+    for (int e = threadIdx.x; e < WL::NW; e += BLK) wl[e] = WC[e];
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
   const int slot = lane / MLPC;
-  if (slot >= MCPW) return;  // lane 63 idles
+  if (slot >= MCPW) return;  // lane 63 idles; no barrier follows
   const int v = lane - MLPC * slot;
   const int n = list ? *counter : g.ncell;
   const int waveId = blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
@@ -310,13 +340,17 @@ __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ 
         vs[m] = sv;
       }
     }
-    // weight tables opaque to loop-invariant code motion (see k_linear_fused)
-    const double *dt0 = DT, *ft0 = FT, *ct0 = CT;
-    asm volatile("" : "+s"(dt0), "+s"(ft0), "+s"(ct0));
-    const cdouble *dt = (const cdouble *)dt0;
     // smoothness indicators and WENO-AO weights
     double om[nSub];
-    WenoSubs<D, 0, nSub>::beta(vs, dt, om);
+    const double *ft0 = FT, *ct0 = CT;
+    if constexpr (WL::USE) {
+      WenoSubs<D, 0, nSub, true>::beta(vs, (const double *)(wl + WL::NDER), om);
+    } else {
+      // weight tables opaque to loop-invariant code motion (see k_linear_fused)
+      const double *dt0 = DT;
+      asm volatile("" : "+s"(dt0), "+s"(ft0), "+s"(ct0));
+      WenoSubs<D, 0, nSub>::beta(vs, (const cdouble *)dt0, om);
+    }
     double tot = 0.0;
 #pragma unroll
     for (int s = 0; s < nSub; s++) {
@@ -332,23 +366,31 @@ __global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ 
     // face points, then interior points, a chunk at a time
 #pragma unroll 1
     for (int h0 = 0; h0 < FQ; h0 += HF) {
-      const double *fc0 = ft0 + h0;
-      asm volatile("" : "+s"(fc0));
       double out[HF];
 #pragma unroll
       for (int h = 0; h < HF; h++) out[h] = 0.0;
-      WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, (const cdouble *)fc0, om, out);
+      if constexpr (WL::USE) {
+        WenoSubs<D, 0, nSub, true>::template faces<HF, FQ>(vs, (const double *)(wl + h0), om, out);
+      } else {
+        const double *fc0 = ft0 + h0;
+        asm volatile("" : "+s"(fc0));
+        WenoSubs<D, 0, nSub>::template faces<HF, FQ>(vs, (const cdouble *)fc0, om, out);
+      }
 #pragma unroll
       for (int h = 0; h < HF; h++) FS[fs_base(c, FQ * 9) + (size_t(h0 + h) * 9 + v) * FSB] = out[h];
     }
 #pragma unroll 1
     for (int p0 = 0; p0 < NI; p0 += HI) {
-      const double *cc0 = ct0 + p0;
-      asm volatile("" : "+s"(cc0));
       double out[HI];
 #pragma unroll
       for (int h = 0; h < HI; h++) out[h] = 0.0;
-      WenoSubs<D, 0, nSub>::template faces<HI, NI>(vs, (const cdouble *)cc0, om, out);
+      if constexpr (WL::USE) {
+        WenoSubs<D, 0, nSub, true>::template faces<HI, NI>(vs, (const double *)(wl + WL::NCELL + p0), om, out);
+      } else {
+        const double *cc0 = ct0 + p0;
+        asm volatile("" : "+s"(cc0));
+        WenoSubs<D, 0, nSub>::template faces<HI, NI>(vs, (const cdouble *)cc0, om, out);
+      }
 #pragma unroll
       for (int h = 0; h < HI; h++) SRC[(size_t(p0 + h) * 9 + v) * nc + c] = out[h];
     }

@@ -825,19 +825,19 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
         double sp;
         if (dir == 0) {
           sp = riemann<NV, 0>(g, ch, uL, uR, Fl);
-          if (NV == 9) bv = 0.5 * (uL[5] + uR[5]);
+          if constexpr (NV == 9) bv = 0.5 * (uL[5] + uR[5]);
         } else if (dir == 1) {
           sp = riemann<NV, 1>(g, ch, uL, uR, Fl);
-          if (NV == 9) bv = 0.5 * (uL[6] + uR[6]);
+          if constexpr (NV == 9) bv = 0.5 * (uL[6] + uR[6]);
         } else {
           sp = riemann<NV, 2>(g, ch, uL, uR, Fl);
-          if (NV == 9) bv = 0.5 * (uL[7] + uR[7]);
+          if constexpr (NV == 9) bv = 0.5 * (uL[7] + uR[7]);
         }
         vmax = sp > vmax ? sp : vmax;
       }
 #pragma unroll
       for (int v = 0; v < NV; v++) fl[v][t] = Fl[v];
-      if (NV == 9) bvf[t] = bv;
+      if constexpr (NV == 9) bvf[t] = bv;
     }
     __syncthreads();
     if (live) {

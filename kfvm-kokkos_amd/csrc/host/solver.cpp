@@ -187,10 +187,11 @@ void Solver::setup() {
   faceWT_ = dalloc<double>(size_t(wts_.F) * wts_.Q * wts_.C, "face weights (pass-major)");
   faceWS_ = dalloc<double>(wts_.face.size(), "face weights (stencil-major)");
   derivWS_ = dalloc<double>(wts_.deriv.size(), "deriv weights (stencil-major)");
-  if (nv_ == 5) {
+  if (nv_ == 5 || nv_ == 9) {
     const int SUB = cfg_.dim == 3 ? SInfo<3>::C + SInfo<3>::Cc + SInfo<3>::B * SInfo<3>::Cb
                                   : SInfo<2>::C + SInfo<2>::Cc + SInfo<2>::B * SInfo<2>::Cb;
-    faceWC_ = dalloc<double>(size_t(SUB) * (FQ + wts_.I), "WENO weights (compact)");
+    const size_t cellRows = nv_ == 9 ? size_t(wts_.nqCell) : 0;
+    faceWC_ = dalloc<double>(size_t(SUB) * (FQ + wts_.I + cellRows), "WENO weights (compact)");
   }
   nPart_ = flux_rk_partials(cfg_.dim, grid_);
   errPart_ = dalloc<double>(nPart_, "err partials");
@@ -697,7 +698,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
     }
     timed("weno", [&] {
       return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, list_, counter_, thNew,
-                             spd_, props_, stream_);
+                             spd_, props_, faceWC_, stream_);
     });
     timed("pp", [&] {
       return launch_mhd_pp(dim, nq, Uin, FS_, SRC_, thNew, spd_, props_, wflag_, grid_, ppList_, counter_ + 1,
@@ -706,7 +707,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
   } else {
     timed("weno", [&] {
       return launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr, thNew,
-                             spd_, props_, stream_);
+                             spd_, props_, faceWC_, stream_);
     });
     useSparseWeno = cfg_.wenoThresh > 0.0;
   }
@@ -1141,7 +1142,7 @@ void Solver::collectSamples(float *X, float *Y, double tau) {
           "mhd_linear");
     check(hipMemcpyAsync(FSlin_, FS_, fsBytes, hipMemcpyDeviceToDevice, stream_), "copy linear states");
     check(launch_mhd_weno(dim, nq, Uin, FS_, SRC_, grid_, faceWS_, derivWS_, cellWS_, nullptr, nullptr, thScratch,
-                          spd_, props_, stream_),
+                          spd_, props_, faceWC_, stream_),
           "mhd_weno");
     check(launch_labels(dim, nq, 9, Uin, FSlin_, FS_, float(tau), Y, grid_, stream_), "labels");
     check(launch_features(dim, 9, Uin, nullptr, X, grid_, stream_), "features");
@@ -1296,8 +1297,10 @@ void Solver::uploadWeights() {
   if (faceWC_) {
     const int SUB = wts_.dim == 3 ? SInfo<3>::C + SInfo<3>::Cc + SInfo<3>::B * SInfo<3>::Cb
                                   : SInfo<2>::C + SInfo<2>::Cc + SInfo<2>::B * SInfo<2>::Cb;
-    std::vector<double> wc(size_t(SUB) * (FQ + I));
-    compact_weno_weights(wts_.dim, FQ, I, wts_.face.data(), wts_.deriv.data(), wc.data());
+    const int nCell = nv_ == 9 ? wts_.nqCell : 0;
+    std::vector<double> wc(size_t(SUB) * (FQ + I + nCell));
+    compact_weno_weights(wts_.dim, FQ, I, wts_.face.data(), wts_.deriv.data(), wc.data(), nCell,
+                         nCell ? wts_.cell.data() : nullptr);
     check(hipMemcpy(faceWC_, wc.data(), wc.size() * sizeof(double), hipMemcpyHostToDevice), "copy weights");
   }
   if (faceWQ_) {  // full-stencil face weights per point: [Q][C][F]
