@@ -1,7 +1,8 @@
 // Run-time selection of the kernel instantiation and the shape-independent
 // small kernels (RK stage start, reductions, fills).
 #define KFVM_SMALL_KERNELS
-#include "launch_impl.hpp"
+#include "launch_decl.hpp"
+#include "stage_kernels.hpp"
 
 namespace kfvm {
 

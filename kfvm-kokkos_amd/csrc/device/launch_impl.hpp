@@ -1,222 +1,261 @@
-// Per-(dimension, face-rule) launchers of the stage kernels.  Each
-// instantiation lives in its own translation unit (inst_*.hip) so the kernel
-// variants compile in parallel; dispatch.hip selects one at run time.
+// Definitions of the per-(dimension, face-rule) launchers declared in
+// launch_decl.hpp; included by the instantiation units inst_*.hip only.
 #pragma once
+
+#include "launch_decl.hpp"
 
 #include "stage_kernels.hpp"
 
 namespace kfvm {
 
-inline unsigned stage_blocks(int64_t n) { return unsigned((n + dev::BLK - 1) / dev::BLK); }
+template <int D, int NQ>
+unsigned Launch<D, NQ>::bricks(const Grid &g) {
+  using Bx = dev::RBox<D>;
+  return unsigned(((g.nx + Bx::BX - 1) / Bx::BX) * ((g.ny + Bx::BY - 1) / Bx::BY) *
+                  ((g.nz + Bx::BZ - 1) / Bx::BZ));
+}
 
 template <int D, int NQ>
-struct Launch {
-  static unsigned bricks(const Grid &g) {
-    using Bx = dev::RBox<D>;
-    return unsigned(((g.nx + Bx::BX - 1) / Bx::BX) * ((g.ny + Bx::BY - 1) / Bx::BY) *
-                    ((g.nz + Bx::BZ - 1) / Bx::BZ));
+hipError_t Launch<D, NQ>::cell_props(const double *U, double *props, const Grid &g, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_cell_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::linear(int nv, const double *U, double *FS, double *WI, double *thetaNew,
+                                 const double *props, const Grid &g, const double *W0, const double *W0T,
+                                 int *ppList, int *ppCount, hipStream_t s, const LearnArgs *la) {
+  if (nv == 5 && la)
+    hipLaunchKernelGGL((dev::k_linear_fused<D, NQ, false, true>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s,
+                       U, FS, WI, thetaNew, props, g, W0T, ppList, ppCount, nullptr, Surf{}, *la);
+  else if (nv == 5)
+    hipLaunchKernelGGL((dev::k_linear_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, FS, WI,
+                       thetaNew, props, g, W0T, ppList, ppCount);
+  else
+    hipLaunchKernelGGL((dev::k_linear<D, NQ, 1>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, WI,
+                       thetaNew, g, W0);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::select(const double *U, const double *WI, const double *theta, double *wflag,
+                                 int *list, int *counter, double wThresh, const Grid &g, int *pos, int cap,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_select<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, WI, theta, wflag,
+                     list, counter, wThresh, g, pos, cap);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::weno(int nv, const double *U, double *FS, double *thetaNew, const Grid &g,
+                               const double *FW, const double *DW, const int *list, const int *counter,
+                               double *WS, int cap, const double *WC, hipStream_t s) {
+  if (nv == 5) {
+    // persistent: every workgroup stages the weights once, then strides over the list
+    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 2048u);
+    if (!WC) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list, counter,
+                       WS, WS ? cap : 0);
+  } else {
+    hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
+                       dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
   }
-  static hipError_t cell_props(const double *U, double *props, const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_cell_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
-    return hipGetLastError();
-  }
-  static hipError_t linear(int nv, const double *U, double *FS, double *WI, double *thetaNew,
-                           const double *props, const Grid &g, const double *W0, const double *W0T,
-                           int *ppList, int *ppCount, hipStream_t s, const LearnArgs *la) {
-    if (nv == 5 && la)
-      hipLaunchKernelGGL((dev::k_linear_fused<D, NQ, false, true>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s,
-                         U, FS, WI, thetaNew, props, g, W0T, ppList, ppCount, nullptr, Surf{}, *la);
-    else if (nv == 5)
-      hipLaunchKernelGGL((dev::k_linear_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, FS, WI,
-                         thetaNew, props, g, W0T, ppList, ppCount);
-    else
-      hipLaunchKernelGGL((dev::k_linear<D, NQ, 1>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FS, WI,
-                         thetaNew, g, W0);
-    return hipGetLastError();
-  }
-  static hipError_t select(const double *U, const double *WI, const double *theta, double *wflag,
-                           int *list, int *counter, double wThresh, const Grid &g, int *pos, int cap,
-                           hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_select<D, NQ>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, WI, theta, wflag,
-                       list, counter, wThresh, g, pos, cap);
-    return hipGetLastError();
-  }
-  static hipError_t weno(int nv, const double *U, double *FS, double *thetaNew, const Grid &g,
-                         const double *FW, const double *DW, const int *list, const int *counter,
-                         double *WS, int cap, const double *WC, hipStream_t s) {
-    if (nv == 5) {
-      // persistent: every workgroup stages the weights once, then strides over the list
-      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 2048u);
-      if (!WC) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list, counter,
-                         WS, WS ? cap : 0);
-    } else {
-      hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
-                         dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
-    }
-    return hipGetLastError();
-  }
-  // ---- fused path (Euler, sparse WENO)
-  static hipError_t linear_surf(const double *U, double *WI, double *thetaNew, const double *props,
-                                const Grid &g, const double *W0T, int *ppList, int *ppCount, double *SB,
-                                const Surf &sb, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_linear_fused<D, NQ, true>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U,
-                       nullptr, WI, thetaNew, props, g, W0T, ppList, ppCount, SB, sb);
-    return hipGetLastError();
-  }
-  static hipError_t pp_rows(const double *U, double *rows, int *pos, double *thetaNew, const double *props,
-                            const double *wflag, const Grid &g, const double *W0, const int *list,
-                            const int *count, const int *wenoCount, int *rowCount, hipStream_t s) {
-    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 4096u);
-    hipLaunchKernelGGL((dev::k_pp_rows<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, rows, pos, thetaNew, props,
-                       wflag, g, W0, list, count, wenoCount, rowCount);
-    return hipGetLastError();
-  }
-  static hipError_t flux_fused(const double *U, const double *WQ, const double *rows, const int *pos,
-                               const double *SB, const Surf &sb, const Grid &g, const RKArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_flux_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, WQ, rows, pos,
-                       SB, sb, g, a);
-    return hipGetLastError();
-  }
-  static hipError_t pp_lanes(const double *U, double *FS, double *thetaNew, const double *props,
-                             const double *wflag, const Grid &g, const int *list, const int *count,
-                             hipStream_t s) {
-    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 4096u);
-    hipLaunchKernelGGL((dev::k_pp_lanes<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, wflag, g,
-                       list, count);
-    return hipGetLastError();
-  }
-  static hipError_t weno_post(const double *U, double *FS, double *thetaNew, const double *props,
-                              const Grid &g, const int *list, const int *counter, double *WS, int cap,
-                              hipStream_t s) {
-    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
-    if (list)
-      hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
-                         counter, WS, WS ? cap : 0);
-    else
-      hipLaunchKernelGGL((dev::k_weno_post_cells<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g);
-    return hipGetLastError();
-  }
-  // ---- GLM-MHD
-  static unsigned mhd_bricks(const Grid &g) {
-    using Bx = dev::MBox<D>;
-    return unsigned(((g.nx + Bx::BX - 1) / Bx::BX) * ((g.ny + Bx::BY - 1) / Bx::BY) *
-                    ((g.nz + Bx::BZ - 1) / Bx::BZ));
-  }
-  static hipError_t mhd_props(const double *U, double *props, const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_mhd_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
-    return hipGetLastError();
-  }
-  static hipError_t mhd_linear(const double *U, double *FS, double *SRC, double *WI, double *thetaNew, double *spd,
-                               const double *props, const Grid &g, const double *WT, int *ppList, int *ppCount,
-                               hipStream_t s) {
-    // WI null: the learned selection does not read the KXRCF indicator
-    hipLaunchKernelGGL((dev::k_mhd_linear<D, NQ>), dim3(mhd_bricks(g)), dim3(dev::MBox<D>::NT), 0, s, U, FS, SRC,
-                       WI, thetaNew, spd, props, g, WT, ppList, ppCount);
-    return hipGetLastError();
-  }
-  static hipError_t mhd_select(const double *U, const double *WI, const double *theta, double *wflag,
-                               int *list, int *counter, double wThresh, const Grid &g, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_select<D, NQ, dev::Mhd>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, WI,
-                       theta, wflag, list, counter, wThresh, g, nullptr, 0);
-    return hipGetLastError();
-  }
-  static hipError_t mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FT,
-                             const double *DT, const double *CT, const int *list, const int *counter,
-                             double *thetaNew, double *spd, const double *props, const double *WC, hipStream_t s) {
-    // weights in LDS: persistent (every workgroup stages the table once, then strides over the list)
-    constexpr bool lds = dev::MWenoLds<D, NQ>::USE;
-    if (lds && !WC) return hipErrorInvalidValue;
-    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::MCPB - 1) / dev::MCPB), lds ? 2048u : 1u << 16);
-    hipLaunchKernelGGL((dev::k_mhd_wenorec<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, SRC, g, FT, DT, CT,
-                       list, counter, WC);
-    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::linear_surf(const double *U, double *WI, double *thetaNew, const double *props,
+                                      const Grid &g, const double *W0T, int *ppList, int *ppCount, double *SB,
+                                      const Surf &sb, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_linear_fused<D, NQ, true>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U,
+                     nullptr, WI, thetaNew, props, g, W0T, ppList, ppCount, SB, sb);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::pp_rows(const double *U, double *rows, int *pos, double *thetaNew, const double *props,
+                                  const double *wflag, const Grid &g, const double *W0, const int *list,
+                                  const int *count, const int *wenoCount, int *rowCount, hipStream_t s) {
+  const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 4096u);
+  hipLaunchKernelGGL((dev::k_pp_rows<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, rows, pos, thetaNew, props,
+                     wflag, g, W0, list, count, wenoCount, rowCount);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::flux_fused(const double *U, const double *WQ, const double *rows, const int *pos,
+                                     const double *SB, const Surf &sb, const Grid &g, const RKArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_flux_fused<D, NQ>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, WQ, rows, pos,
+                     SB, sb, g, a);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::pp_lanes(const double *U, double *FS, double *thetaNew, const double *props,
+                                   const double *wflag, const Grid &g, const int *list, const int *count,
+                                   hipStream_t s) {
+  const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 4096u);
+  hipLaunchKernelGGL((dev::k_pp_lanes<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, wflag, g,
+                     list, count);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::weno_post(const double *U, double *FS, double *thetaNew, const double *props,
+                                    const Grid &g, const int *list, const int *counter, double *WS, int cap,
+                                    hipStream_t s) {
+  const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
+  if (list)
+    hipLaunchKernelGGL((dev::k_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g, list,
+                       counter, WS, WS ? cap : 0);
+  else
+    hipLaunchKernelGGL((dev::k_weno_post_cells<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, thetaNew, props, g);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+unsigned Launch<D, NQ>::mhd_bricks(const Grid &g) {
+  using Bx = dev::MBox<D>;
+  return unsigned(((g.nx + Bx::BX - 1) / Bx::BX) * ((g.ny + Bx::BY - 1) / Bx::BY) *
+                  ((g.nz + Bx::BZ - 1) / Bx::BZ));
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::mhd_props(const double *U, double *props, const Grid &g, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_mhd_props<D>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, props, g);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::mhd_linear(const double *U, double *FS, double *SRC, double *WI, double *thetaNew,
+                                     double *spd, const double *props, const Grid &g, const double *WT,
+                                     int *ppList, int *ppCount, hipStream_t s) {
+  // WI null: the learned selection does not read the KXRCF indicator
+  hipLaunchKernelGGL((dev::k_mhd_linear<D, NQ>), dim3(mhd_bricks(g)), dim3(dev::MBox<D>::NT), 0, s, U, FS, SRC,
+                     WI, thetaNew, spd, props, g, WT, ppList, ppCount);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::mhd_select(const double *U, const double *WI, const double *theta, double *wflag,
+                                     int *list, int *counter, double wThresh, const Grid &g, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_select<D, NQ, dev::Mhd>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, WI,
+                     theta, wflag, list, counter, wThresh, g, nullptr, 0);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::mhd_weno(const double *U, double *FS, double *SRC, const Grid &g, const double *FT,
+                                   const double *DT, const double *CT, const int *list, const int *counter,
+                                   double *thetaNew, double *spd, const double *props, const double *WC,
+                                   hipStream_t s) {
+  // weights in LDS: persistent (every workgroup stages the table once, then strides over the list)
+  constexpr bool lds = dev::MWenoLds<D, NQ>::USE;
+  if (lds && !WC) return hipErrorInvalidValue;
+  const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::MCPB - 1) / dev::MCPB), lds ? 2048u : 1u << 16);
+  hipLaunchKernelGGL((dev::k_mhd_wenorec<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, SRC, g, FT, DT, CT,
+                     list, counter, WC);
+  const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1u << 16);
+  if (list)
     hipLaunchKernelGGL((dev::k_mhd_weno_post<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, g, list,
                        counter, thetaNew, spd, props);
-    return hipGetLastError();
-  }
-  static hipError_t mhd_pp(const double *U, double *FS, double *SRC, double *thetaNew, double *spd,
-                           const double *props, const double *wflag, const Grid &g, const int *list,
-                           const int *count, hipStream_t s) {
-    const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1024u);
-    hipLaunchKernelGGL((dev::k_mhd_pp<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, thetaNew, spd, props,
-                       wflag, g, list, count);
-    return hipGetLastError();
-  }
-  // ---- learned troubled-cell indicator
-  // nv 5 (Euler; eint = the limiter's internal-energy plane) or 9 (GLM-MHD;
-  // the internal energy comes from the conserved variables)
-  static hipError_t learned_select(int nv, const double *U, const double *eint, const double *theta,
-                                   const double *nadFlag, const float *P, float thr, double *wflag, int *list,
-                                   int *counter, const Grid &g, int *pos, int cap, hipStream_t s) {
-    if (nv == 9)
-      hipLaunchKernelGGL((dev::k_learned_select<D, 9>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
-                         theta, nadFlag, P, thr, wflag, list, counter, g, pos, cap);
-    else
-      hipLaunchKernelGGL((dev::k_learned_select<D, 5>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
-                         theta, nadFlag, P, thr, wflag, list, counter, g, pos, cap);
-    return hipGetLastError();
-  }
-  static hipError_t features(int nv, const double *U, const double *eint, float *X, const Grid &g, hipStream_t s) {
-    if (nv == 9)
-      hipLaunchKernelGGL((dev::k_features<D, 9>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
-    else
-      hipLaunchKernelGGL((dev::k_features<D, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
-    return hipGetLastError();
-  }
-  static hipError_t labels(int nv, const double *U, const double *FSL, const double *FSW, float tau, float *Y,
-                           const Grid &g, hipStream_t s) {
-    if (nv == 9)
-      hipLaunchKernelGGL((dev::k_labels<D, NQ, 9>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW,
-                         tau, Y, g);
-    else
-      hipLaunchKernelGGL((dev::k_labels<D, NQ, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW,
-                         tau, Y, g);
-    return hipGetLastError();
-  }
-  static hipError_t logits(const float *X, const float *P, float *Z, int n, hipStream_t s) {
-    hipLaunchKernelGGL((dev::k_logits<D>), dim3(stage_blocks(n)), dim3(dev::BLK), 0, s, X, P, Z, n);
-    return hipGetLastError();
-  }
-  static hipError_t train_grad(const float *X, const float *Y, const int *idx, int n, int nb, uint64_t seed,
-                               const float *P, float posW, float *part, float *lossPart, hipStream_t s) {
-    const unsigned nblk = unsigned((nb + dev::TB - 1) / dev::TB);
-    hipLaunchKernelGGL((dev::k_train_grad<D>), dim3(nblk), dim3(dev::TB), 0, s, X, Y, idx, n, nb, seed, P, posW,
-                       part, lossPart);
-    return hipGetLastError();
-  }
-  static hipError_t nad(int nv, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
-                        int *count, const Grid &g, hipStream_t s) {
-    if (nv == 9)
-      hipLaunchKernelGGL((dev::k_nad<D, 9>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
-                         nadFlag, count, g);
-    else
-      hipLaunchKernelGGL((dev::k_nad<D, 5>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
-                         nadFlag, count, g);
-    return hipGetLastError();
-  }
-  static dim3 flux_grid(const Grid &g) {  // one-dimensional: the kernel renumbers the bricks
-    using Bk = dev::Brick<D>;
-    return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X) * unsigned((g.ny + Bk::Y - 1) / Bk::Y) *
-                unsigned((g.nz + Bk::Z - 1) / Bk::Z));
-  }
-  static hipError_t flux_rk(int nv, const double *U, const double *FS, const Grid &g,
-                            const RKArgs &a, hipStream_t s) {
-    using Bk = dev::Brick<D>;
-    const dim3 blk(Bk::X, Bk::Y, Bk::Z);
-    if (nv == 5)
-      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 5>), flux_grid(g), blk, 0, s, U, FS, g, a);
-    else if (nv == 9)
-      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 9>), flux_grid(g), blk, 0, s, U, FS, g, a);
-    else
-      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 1>), flux_grid(g), blk, 0, s, U, FS, g, a);
-    return hipGetLastError();
-  }
-};
+  else
+    hipLaunchKernelGGL((dev::k_mhd_weno_post_cells<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, g,
+                       thetaNew, spd, props);
+  return hipGetLastError();
+}
 
-extern template struct Launch<2, 2>;
-extern template struct Launch<2, 3>;
-extern template struct Launch<3, 2>;
-extern template struct Launch<3, 3>;
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::mhd_pp(const double *U, double *FS, double *SRC, double *thetaNew, double *spd,
+                                 const double *props, const double *wflag, const Grid &g, const int *list,
+                                 const int *count, hipStream_t s) {
+  const unsigned np = std::min<unsigned>(stage_blocks(g.ncell), 1024u);
+  hipLaunchKernelGGL((dev::k_mhd_pp<D, NQ>), dim3(np), dim3(dev::BLK), 0, s, U, FS, SRC, thetaNew, spd, props,
+                     wflag, g, list, count);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::learned_select(int nv, const double *U, const double *eint, const double *theta,
+                                         const double *nadFlag, const float *P, float thr, double *wflag, int *list,
+                                         int *counter, const Grid &g, int *pos, int cap, hipStream_t s) {
+  if (nv == 9)
+    hipLaunchKernelGGL((dev::k_learned_select<D, 9>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
+                       theta, nadFlag, P, thr, wflag, list, counter, g, pos, cap);
+  else
+    hipLaunchKernelGGL((dev::k_learned_select<D, 5>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, U, eint,
+                       theta, nadFlag, P, thr, wflag, list, counter, g, pos, cap);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::features(int nv, const double *U, const double *eint, float *X, const Grid &g,
+                                   hipStream_t s) {
+  if (nv == 9)
+    hipLaunchKernelGGL((dev::k_features<D, 9>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
+  else
+    hipLaunchKernelGGL((dev::k_features<D, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, eint, X, g);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::labels(int nv, const double *U, const double *FSL, const double *FSW, float tau, float *Y,
+                                 const Grid &g, hipStream_t s) {
+  if (nv == 9)
+    hipLaunchKernelGGL((dev::k_labels<D, NQ, 9>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW,
+                       tau, Y, g);
+  else
+    hipLaunchKernelGGL((dev::k_labels<D, NQ, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, FSL, FSW,
+                       tau, Y, g);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::logits(const float *X, const float *P, float *Z, int n, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_logits<D>), dim3(stage_blocks(n)), dim3(dev::BLK), 0, s, X, P, Z, n);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::train_grad(const float *X, const float *Y, const int *idx, int n, int nb, uint64_t seed,
+                                     const float *P, float posW, float *part, float *lossPart, hipStream_t s) {
+  const unsigned nblk = unsigned((nb + dev::TB - 1) / dev::TB);
+  hipLaunchKernelGGL((dev::k_train_grad<D>), dim3(nblk), dim3(dev::TB), 0, s, X, Y, idx, n, nb, seed, P, posW,
+                     part, lossPart);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::nad(int nv, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
+                              int *count, const Grid &g, hipStream_t s) {
+  if (nv == 9)
+    hipLaunchKernelGGL((dev::k_nad<D, 9>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
+                       nadFlag, count, g);
+  else
+    hipLaunchKernelGGL((dev::k_nad<D, 5>), dim3(bricks(g)), dim3(dev::RBox<D>::NT), 0, s, Uold, Unew, fr, fv,
+                       nadFlag, count, g);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
+dim3 Launch<D, NQ>::flux_grid(const Grid &g) {
+  using Bk = dev::Brick<D>;
+  return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X) * unsigned((g.ny + Bk::Y - 1) / Bk::Y) *
+              unsigned((g.nz + Bk::Z - 1) / Bk::Z));
+}
+
+template <int D, int NQ>
+hipError_t Launch<D, NQ>::flux_rk(int nv, const double *U, const double *FS, const Grid &g,
+                                  const RKArgs &a, hipStream_t s) {
+  using Bk = dev::Brick<D>;
+  const dim3 blk(Bk::X, Bk::Y, Bk::Z);
+  if (nv == 5)
+    hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 5>), flux_grid(g), blk, 0, s, U, FS, g, a);
+  else if (nv == 9)
+    hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 9>), flux_grid(g), blk, 0, s, U, FS, g, a);
+  else
+    hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 1>), flux_grid(g), blk, 0, s, U, FS, g, a);
+  return hipGetLastError();
+}
 
 }  // namespace kfvm

@@ -10,7 +10,8 @@
 //   k_mhd_wenorec    KernelWenoRecon_K + fillInterior (Stencil_K.H:164-504),
 //                    one lane per (cell, variable)
 //   k_mhd_weno_post  VarConvert<MHD_GLM> back-conversion (MHD_GLM_K.H:24-69),
-//                    then the limiter of the WENO cells
+//                    then the limiter of the WENO cells: lanes per listed
+//                    cell (k_mhd_weno_post_cells: thread per cell, full WENO)
 //   k_mhd_pp         PositivityPreserve_K with interior points
 //                    (Common_K.H:299-643) for the cells that failed the fast
 //                    test, then SpeedEstimate_K (MHD_GLM_K.H:213-310) on the
@@ -536,11 +537,17 @@ __device__ inline void mhd_pp_cell(const double *__restrict__ U, double *__restr
 }
 
 // Back to conserved variables (linPrim2Cons, MHD_GLM_K.H:24-69) at every face
-// and interior point of the listed cells (list == nullptr: every cell), then
-// the limiter's exact sequence on the cell.  The energy sums its terms in the
-// reference's operand order
+// and interior point of the listed cells, then the limiter's exact sequence
+// on the cell, with one lane per point: a group
+// of LPG lanes per cell (PPL points per lane where the cell has more points
+// than 64).  The energy sums its terms in the reference's operand order
 //   E = vsq d + rho u mx + rho v my + rho w mz + bx Bx + by By + bz Bz
 //       + e / (gamma - 1) + psi0 psi.
+// The limiter's folds over the points (the density and energy theta, the
+// speed maxima) become group reductions: min and max do not depend on the
+// order (the NaN-ignoring fmin / fmax and compares of the sequential form),
+// so the results equal mhd_pp_cell's.  The limiter bounds come from the
+// group's lanes, one neighbour each.
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK) void k_mhd_weno_post(const double *__restrict__ U,
                                                        double *__restrict__ FS,
@@ -549,11 +556,114 @@ __global__ __launch_bounds__(BLK) void k_mhd_weno_post(const double *__restrict_
                                                        const int *__restrict__ counter,
                                                        double *__restrict__ thetaNew, double *__restrict__ spd,
                                                        const double *__restrict__ props) {
+  constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI, NPT = FQ + NI;
+  constexpr int LPG = PPGroup<NPT>::LPG, GPW = 64 / LPG, PPL = (NPT + LPG - 1) / LPG;
+  const int n = *counter;
+  const size_t nc = g.ncell;
+  const int lane = threadIdx.x & 63, pl = lane % LPG;
+  const int gid = (blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)) * GPW + lane / LPG;
+  const int ngroups = gridDim.x * (BLK / 64) * GPW;
+  for (int t = gid; t < n; t += ngroups) {  // uniform over the group
+    const int c = list[t];
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    double ref[9];
+#pragma unroll
+    for (int w = 0; w < 9; w++) ref[w] = U[size_t(w) * nc + c];
+    const MhdLin lp(ref, g.gamma);
+    // this lane's points: pl, pl + LPG, ...
+    double st[PPL][9];
+    bool own[PPL];
+#pragma unroll
+    for (int r = 0; r < PPL; r++) {
+      const int p = pl + r * LPG;
+      own[r] = p < NPT;
+      const double *P = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB
+                               : SRC + size_t(own[r] ? p - FQ : 0) * 9 * nc + c;
+      const size_t ps = p < FQ ? size_t(FSB) : nc;
+      double a[9];
+#pragma unroll
+      for (int w = 0; w < 9; w++) a[w] = own[r] ? P[w * ps] : ref[w];
+      const double d = a[0];
+      double e = lp.vsq * d;
+      const double mx = lp.rho * a[1] + lp.u * d;
+      e = e + lp.rho * lp.u * a[1];
+      const double my = lp.rho * a[2] + lp.v * d;
+      e = e + lp.rho * lp.v * a[2];
+      const double mz = lp.rho * a[3] + lp.w * d;
+      e = e + lp.rho * lp.w * a[3];
+      e = e + lp.bx * a[5];
+      e = e + lp.by * a[6];
+      e = e + lp.bz * a[7];
+      e = e + a[4] / lp.gamm;
+      e = e + lp.rPsi * a[8];
+#pragma unroll
+      for (int w = 0; w < 9; w++) st[r][w] = a[w];
+      st[r][1] = own[r] ? mx : ref[1];
+      st[r][2] = own[r] ? my : ref[2];
+      st[r][3] = own[r] ? mz : ref[3];
+      st[r][4] = own[r] ? e : ref[4];
+    }
+    // limiter bounds (Common_K.H:329-387), then density, then internal energy
+    const PPBounds pb = grp_pp_bounds<D, LPG>(U, props, g, i, j, k, ref, pl);
+    double thD = 1.0;
+#pragma unroll
+    for (int r = 0; r < PPL; r++) thD = own[r] ? dens_theta(thD, st[r][0], pb) : thD;
+    thD = grp_min<LPG>(thD);
+    const bool mixD = thD > 0.0 && thD < 1.0, repD = !mixD && thD < 1.0;
+    double thE = 1.0, maxLam = 0.0, maxVel = 0.0;
+#pragma unroll
+    for (int r = 0; r < PPL; r++) {
+#pragma unroll
+      for (int w = 0; w < 9; w++)
+        st[r][w] = repD ? ref[w] : (mixD ? ref[w] + thD * (st[r][w] - ref[w]) : st[r][w]);
+      if (own[r] && !(Mhd::eint(st[r]) > pb.minEint)) thE = fmin(thE, mhd_eint_pos(pb.minEint, ref, st[r]));
+    }
+    const double thP = grp_min<LPG>(thE);
+    const bool mixP = thP > 0.0 && thP < 1.0, repP = !mixP && thP < 1.0;
+#pragma unroll
+    for (int r = 0; r < PPL; r++) {
+#pragma unroll
+      for (int w = 0; w < 9; w++)
+        st[r][w] = repP ? ref[w] : (mixP ? ref[w] + thP * (st[r][w] - ref[w]) : st[r][w]);
+      if (own[r] && pl + r * LPG < FQ) mhd_face_speed(g.gamma, st[r], maxLam, maxVel);
+    }
+    maxLam = grp_max<LPG>(maxLam);
+    maxVel = grp_max<LPG>(maxVel);
+    // the converted momenta and energy, or every variable of a limited cell
+    const bool all = thD < 1.0 || thP < 1.0;
+#pragma unroll
+    for (int r = 0; r < PPL; r++) {
+      const int p = pl + r * LPG;
+      if (!own[r]) continue;
+      double *P = p < FQ ? FS + fs_base(c, FQ * 9) + size_t(p) * 9 * FSB : SRC + size_t(p - FQ) * 9 * nc + c;
+      const size_t ps = p < FQ ? size_t(FSB) : nc;
+#pragma unroll
+      for (int w = 0; w < 9; w++)
+        if (all || (w >= 1 && w <= 4)) P[w * ps] = st[r][w];
+    }
+    if (pl == 0) {
+      thetaNew[c] = fmin(thD, thP);
+      spd[c] = maxLam;
+      spd[nc + c] = maxVel;
+    }
+  }
+}
+
+// k_mhd_weno_post for every cell (full WENO), one thread per cell: the same
+// conversion and mhd_pp_cell's sequence; consecutive threads read consecutive
+// cells' values, which a list does not give.
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK) void k_mhd_weno_post_cells(const double *__restrict__ U,
+                                                             double *__restrict__ FS,
+                                                             double *__restrict__ SRC, Grid g,
+                                                             double *__restrict__ thetaNew, double *__restrict__ spd,
+                                                             const double *__restrict__ props) {
   constexpr int FQ = MShape<D, NQ>::FQ, NI = MShape<D, NQ>::NI;
-  const int n = list ? *counter : g.ncell;
+  const int n = g.ncell;
   const size_t nc = g.ncell;
   for (int t = blockIdx.x * BLK + threadIdx.x; t < n; t += gridDim.x * BLK) {
-    const int c = list ? list[t] : t;
+    const int c = t;
     double ref[9];
 #pragma unroll
     for (int w = 0; w < 9; w++) ref[w] = U[size_t(w) * nc + c];

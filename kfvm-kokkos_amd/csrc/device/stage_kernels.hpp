@@ -923,6 +923,11 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
 #include "indicator.hpp"
 
 namespace kfvm {
+// blocks of BLK threads covering n items
+inline unsigned stage_blocks(int64_t n) { return unsigned((n + dev::BLK - 1) / dev::BLK); }
+}  // namespace kfvm
+
+namespace kfvm {
 namespace dev {
 
 #ifdef KFVM_SMALL_KERNELS
