@@ -769,58 +769,131 @@ __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
     return NV == 5 ? rowTab[e] : -1;
   };
 
+  // Face items (the same for every face point q): d0 = t | dir << 20 |
+  // needed << 30 (-1: padding), and for each side the source of its states:
+  // a cell index (>= 0, its FS slots), a compact WENO row r (-(r + 2)), or the
+  // mirror of the other side (-1: a wall).  Euler and linear advection
+  // resolve them once per brick into LDS (each thread reads back only the
+  // items it wrote); GLM-MHD, whose flux planes leave no LDS for them at two
+  // workgroups per CU, resolves them at every point.
+  constexpr bool FDL = NV != 9;
+  __shared__ int fd0[FDL ? Bk::PF : 1], fdL[FDL ? Bk::PF : 1], fdR[FDL ? Bk::PF : 1];
+  constexpr int MIRROR = -1;
+  auto describe = [&](int w, int &d0, int &sL, int &sR) {
+    // padded work item w -> face t of the unpadded numbering (x, then y, then z faces)
+    int dir, fi, fj, fk, t;
+    bool pad = false;
+    if (w < Bk::PXF) {
+      pad = w >= Bk::NXF;
+      t = w;
+      dir = 0;
+      fi = t % (Bk::X + 1);
+      fj = (t / (Bk::X + 1)) % Bk::Y;
+      fk = t / ((Bk::X + 1) * Bk::Y);
+    } else if (w < Bk::PXF + Bk::PYF) {
+      const int u = w - Bk::PXF;
+      pad = u >= Bk::NYF;
+      t = Bk::NXF + u;
+      dir = 1;
+      fi = u % Bk::X;
+      fj = (u / Bk::X) % (Bk::Y + 1);
+      fk = u / (Bk::X * (Bk::Y + 1));
+    } else {
+      const int u = w - Bk::PXF - Bk::PYF;
+      pad = u >= Bk::NZF;
+      t = Bk::NXF + Bk::NYF + u;
+      dir = 2;
+      fi = u % Bk::X;
+      fj = (u / Bk::X) % Bk::Y;
+      fk = u / (Bk::X * Bk::Y);
+    }
+    sL = sR = MIRROR;
+    if (pad) {
+      d0 = -1;
+      return;
+    }
+    // right cell (the face is its lower face) and left cell
+    const int ri = i0 + fi, rj = j0 + fj, rk = k0 + fk;
+    const int li = ri - (dir == 0), lj = rj - (dir == 1), lk = rk - (dir == 2);
+    const bool rIn = ri < g.nx && rj < g.ny && (D == 2 || rk < g.nz);
+    const bool lIn = li >= 0 && lj >= 0 && lk >= 0 && li < g.nx && lj < g.ny && (D == 2 || lk < g.nz);
+    // a face is needed when one of its cells is a live cell of this brick
+    const bool rBrick = fi < Bk::X && fj < Bk::Y && fk < Bk::Z && rIn;
+    const bool lBrick = (dir == 0 ? fi > 0 : (dir == 1 ? fj > 0 : fk > 0)) && lIn;
+    const bool needed = rBrick || lBrick;
+    // the side's source; a cell off the grid along dir wraps (periodic) or
+    // mirrors the other side (wall), as face_state resolves it
+    auto source = [&](int x, int y, int z) {
+      const int n_ax = dir == 0 ? g.nx : (dir == 1 ? g.ny : g.nz);
+      int pos = dir == 0 ? x : (dir == 1 ? y : z);
+      const int row = row_of(x, y, z);
+      if (pos < 0 || pos >= n_ax) {
+        const bool low = pos < 0;
+        if ((low ? g.bc[2 * dir] : g.bc[2 * dir + 1]) != 0) return MIRROR;
+        pos = low ? pos + n_ax : pos - n_ax;
+        if (dir == 0) x = pos;
+        else if (dir == 1) y = pos;
+        else z = pos;
+      }
+      if (NV == 5 && row >= 0) return -(row + 2);
+      return (z * g.ny + y) * g.nx + x;
+    };
+    if (needed) {
+      sR = source(ri, rj, rk);
+      sL = source(li, lj, lk);
+    }
+    d0 = t | (dir << 20) | (needed ? 1 << 30 : 0);
+  };
+  if constexpr (FDL) {
+    for (int w = tid; w < Bk::PF; w += BLK) describe(w, fd0[w], fdL[w], fdR[w]);
+  }
+  // states of face f at point q from a side's source
+  auto load_side = [&](int src, int f, int q, double *out) {
+    if (NV == 5 && src <= -2) {
+      const double *r = a.ws + size_t(-src - 2) * (Shape<D, NQ>::F * Q * 5) + (f * Q + q) * 5;
+#pragma unroll
+      for (int v = 0; v < NV; v++) out[v] = r[v];
+    } else {
+      const double *p = FS + fs_base(size_t(src), Shape<D, NQ>::F * Q * NV) + size_t(f * Q + q) * NV * FSB;
+#pragma unroll
+      for (int v = 0; v < NV; v++) out[v] = p[size_t(v) * FSB];
+    }
+  };
+  auto mirror_of = [&](const double *inner, unsigned m, double *out) {
+    const unsigned mm = NV >= 5 ? m : 0u;
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = ((mm >> v) & 1u) ? -inner[v] : inner[v];
+  };
+
 #pragma unroll 1
   for (int q = 0; q < Q; q++) {
     // every face of the brick at point q, solved once
 #pragma unroll 1
     for (int w = tid; w < Bk::PF; w += BLK) {
-      // padded work item w -> face t of the unpadded numbering (x, then y, then z faces)
-      int dir, fi, fj, fk, t;
-      if (w < Bk::PXF) {
-        if (w >= Bk::NXF) continue;
-        t = w;
-        dir = 0;
-        fi = t % (Bk::X + 1);
-        fj = (t / (Bk::X + 1)) % Bk::Y;
-        fk = t / ((Bk::X + 1) * Bk::Y);
-      } else if (w < Bk::PXF + Bk::PYF) {
-        const int u = w - Bk::PXF;
-        if (u >= Bk::NYF) continue;
-        t = Bk::NXF + u;
-        dir = 1;
-        fi = u % Bk::X;
-        fj = (u / Bk::X) % (Bk::Y + 1);
-        fk = u / (Bk::X * (Bk::Y + 1));
+      int d0, sL, sR;
+      if constexpr (FDL) {
+        d0 = fd0[w];
+        sL = fdL[w];
+        sR = fdR[w];
       } else {
-        const int u = w - Bk::PXF - Bk::PYF;
-        if (u >= Bk::NZF) continue;
-        t = Bk::NXF + Bk::NYF + u;
-        dir = 2;
-        fi = u % Bk::X;
-        fj = (u / Bk::X) % Bk::Y;
-        fk = u / (Bk::X * Bk::Y);
+        describe(w, d0, sL, sR);
       }
-      // right cell (the face is its lower face) and left cell
-      const int ri = i0 + fi, rj = j0 + fj, rk = k0 + fk;
-      const int li = ri - (dir == 0), lj = rj - (dir == 1), lk = rk - (dir == 2);
-      const bool rIn = ri < g.nx && rj < g.ny && (D == 2 || rk < g.nz);
-      const bool lIn = li >= 0 && lj >= 0 && lk >= 0 && li < g.nx && lj < g.ny && (D == 2 || lk < g.nz);
+      if (d0 < 0) continue;
+      const int t = d0 & 0xFFFFF, dir = (d0 >> 20) & 3;
       double Fl[NV];
 #pragma unroll
       for (int v = 0; v < NV; v++) Fl[v] = 0.0;
       double bv = 0.0;
-      // a face is needed when one of its cells is a live cell of this brick
-      const bool rBrick = fi < Bk::X && fj < Bk::Y && fk < Bk::Z && rIn;
-      const bool lBrick = (dir == 0 ? fi > 0 : (dir == 1 ? fj > 0 : fk > 0)) && lIn;
-      if (rBrick || lBrick) {
+      if (d0 >> 30) {
         double uL[NV], uR[NV];
         const int lo = 2 * dir + 1, hi = 2 * dir;  // left cell's upper face, right cell's lower face
-        if (rIn) {
-          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, nullptr, uR, row_of(ri, rj, rk), a.ws);
-          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, uR, uL, row_of(li, lj, lk), a.ws);
+        if (sR != MIRROR) {
+          load_side(sR, hi, q, uR);
+          if (sL == MIRROR) mirror_of(uR, g.negmask[2 * dir], uL);
+          else load_side(sL, lo, q, uL);
         } else {
-          face_state<D, NQ, NV>(FS, g, li, lj, lk, lo, q, nullptr, uL, row_of(li, lj, lk), a.ws);
-          face_state<D, NQ, NV>(FS, g, ri, rj, rk, hi, q, uL, uR, row_of(ri, rj, rk), a.ws);
+          load_side(sL, lo, q, uL);
+          mirror_of(uL, g.negmask[2 * dir + 1], uR);
         }
         double sp;
         if (dir == 0) {
