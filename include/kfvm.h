@@ -132,6 +132,19 @@ int kfvm_set_weights(kfvm_solver *s, const double *face, int64_t nface, const do
 int kfvm_write_plot(kfvm_solver *s, char *path, int64_t len);
 int kfvm_write_checkpoint(kfvm_solver *s, char *path, int64_t len);
 
+/* Flow statistics of the current state (Solver::evalFlowStats,
+ * src/Solver.cpp:154-186, with FlowStats<Hydro> Hydro_K.H:103-173 and
+ * FlowStats<MHD_GLM> MHD_GLM_K.H:108-207): domain averages of the kinetic
+ * energy and the enstrophy (and, GLM-MHD, B . curl B), with central
+ * differences through the boundary conditions; linear advection has one
+ * statistic, always 0.  stats receives up to 3 values, *n their number.
+ * kfvm_solve writes them as rows (time, statistics) of
+ * <datadir>/R2_NX.._NY..[_NZ..]/<basename>_stats.h5, dataset stats_data
+ * [time:maxsteps][1 + n], every io:statsfreq steps, before the first step and
+ * after the last (WriterPDI::writeFlowStats, src/io/WriterPDI.cpp:69-86).
+ * Single problems only (not batches). */
+int kfvm_flow_stats(kfvm_solver *s, double *stats, int *n);
+
 /* Per-kernel device timing with HIP events on the solver's stream.  While
  * enabled, every launch is bracketed by events; kfvm_profile_get returns the
  * accumulated milliseconds and launch count of one kernel family

@@ -229,6 +229,11 @@ hipError_t launch_mhd_weno(int dim, int nq, const double *U, double *FS, double 
 hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *SRC, double *thetaNew,
                          double *spd, const double *props, const double *wflag, const Grid &g, const int *list,
                          const int *count, hipStream_t s);
+// Flow statistics of U (Euler nv 5, GLM-MHD nv 9): per-block partial sums
+// out[3 b + s] (s: kinetic energy, enstrophy, magnetic helicity) over
+// stage_blocks(ncell) blocks, each cell weighted by ifac = dvol / volume.
+hipError_t launch_flow_stats(int dim, int nv, const double *U, const Grid &g, double ifac, double *out,
+                             hipStream_t s);
 // Cleaning speed: the per-cell maxima spd folded into chBits (zeroed by the
 // caller).  The source terms are formed inside launch_flux_rk (RKArgs src,
 // theta, Dm).

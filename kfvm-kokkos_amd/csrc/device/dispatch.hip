@@ -117,6 +117,11 @@ hipError_t launch_mhd_pp(int dim, int nq, const double *U, double *FS, double *S
                          const int *count, hipStream_t s) {
   KFVM_SELECT(dim, nq, L::mhd_pp(U, FS, SRC, thetaNew, spd, props, wflag, g, list, count, s));
 }
+hipError_t launch_flow_stats(int dim, int nv, const double *U, const Grid &g, double ifac, double *out,
+                            hipStream_t s) {
+  if (nv != 5 && nv != 9) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, 2, L::flow_stats(nv, U, g, ifac, out, s));
+}
 hipError_t launch_mhd_chmax(int dim, const double *spd, const Grid &g, unsigned long long *chBits, hipStream_t s) {
   (void)dim;
   const unsigned nb = std::min<unsigned>(stage_blocks(g.ncell), 2048u);

@@ -69,6 +69,7 @@ struct Launch {
                                const float *P, float posW, float *part, float *lossPart, hipStream_t s);
   static hipError_t nad(int nv, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
                         int *count, const Grid &g, hipStream_t s);
+  static hipError_t flow_stats(int nv, const double *U, const Grid &g, double ifac, double *out, hipStream_t s);
   static dim3 flux_grid(const Grid &g);  // one-dimensional: the kernel renumbers the bricks
   static hipError_t flux_rk(int nv, const double *U, const double *FS, const Grid &g, const RKArgs &a,
                             hipStream_t s);

@@ -238,6 +238,16 @@ hipError_t Launch<D, NQ>::nad(int nv, const double *Uold, const double *Unew, do
 }
 
 template <int D, int NQ>
+hipError_t Launch<D, NQ>::flow_stats(int nv, const double *U, const Grid &g, double ifac, double *out,
+                                     hipStream_t s) {
+  if (nv == 9)
+    hipLaunchKernelGGL((dev::k_flow_stats<D, 9>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, g, ifac, out);
+  else
+    hipLaunchKernelGGL((dev::k_flow_stats<D, 5>), dim3(stage_blocks(g.ncell)), dim3(dev::BLK), 0, s, U, g, ifac, out);
+  return hipGetLastError();
+}
+
+template <int D, int NQ>
 dim3 Launch<D, NQ>::flux_grid(const Grid &g) {
   using Bk = dev::Brick<D>;
   return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X) * unsigned((g.ny + Bk::Y - 1) / Bk::Y) *

@@ -1003,6 +1003,65 @@ inline unsigned stage_blocks(int64_t n) { return unsigned((n + dev::BLK - 1) / d
 namespace kfvm {
 namespace dev {
 
+// Flow statistics (FlowStats<Hydro> Hydro_K.H:103-173, FlowStats<MHD_GLM>
+// MHD_GLM_K.H:108-207): per cell, with ifac = dvol / volume,
+//   ekin += ifac |m|^2 / (2 rho)
+//   enst += ifac rho |omega|^2 / 2, omega = curl u from central differences
+//   heli += ifac B . curl B (GLM-MHD; central differences of B)
+// in the reference's operation order; the neighbours come through the
+// boundary index maps, as the reference's ghost cells.  One partial sum per
+// block: out[3 blockIdx + s].
+template <int D, int NV>
+__global__ __launch_bounds__(BLK) void k_flow_stats(const double *__restrict__ U, Grid g, double ifac,
+                                                    double *__restrict__ out) {
+  __shared__ double sh[BLK];
+  const int c = blockIdx.x * BLK + threadIdx.x;
+  double ek = 0.0, et = 0.0, he = 0.0;
+  if (c < g.ncell) {
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    auto u = [&](int di, int dj, int dk, int v) { return load_u<D, NV>(U, g, i + di, j + dj, k + dk, v); };
+    const size_t nc = g.ncell;
+    const double rho = U[c], mx = U[nc + c], my = U[2 * nc + c], mz = U[3 * nc + c];
+    const double ekin = 0.5 * (mx * mx + my * my + mz * mz) / rho;
+    const double dyu = 0.5 * (u(0, 1, 0, 1) / u(0, 1, 0, 0) - u(0, -1, 0, 1) / u(0, -1, 0, 0)) / g.dy;
+    const double dxv = 0.5 * (u(1, 0, 0, 2) / u(1, 0, 0, 0) - u(-1, 0, 0, 2) / u(-1, 0, 0, 0)) / g.dx;
+    double dzu = 0.0, dzv = 0.0, dxw = 0.0, dyw = 0.0;
+    if (D == 3) {
+      dzu = 0.5 * (u(0, 0, 1, 1) / u(0, 0, 1, 0) - u(0, 0, -1, 1) / u(0, 0, -1, 0)) / g.dz;
+      dzv = 0.5 * (u(0, 0, 1, 2) / u(0, 0, 1, 0) - u(0, 0, -1, 2) / u(0, 0, -1, 0)) / g.dz;
+      dxw = 0.5 * (u(1, 0, 0, 3) / u(1, 0, 0, 0) - u(-1, 0, 0, 3) / u(-1, 0, 0, 0)) / g.dx;
+      dyw = 0.5 * (u(0, 1, 0, 3) / u(0, 1, 0, 0) - u(0, -1, 0, 3) / u(0, -1, 0, 0)) / g.dy;
+    }
+    const double omx = dyw - dzv, omy = dzu - dxw, omz = dxv - dyu;
+    const double enst = 0.5 * rho * (omx * omx + omy * omy + omz * omz);
+    ek = ifac * ekin;
+    et = ifac * enst;
+    if (NV == 9) {
+      const double dybx = 0.5 * (u(0, 1, 0, 5) - u(0, -1, 0, 5)) / g.dy;
+      const double dxby = 0.5 * (u(1, 0, 0, 6) - u(-1, 0, 0, 6)) / g.dx;
+      double dzbx = 0.0, dzby = 0.0, dxbz = 0.0, dybz = 0.0;
+      if (D == 3) {
+        dzbx = 0.5 * (u(0, 0, 1, 5) - u(0, 0, -1, 5)) / g.dz;
+        dzby = 0.5 * (u(0, 0, 1, 6) - u(0, 0, -1, 6)) / g.dz;
+        dxbz = 0.5 * (u(1, 0, 0, 7) - u(-1, 0, 0, 7)) / g.dx;
+        dybz = 0.5 * (u(0, 1, 0, 7) - u(0, -1, 0, 7)) / g.dy;
+      }
+      const double cbx = dybz - dzby, cby = dzbx - dxbz, cbz = dxby - dybx;
+      const double bcb = U[5 * nc + c] * cbx + U[6 * nc + c] * cby + U[7 * nc + c] * cbz;
+      he = ifac * bcb;
+    }
+  }
+  const double se = block_sum(ek, sh, threadIdx.x);
+  const double st = block_sum(et, sh, threadIdx.x);
+  const double sb = block_sum(he, sh, threadIdx.x);
+  if (threadIdx.x == 0) {
+    out[3 * blockIdx.x] = se;
+    out[3 * blockIdx.x + 1] = st;
+    out[3 * blockIdx.x + 2] = sb;
+  }
+}
+
 #ifdef KFVM_SMALL_KERNELS
 // ------------------------------------------------------ small register kernels
 template <int NV>

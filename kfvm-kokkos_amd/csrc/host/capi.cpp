@@ -432,6 +432,15 @@ int kfvm_write_checkpoint(kfvm_solver *s, char *path, int64_t len) {
   return guarded([&] { return copy_path(s->s->writeCkpt(), path, len); });
 }
 
+int kfvm_flow_stats(kfvm_solver *s, double *stats, int *n) {
+  CHECK_SOLVER(s);
+  if (!stats || !n) return fail("kfvm_flow_stats: null output");
+  return guarded([&] {
+    *n = s->s->flowStats(stats);
+    return 0;
+  });
+}
+
 int kfvm_force_full_weno(kfvm_solver *s) {
   CHECK_SOLVER(s);
   s->s->useSparseWeno = false;

@@ -602,7 +602,7 @@ void Solver::release() noexcept {
   } catch (...) {
   }
   void *bufs[] = {SRC_, cellW_, cellWS_, diffM_, chBits_, U_[0], U_[1], Uprev_, Uhat_, K_, Utmp_, FS_, WI_, props_, theta_[0], theta_[1], wflag_, list_, counter_, ppList_,
-                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_, faceWC_, spd_, mhdWT_};
+                  faceW_, derivW_, faceWT_, faceWS_, derivWS_, errPart_, posPart_, redOut_, vmax_, netP_, nadCount_, FSlin_, nadFlag_, sDtDev_, sRed_, sMask_, WS_, pos_, SB_, faceWQ_, thBak_, wfBak_, faceWC_, spd_, mhdWT_, statsPart_};
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto &kv : graphs_)
     if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -1214,13 +1214,52 @@ void Solver::advanceStages(int64_t n, double dtFixed) {
   }
 }
 
+int Solver::flowStats(double *out) {
+  if (slices_ > 1) throw std::runtime_error("flow statistics are built for single problems");
+  if (nv_ == 1) {  // the reference's LinAdv statistics add nothing (LinAdv_K.H:53-64)
+    out[0] = 0.0;
+    return 1;
+  }
+  const Config &c = cfg_;
+  const bool d3 = c.dim == 3;
+  // dvol / volume of Geometry.H:35-40
+  const double volume = d3 ? (c.xUp - c.xLo) * (c.yUp - c.yLo) * (c.zUp - c.zLo) : (c.xUp - c.xLo) * (c.yUp - c.yLo);
+  const double dvol = d3 ? grid_.dx * grid_.dy * grid_.dz : grid_.dx * grid_.dy;
+  const size_t nb = (size_t(ncell_) + 255) / 256;
+  if (!statsPart_) statsPart_ = dalloc<double>(3 * nb, "flow statistics partials");
+  const double *U = stateInCur_ ? U_[cur_] : Uprev_;
+  check(launch_flow_stats(c.dim, nv_, U, grid_, dvol / volume, statsPart_, stream_), "flow_stats");
+  std::vector<double> part(3 * nb);
+  check(hipStreamSynchronize(stream_), "sync");
+  check(hipMemcpy(part.data(), statsPart_, part.size() * sizeof(double), hipMemcpyDeviceToHost), "get stats");
+  const int n = nv_ == 9 ? 3 : 2;
+  for (int st = 0; st < n; st++) {
+    double sum = 0.0;
+    for (size_t b = 0; b < nb; b++) sum += part[3 * b + size_t(st)];
+    out[st] = sum;
+  }
+  return n;
+}
+
+// One row of <basename>_stats.h5 (Solver.cpp:154-186, WriterPDI.cpp:69-86).
+void Solver::evalFlowStats() {
+  Region region("Solver::evalFlowStats");
+  double st[3];
+  const int n = flowStats(st);
+  if (!writer_) writer_.reset(new Writer(cfg_));
+  writer_->writeStats(statsRow_, cfg_.maxSteps, time, st, n);
+  statsRow_++;
+}
+
 int64_t Solver::solve() {
   // Solver::Solve (Solver.cpp:84-140): step, then the outputs that are due
   Region region("Solver::Solve");
+  if (cfg_.statsFreq > 0) evalFlowStats();
   int64_t n = 0;
   for (; nTS < cfg_.maxSteps && !lastTimeStep; n++) {
     const int64_t step = nTS;  // the reference advances its counter after the outputs
     takeStep();
+    if (cfg_.statsFreq > 0 && step % cfg_.statsFreq == 0) evalFlowStats();
     const bool last = lastTimeStep || step == cfg_.maxSteps - 1;
     if (cfg_.plotFreq > 0.0 && (writePlot_ || last)) {
       writePlot_ = false;
@@ -1233,6 +1272,7 @@ int64_t Solver::solve() {
       writeCkpt();
       nTS = keep;
     }
+    if (lastTimeStep && cfg_.statsFreq > 0) evalFlowStats();
   }
   return n;
 }

@@ -62,6 +62,10 @@ class Solver {
   // writeCkpt); return the path of the HDF5 file.
   std::string writePlot();
   std::string writeCkpt();
+  // Flow statistics of the current state (Solver::evalFlowStats,
+  // Solver.cpp:154-186): Euler kinetic energy and enstrophy, GLM-MHD also
+  // magnetic helicity, linear advection one zero; returns their number (<= 3).
+  int flowStats(double *out);
   int plotNum = 0;  // index of the next plot (Solver.H plotNum)
 
   const Config &config() const { return cfg_; }
@@ -165,6 +169,9 @@ class Solver {
   unsigned long long *chBits_ = nullptr;
   // GLM-MHD: per-cell cleaning-speed maxima [2][ncell]; linear-pass weights (face, then interior passes)
   double *spd_ = nullptr, *mhdWT_ = nullptr;
+  double *statsPart_ = nullptr;  // flow statistics: per-block partial sums [3][blocks]
+  int statsRow_ = 0;             // next row of <basename>_stats.h5
+  void evalFlowStats();
   double *faceW_ = nullptr, *derivW_ = nullptr, *faceWT_ = nullptr, *faceWS_ = nullptr, *derivWS_ = nullptr;
   double *faceWC_ = nullptr;  // Euler WENO: compact face + derivative weights (compact_weno_weights)
   double *errPart_ = nullptr, *posPart_ = nullptr, *redOut_ = nullptr;

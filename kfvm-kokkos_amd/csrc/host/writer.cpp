@@ -23,8 +23,8 @@ using hid_t = int64_t;
 using herr_t = int;
 using hsize_t = unsigned long long;
 constexpr hid_t P_DEFAULT = 0, S_ALL = 0;
-constexpr unsigned F_ACC_TRUNC = 0x0002u, F_ACC_RDONLY = 0x0000u;
-constexpr int S_SCALAR = 0;
+constexpr unsigned F_ACC_TRUNC = 0x0002u, F_ACC_RDONLY = 0x0000u, F_ACC_RDWR = 0x0001u;
+constexpr int S_SCALAR = 0, S_SELECT_SET = 0;
 
 struct H5 {
   herr_t (*open)();
@@ -38,6 +38,7 @@ struct H5 {
   herr_t (*sclose)(hid_t);
   int (*sget_ndims)(hid_t);
   int (*sget_dims)(hid_t, hsize_t *, hsize_t *);
+  herr_t (*sselect_hyperslab)(hid_t, int, const hsize_t *, const hsize_t *, const hsize_t *, const hsize_t *);
   hid_t (*dcreate)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t, hid_t);
   hid_t (*dopen)(hid_t, const char *, hid_t);
   hid_t (*dget_space)(hid_t);
@@ -81,6 +82,7 @@ const H5 &h5() {
     KFVM_H5SYM(sclose, "H5Sclose");
     KFVM_H5SYM(sget_ndims, "H5Sget_simple_extent_ndims");
     KFVM_H5SYM(sget_dims, "H5Sget_simple_extent_dims");
+    KFVM_H5SYM(sselect_hyperslab, "H5Sselect_hyperslab");
     KFVM_H5SYM(dcreate, "H5Dcreate2");
     KFVM_H5SYM(dopen, "H5Dopen2");
     KFVM_H5SYM(dget_space, "H5Dget_space");
@@ -318,6 +320,47 @@ std::string Writer::writePlot(const OutputFields &f, int num, double time) {
   writeXdmf(prefix_ + "plot/" + step_name(cfg_.baseName, num, "xmf"), h5name, time, true);
   writeFile(prefix_ + "plot/" + h5name, f, true, num, time, 0.0, 1.0);
   return prefix_ + "plot/" + h5name;
+}
+
+// Flow statistics file (WriterPDI::writeFlowStats, WriterPDI.cpp:69-86 and
+// the stats_data dataset of user/pdi_*_xdmf.yml): <prefix><basename>_stats.h5
+// holds stats_data [maxRows][1 + nstats] (time, then the statistics), row
+// `row` written in place; the file is created (zero-filled) with the first
+// row.
+void Writer::writeStats(int row, int maxRows, double time, const double *stats, int nstats) {
+  const H5 &H = h5();
+  if (row < 0 || row >= maxRows) throw std::runtime_error("flow statistics: row beyond time:maxsteps");
+  std::filesystem::create_directories(prefix_);
+  const std::string path = prefix_ + cfg_.baseName + "_stats.h5";
+  const hsize_t ncol = hsize_t(nstats + 1);
+  hid_t file, ds;
+  if (row == 0) {
+    file = chkid(H.fcreate(path.c_str(), F_ACC_TRUNC, P_DEFAULT, P_DEFAULT), "create stats file");
+    const hsize_t dims[2] = {hsize_t(maxRows), ncol};
+    const hid_t sp = chkid(H.screate_simple(2, dims, nullptr), "create dataspace");
+    ds = H.dcreate(file, "stats_data", *H.native_double, sp, P_DEFAULT, P_DEFAULT, P_DEFAULT);
+    H.sclose(sp);
+  } else {
+    file = chkid(H.fopen(path.c_str(), F_ACC_RDWR, P_DEFAULT), "open stats file");
+    ds = H.dopen(file, "stats_data", P_DEFAULT);
+  }
+  if (ds < 0) {
+    H.fclose(file);
+    throw std::runtime_error("HDF5: stats_data dataset failed");
+  }
+  std::vector<double> rowv(static_cast<size_t>(ncol));
+  rowv[0] = time;
+  for (int n = 0; n < nstats; n++) rowv[size_t(n) + 1] = stats[n];
+  const hid_t fsp = H.dget_space(ds);
+  const hsize_t start[2] = {hsize_t(row), 0}, count[2] = {1, ncol};
+  herr_t r = H.sselect_hyperslab(fsp, S_SELECT_SET, start, nullptr, count, nullptr);
+  const hid_t msp = H.screate_simple(2, count, nullptr);
+  if (r >= 0) r = H.dwrite(ds, *H.native_double, msp, fsp, P_DEFAULT, rowv.data());
+  H.sclose(msp);
+  H.sclose(fsp);
+  H.dclose(ds);
+  H.fclose(file);
+  chk(r, "write stats row");
 }
 
 std::string Writer::writeCkpt(const OutputFields &f, int step, double time, double dt, double errEst) {

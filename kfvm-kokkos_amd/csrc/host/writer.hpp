@@ -38,6 +38,8 @@ class Writer {
   std::string writePlot(const OutputFields &f, int num, double time);
   // checkpoint: conserved, weno, fomx, time, step, step size (+ errEst)
   std::string writeCkpt(const OutputFields &f, int step, double time, double dt, double errEst);
+  // flow statistics row `row` of <basename>_stats.h5: time, then nstats values
+  void writeStats(int row, int maxRows, double time, const double *stats, int nstats);
   // read a checkpoint written by writeCkpt (or by the reference's writer)
   CkptData readCkpt(const std::string &path) const;
 

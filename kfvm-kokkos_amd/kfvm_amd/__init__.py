@@ -34,7 +34,7 @@ EXPORTS = [
     "kfvm_indicator_destroy", "kfvm_indicator_collect", "kfvm_indicator_samples", "kfvm_indicator_init",
     "kfvm_indicator_train", "kfvm_indicator_get_params", "kfvm_indicator_set_params",
     "kfvm_indicator_get_weights", "kfvm_indicator_gradient", "kfvm_indicator_logits",
-    "kfvm_indicator_sizes", "kfvm_set_nad", "kfvm_write_plot", "kfvm_write_checkpoint",
+    "kfvm_indicator_sizes", "kfvm_set_nad", "kfvm_write_plot", "kfvm_write_checkpoint", "kfvm_flow_stats",
     "kfvm_write_output_file", "kfvm_read_checkpoint_file", "kfvm_create_batch", "kfvm_get_slice_times",
     "kfvm_initial_condition", "kfvm_eval_rhs_noalloc",
 ]
@@ -102,6 +102,7 @@ def load_library(path=None):
         "kfvm_get_slice_times": [P, D, D, I64],
         "kfvm_write_plot": [P, ctypes.c_char_p, I64],
         "kfvm_write_checkpoint": [P, ctypes.c_char_p, I64],
+        "kfvm_flow_stats": [P, D, ctypes.POINTER(ctypes.c_int)],
         "kfvm_write_output_file": [ctypes.c_char_p, ctypes.c_int, D, D, I64, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_char_p, I64],
         "kfvm_read_checkpoint_file": [ctypes.c_char_p, ctypes.c_char_p, D, I64, ctypes.POINTER(ctypes.c_int),
@@ -438,6 +439,17 @@ class Solver:
         buf = ctypes.create_string_buffer(4096)
         self._call("kfvm_write_checkpoint", self._h, buf, len(buf))
         return buf.value.decode()
+
+    @property
+    def flow_stats(self):
+        """Flow statistics of the current state (Solver::evalFlowStats): Euler
+        (kinetic energy, enstrophy), GLM-MHD (the same and B . curl B), linear
+        advection (0,); domain averages."""
+        out = np.zeros(3)
+        n = ctypes.c_int(0)
+        self._call("kfvm_flow_stats", self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                   ctypes.byref(n))
+        return out[: n.value].copy()
 
     def set_nad(self, on=True, range_frac=0.05, value_frac=1e-3, max_frac=0.02):
         """Admissibility check of learned steps: tolerance max(value_frac |bound|,
