@@ -159,6 +159,60 @@ def cpu_baseline(n=64, reps=5):
                       f"value = KXRCF-sparse, full_weno_value = full WENO"}
 
 
+def cpu_workloads(big=256, big_steps=2, warm_stages=3, log=None):
+    """The oracle on the two CPU workloads of the path's measurement plan, on
+    every core this process may use, for KXRCF-sparse and full WENO:
+      * 64^3 from t = 0 to t = 0.2 with the adaptive controller (wall time,
+        steps, zone-updates per stage per second);
+      * big^3 with a fixed dt: warm_stages untimed RHS evaluations (the FSAL
+        prime and the first stages), then big_steps timed steps (their median).
+    No GPU is used."""
+    import numpy as np
+    import kfvm_amd
+    from orc import Oracle
+    threads, model = _host_cores()
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
+
+    def make(n, wt, fixed):
+        text = init_text(n, wt)
+        p = kfvm_amd.parse_init(text)
+        f, d = kfvm_amd.build_weights(3, 2, p["lfac"], 1.0 / n, 1.0 / n, 1.0 / n)
+        o = Oracle(dim=3, n=(n, n, n), face_w=f, deriv_w=d, nq=2, eq="hydro", rs="hllc",
+                   rk="ssp4_3_2", bc="outflow", gamma=p["gamma"], weno_thresh=p["wenothresh"],
+                   cfl=p["cfl"], final_time=0.2, atol=p["atol"], rtol=p["rtol"],
+                   fixed_dt=fixed, threads=threads, max_steps=100000)
+        o.set_ic("riemann3d_planar", (1.5,))
+        return o
+
+    out = {"cores": threads, "cpu": model}
+    for label, wt in (("sparse", 1.0), ("full_weno", -1.0)):
+        o = make(64, wt, -1.0)
+        t0 = time.perf_counter()
+        o.prime()
+        steps = 0
+        while o.time[0] < 0.2 - 1e-12:
+            o.take_step()
+            steps += 1
+        el = time.perf_counter() - t0
+        out[f"n64_to_t02_{label}"] = {"seconds": el, "steps": steps, "rhs_evals": o.rhs_evals,
+                                      "mzone_updates_per_s": 64 ** 3 * o.rhs_evals / el / 1e6}
+        log(f"[cpu] 64^3 {label} to t = 0.2: {el:.1f} s, {steps} steps")
+        o = make(big, wt, 0.0025)
+        o.prime()
+        while o.rhs_evals < warm_stages:
+            o.eval_rhs(0)
+        times = []
+        for k in range(big_steps):
+            e0 = o.rhs_evals
+            t0 = time.perf_counter()
+            o.take_step()
+            times.append((time.perf_counter() - t0) / (o.rhs_evals - e0))
+            log(f"[cpu] {big}^3 {label} step {k + 1}: {times[-1]:.2f} s per stage")
+        out[f"n{big}_{label}"] = {"s_per_stage_median": float(np.median(times)), "steps": big_steps,
+                                  "mzone_updates_per_s": big ** 3 / float(np.median(times)) / 1e6}
+    return out
+
+
 def _mode_run(n, weights, a, profile):
     """One run of the planar Riemann problem from t = 0 to t = 0.2: untimed
     warm-up steps, a timed window of K whole RK steps (take_step: stages,
@@ -288,10 +342,15 @@ def main():
                     help="logit above which the learned indicator selects WENO")
     ap.add_argument("--fused", action="store_true",
                     help="fused reconstruction + flux path (the flux kernel recomputes the linear states)")
+    ap.add_argument("--cpu-workloads", type=int, default=0, metavar="N",
+                    help="only time the CPU oracle: 64^3 to t = 0.2 and N^3 for 2 fixed-dt steps (no GPU)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (5 launches), the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
+    if a.cpu_workloads:
+        print(json.dumps(cpu_workloads(big=a.cpu_workloads)), flush=True)
+        return
     if a.calibrate_copy:
         import torch
         nn = a.calibrate_copy * (1 << 20) // 8
