@@ -87,7 +87,7 @@ PATH_F = {2: (12.4e3, 51.9e3), 3: (26.9e3, 87.9e3)}  # (linear path, full WENO) 
 # WRITE_SIZE passes over this workload (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the
 # factors a plain copy of known size gives), with the profile they come from;
 # None where not measured.
-TRAFFIC = {"flux_rk": (23.54e9, "profiles/flux_rk_traffic_riemann3d_256.md"),
+TRAFFIC = {"flux_rk": (23.33e9, "profiles/flux_rk_traffic_riemann3d_256.md"),
            "linear": (20.87e9, "profiles/flux_rk_traffic_riemann3d_256.md")}
 # Algorithmic FP64 work per cell of the Riemann / divergence / RK kernel: 3 NQ^2
 # face solves per cell at ~200 flop-equivalents each (HLLC: 8 square roots,
