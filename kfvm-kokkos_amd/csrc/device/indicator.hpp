@@ -361,7 +361,8 @@ __global__ __launch_bounds__(RBox<D>::NT) void k_learned_select(const double *__
 // The learned selection on the brick the Euler linear pass has staged (its
 // five conserved planes in lds, radius-2 halo): the same features, network and
 // decisions as k_learned_select, with the density from plane 0 and the
-// pressure written over plane 1 (the momenta are no longer needed), and each
+// pressure, formed from the staged planes, written over plane 1 (the momenta
+// are no longer needed), and each
 // cell's centre values, inverse deviations and range features passed to the
 // network lanes by shuffles within the wave instead of through LDS.
 template <int D>
@@ -382,8 +383,15 @@ __device__ void learned_tail(double *lds, const LearnArgs &la, const Grid &g, in
     const int ez = D == 3 ? box_out<Bx::BZ, H>(cz_) : 0;
     const int gi = i0 + a - H, gj = j0 + bb - H, gk = D == 3 ? k0 + cz_ - H : k0;
     // the cells the linear pass staged (every stencil cell of a live cell)
-    if (ex * ex + ey * ey + ez * ez <= H * H && gi < g.nx + H && gj < g.ny + H && (D == 2 || gk < g.nz + H))
-      sp[idx] = gm1 * la.eint[map_cell<D, 5>(g, gi, gj, gk).idx];
+    // from the five staged planes: E - |m|^2 / (2 rho) is even in the mirrored
+    // momenta, so it equals the limiter's per-cell plane bit for bit without
+    // a global load
+    if (ex * ex + ey * ey + ez * ez <= H * H && gi < g.nx + H && gj < g.ny + H && (D == 2 || gk < g.nz + H)) {
+      double st[5];
+#pragma unroll
+      for (int w = 0; w < 5; w++) st[w] = lds[w * NP + idx];
+      sp[idx] = gm1 * Hydro::eint(st);
+    }
   }
   __syncthreads();
   auto corner_of = [&](int t) {
