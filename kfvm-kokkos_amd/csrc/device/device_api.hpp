@@ -238,9 +238,11 @@ hipError_t launch_flow_stats(int dim, int nv, const double *U, const Grid &g, do
 // caller).  The source terms are formed inside launch_flux_rk (RKArgs src,
 // theta, Dm).
 hipError_t launch_mhd_chmax(int dim, const double *spd, const Grid &g, unsigned long long *chBits, hipStream_t s);
+// shape: the flux kernel's brick shape (2D: 0 = 16x16, 1 = 32x8, 2 = 64x4;
+// 3D: 0 = 16x4x4); the partial count follows it.
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
-                          const Grid &g, const RKArgs &a, hipStream_t s);
-int flux_rk_partials(int dim, const Grid &g);
+                          const Grid &g, const RKArgs &a, hipStream_t s, int shape = 0);
+int flux_rk_partials(int dim, const Grid &g, int shape = 0);
 // Batch of slices: per-slice finalize (out[2 s], out[2 s + 1]) over the flux
 // kernel's blocks, which are slice-major (blocksPerSlice each), and the copy of
 // the accepted slices' states (mask[s] != 0) from U to Uprev.

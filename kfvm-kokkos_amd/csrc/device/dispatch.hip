@@ -130,13 +130,14 @@ hipError_t launch_mhd_chmax(int dim, const double *spd, const Grid &g, unsigned 
 }
 
 hipError_t launch_flux_rk(int dim, int nq, int nv, const double *U, const double *FS,
-                          const Grid &g, const RKArgs &a, hipStream_t s) {
+                          const Grid &g, const RKArgs &a, hipStream_t s, int shape) {
   if (nv != 5 && nv != 1 && nv != 9) return hipErrorInvalidValue;
-  KFVM_SELECT(dim, nq, L::flux_rk(nv, U, FS, g, a, s));
+  if (shape < 0 || shape > (dim == 2 ? 2 : 0)) return hipErrorInvalidValue;
+  KFVM_SELECT(dim, nq, L::flux_rk(nv, U, FS, g, a, s, shape));
 }
 
-int flux_rk_partials(int dim, const Grid &g) {
-  const dim3 gr = dim == 3 ? Launch<3, 2>::flux_grid(g) : Launch<2, 2>::flux_grid(g);
+int flux_rk_partials(int dim, const Grid &g, int shape) {
+  const dim3 gr = dim == 3 ? Launch<3, 2>::flux_grid(g, 0) : Launch<2, 2>::flux_grid(g, shape);
   return int(gr.x * gr.y * gr.z);
 }
 

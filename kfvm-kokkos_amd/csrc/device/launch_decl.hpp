@@ -70,9 +70,10 @@ struct Launch {
   static hipError_t nad(int nv, const double *Uold, const double *Unew, double fr, double fv, double *nadFlag,
                         int *count, const Grid &g, hipStream_t s);
   static hipError_t flow_stats(int nv, const double *U, const Grid &g, double ifac, double *out, hipStream_t s);
-  static dim3 flux_grid(const Grid &g);  // one-dimensional: the kernel renumbers the bricks
+  // one-dimensional: the kernel renumbers the bricks; shape: the brick shape (2D: 0-2, 3D: 0)
+  static dim3 flux_grid(const Grid &g, int shape);
   static hipError_t flux_rk(int nv, const double *U, const double *FS, const Grid &g, const RKArgs &a,
-                            hipStream_t s);
+                            hipStream_t s, int shape);
 };
 
 extern template struct Launch<2, 2>;

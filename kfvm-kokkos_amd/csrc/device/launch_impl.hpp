@@ -2,6 +2,8 @@
 // launch_decl.hpp; included by the instantiation units inst_*.hip only.
 #pragma once
 
+#include <type_traits>
+
 #include "launch_decl.hpp"
 
 #include "stage_kernels.hpp"
@@ -248,24 +250,38 @@ hipError_t Launch<D, NQ>::flow_stats(int nv, const double *U, const Grid &g, dou
 }
 
 template <int D, int NQ>
-dim3 Launch<D, NQ>::flux_grid(const Grid &g) {
-  using Bk = dev::Brick<D>;
-  return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X) * unsigned((g.ny + Bk::Y - 1) / Bk::Y) *
-              unsigned((g.nz + Bk::Z - 1) / Bk::Z));
+dim3 Launch<D, NQ>::flux_grid(const Grid &g, int shape) {
+  auto count = [&](auto bk) {
+    using Bk = decltype(bk);
+    return dim3(unsigned((g.nx + Bk::X - 1) / Bk::X) * unsigned((g.ny + Bk::Y - 1) / Bk::Y) *
+                unsigned((g.nz + Bk::Z - 1) / Bk::Z));
+  };
+  if (D == 2 && shape == 1) return count(dev::Brick<D, 1>{});
+  if (D == 2 && shape == 2) return count(dev::Brick<D, 2>{});
+  return count(dev::Brick<D, 0>{});
 }
 
 template <int D, int NQ>
-hipError_t Launch<D, NQ>::flux_rk(int nv, const double *U, const double *FS, const Grid &g,
-                                  const RKArgs &a, hipStream_t s) {
-  using Bk = dev::Brick<D>;
-  const dim3 blk(Bk::X, Bk::Y, Bk::Z);
-  if (nv == 5)
-    hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 5>), flux_grid(g), blk, 0, s, U, FS, g, a);
-  else if (nv == 9)
-    hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 9>), flux_grid(g), blk, 0, s, U, FS, g, a);
-  else
-    hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 1>), flux_grid(g), blk, 0, s, U, FS, g, a);
-  return hipGetLastError();
+hipError_t Launch<D, NQ>::flux_rk(int nv, const double *U, const double *FS, const Grid &g, const RKArgs &a,
+                                  hipStream_t s, int shape) {
+  auto run = [&](auto shp) {
+    constexpr int S = decltype(shp)::value;
+    using Bk = dev::Brick<D, S>;
+    const dim3 blk(Bk::X, Bk::Y, Bk::Z);
+    if (nv == 5)
+      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 5, S>), flux_grid(g, S), blk, 0, s, U, FS, g, a);
+    else if (nv == 9)
+      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 9, S>), flux_grid(g, S), blk, 0, s, U, FS, g, a);
+    else
+      hipLaunchKernelGGL((dev::k_flux_rk<D, NQ, 1, S>), flux_grid(g, S), blk, 0, s, U, FS, g, a);
+    return hipGetLastError();
+  };
+  if constexpr (D == 2) {
+    if (shape == 1) return run(std::integral_constant<int, 1>{});
+    if (shape == 2) return run(std::integral_constant<int, 2>{});
+  }
+  if (shape != 0 && D == 3) return hipErrorInvalidValue;
+  return run(std::integral_constant<int, 0>{});
 }
 
 }  // namespace kfvm

@@ -635,10 +635,15 @@ __device__ inline int xcd_block(int b, int nb) {
 
 // Brick of cells handled by one workgroup of the flux kernel.  16 cells in x
 // make every row of a face-state plane one full 128-byte line, so no line is
-// fetched twice by the two bricks that share it.
-template <int D>
+// fetched twice by the two bricks that share it.  2D has two more shapes of
+// the same 256 cells, S = 1 (32x8) and S = 2 (64x4), picked per grid size by
+// the tile autotuner (kfvm_amd/autotune.py; init key tune:fluxbrick).
+template <int D, int S = 0>
 struct Brick {
-  static constexpr int X = 16, Y = D == 3 ? 4 : 16, Z = D == 3 ? 4 : 1;
+  static constexpr int X = D == 3 || S == 0 ? 16 : (S == 1 ? 32 : 64);
+  static constexpr int Y = D == 3 ? 4 : (S == 0 ? 16 : (S == 1 ? 8 : 4));
+  static constexpr int Z = D == 3 ? 4 : 1;
+  static_assert(X * Y * Z == BLK, "one cell per thread");
   static constexpr int NXF = (X + 1) * Y * Z, NYF = X * (Y + 1) * Z, NZF = D == 3 ? X * Y * (Z + 1) : 0;
   static constexpr int NF = NXF + NYF + NZF;
   // face work items with each direction padded to whole waves, so that no
@@ -698,11 +703,11 @@ __device__ void mhd_source_rhs(const double *__restrict__ U, const double *__res
                                const double *__restrict__ Dm, size_t c, int i, int j, int k, double divB,
                                double *rhs);
 
-template <int D, int NQ, int NV>
+template <int D, int NQ, int NV, int S = 0>
 __global__ __launch_bounds__(BLK) void k_flux_rk(const double *__restrict__ U,
                                                  const double *__restrict__ FS, Grid g,
                                                  RKArgs a) {
-  using Bk = Brick<D>;
+  using Bk = Brick<D, S>;
   constexpr int Q = Shape<D, NQ>::Q;
   __shared__ double fl[NV][Bk::NF];
   // GLM-MHD: normal field averaged over the two states of each face (div B)

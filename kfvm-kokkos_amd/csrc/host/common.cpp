@@ -312,6 +312,10 @@ bool apply_line(const std::string &category, const std::string &setting, Config 
       else if (key == "pdiconf") {
         // PDI yaml path: accepted, the HDF5 layout is built in
       } else return false;
+    } else if (category == "tune") {
+      // launch shapes chosen by the tile autotuner (not in the reference)
+      if (key == "fluxbrick") c.fluxBrick = std::stoi(val(1));
+      else return false;
     } else if (category == "problem") {
       // Replaces the reference's compile-time Definitions.H / ProblemSetup.cpp
       if (key == "name") {

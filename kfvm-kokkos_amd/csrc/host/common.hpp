@@ -97,6 +97,8 @@ struct Config {
   std::string baseName = "default", dataDir = "data";
   double plotFreq = -1.0;
   int ckptFreq = -1, statsFreq = -1, verbose = 0;
+  // tune:fluxbrick: the flux kernel's brick shape (2D: 0 = 16x16, 1 = 32x8, 2 = 64x4; 3D: 0)
+  int fluxBrick = 0;
   std::string restartFile;
 
   double dx() const { return (xUp - xLo) / double(nX); }

@@ -67,6 +67,8 @@ void Solver::setup() {
   if (cfg_.dim != 2 && cfg_.dim != 3) throw std::runtime_error("problem:dim must be 2 or 3");
   if (cfg_.rad != 2) throw std::runtime_error("only stencil radius 2 is built");
   if (cfg_.nQuad != 2 && cfg_.nQuad != 3) throw std::runtime_error("problem:nquad must be 2 or 3");
+  if (cfg_.fluxBrick < 0 || cfg_.fluxBrick > (cfg_.dim == 2 ? 2 : 0))
+    throw std::runtime_error("tune:fluxbrick must be 0, 1 or 2 in 2D and 0 in 3D");
   if (cfg_.dim == 2) cfg_.nZ = 1;
   const int64_t nmin = cfg_.dim == 3 ? std::min({cfg_.nX, cfg_.nY, cfg_.nZ}) : std::min(cfg_.nX, cfg_.nY);
   if (nmin < 2) throw std::runtime_error("every direction needs at least 2 cells");
@@ -193,7 +195,7 @@ void Solver::setup() {
     const size_t cellRows = nv_ == 9 ? size_t(wts_.nqCell) : 0;
     faceWC_ = dalloc<double>(size_t(SUB) * (FQ + wts_.I + cellRows), "WENO weights (compact)");
   }
-  nPart_ = flux_rk_partials(cfg_.dim, grid_);
+  nPart_ = flux_rk_partials(cfg_.dim, grid_, cfg_.fluxBrick);
   errPart_ = dalloc<double>(nPart_, "err partials");
   posPart_ = dalloc<double>(nPart_, "pos partials");
   redOut_ = dalloc<double>(2, "reduction");
@@ -719,7 +721,7 @@ void Solver::pipelineMhd(const double *Uin, const RKArgs &a0) {
   a.src = SRC_;
   a.theta = thNew;
   a.Dm = diffM_;
-  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, a, stream_); });
+  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, a, stream_, cfg_.fluxBrick); });
   nRhsEval++;
 }
 
@@ -837,7 +839,7 @@ void Solver::pipeline(const double *Uin, const RKArgs &a) {
     ar.pos = pos_;
     ar.ws = WS_;
   }
-  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, ar, stream_); });
+  timed("flux_rk", [&] { return launch_flux_rk(dim, nq, nv_, Uin, FS_, grid_, ar, stream_, cfg_.fluxBrick); });
   nRhsEval++;
 }
 

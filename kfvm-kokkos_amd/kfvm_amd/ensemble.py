@@ -34,13 +34,19 @@ def quadrant_text(n, t_end, nq=2, wenothresh=1.0):
 
 class Ensemble:
     """problems: list of (grid size n, seed); text(n) gives the init text of a
-    problem of size n."""
+    problem of size n.  tuning: {size: flux brick shape} (default: the table of
+    kfvm_amd.autotune, when one was written; {} leaves every batch at the
+    default 16x16 bricks)."""
 
-    def __init__(self, problems, text):
+    def __init__(self, problems, text, tuning=None):
+        from kfvm_amd.autotune import load_table, shape_for
+        table = load_table() if tuning is None else tuning
         self.groups = {}
         for k, (n, seed) in enumerate(problems):
             self.groups.setdefault(int(n), []).append((k, int(seed)))
-        self.batches = {n: kfvm_amd.Solver(init_text=text(n), seeds=[sd for _, sd in g])
+        self.shapes = {n: shape_for(n, table) for n in self.groups}
+        self.batches = {n: kfvm_amd.Solver(init_text=text(n) + f"tune:fluxbrick {self.shapes[n]}\n",
+                                           seeds=[sd for _, sd in g])
                         for n, g in sorted(self.groups.items())}
 
     def run(self):
@@ -95,7 +101,8 @@ def main():
     steps = {n: s.time[2] - 1 for n, s in e.batches.items()}
     print(json.dumps({"problems": a.problems, "cells": int(sum(int(n) ** 2 for n in sizes)), "per_size": per,
                       "t_end": a.t_end, "seconds": el, "zone_updates": zones,
-                      "mzone_updates_per_s": zones / el / 1e6, "steps_per_batch": steps}))
+                      "mzone_updates_per_s": zones / el / 1e6, "steps_per_batch": steps,
+                      "flux_brick": {n: e.shapes[n] for n in sorted(e.shapes)}}))
     e.close()
 
 

@@ -86,3 +86,13 @@ def test_solver_rejects_invalid_configuration_before_device_work():
                       ("mesh:resolution 8 8 8\ntime:final 0.1\n", "names no problem")]:
         with pytest.raises(kfvm_amd.KfvmError, match=msg):
             kfvm_amd.Solver(init_text=text)
+
+
+def test_flux_brick_shape_is_checked_before_any_device_work():
+    """tune:fluxbrick selects among the 2D flux-kernel brick shapes; 3D has one
+    shape, and the solver refuses another before it touches the device."""
+    import kfvm_amd
+    from helpers import config, init_text
+    text = init_text(config(n=(8, 8, 8))) + "tune:fluxbrick 1\n"
+    with pytest.raises(RuntimeError, match="tune:fluxbrick"):
+        kfvm_amd.Solver(init_text=text)

@@ -80,7 +80,7 @@ def test_ensemble_of_uneven_sizes_reproduces_its_members():
     from kfvm_amd.ensemble import Ensemble, quadrant_text
     import kfvm_amd
     problems = [(32, 2), (48, 3), (32, 4), (48, 5), (64, 6)]
-    e = Ensemble(problems, lambda n: quadrant_text(n, 0.02))
+    e = Ensemble(problems, lambda n: quadrant_text(n, 0.02), tuning={})  # the single runs' 16x16 bricks
     el, zones = e.run()
     assert zones > 0
     for k, (n, sd) in enumerate(problems):
@@ -157,3 +157,31 @@ def test_graph_replayed_steps_equal_eager_steps(kind, monkeypatch, capfd):
         assert cg["rejects_tol"] > 0
     # the graphs were captured (a failed capture warns and runs eagerly)
     assert "graph capture" not in capfd.readouterr().err
+
+
+@pytest.mark.parametrize("case", ["euler_batch_nq2", "mhd_nq3", "linadv"])
+def test_flux_brick_shapes_give_identical_states(case):
+    """The 2D flux kernel's brick shapes (tune:fluxbrick 0 = 16x16, 1 = 32x8,
+    2 = 64x4) tile the same faces: with a fixed step, three steps give the same
+    states bit for bit, on grids that are no multiple of the bricks."""
+    import kfvm_amd
+    from kfvm_amd.ensemble import quadrant_text
+    if case == "euler_batch_nq2":
+        base, seeds = quadrant_text(48, 1.0) + "time:fixeddt 0.002\n", [1, 2, 3]
+    else:
+        c = (config(dim=2, n=(40, 36, 1), nq=3, equation="mhd_glm", riemann="kepes", problem="orszag_tang",
+                    user=(), bc="periodic", gamma=5.0 / 3.0, fixeddt=0.001) if case == "mhd_nq3" else
+             config(dim=2, n=(44, 30, 1), equation="linadv", riemann="exact", problem="linadv_sine",
+                    user=(1.0, 1.0, 0.0), bc="periodic", advvel=(1.0, 1.0, 0.0), fixeddt=0.004))
+        base, seeds = init_text(c), None
+    states = []
+    for shape in (0, 1, 2):
+        s = kfvm_amd.Solver(init_text=base + f"tune:fluxbrick {shape}\n", seeds=seeds)
+        for _ in range(3):
+            assert s.take_step() == 0
+        states.append(s.state)
+        s.close()
+    for shape in (1, 2):
+        d = np.abs(states[shape] - states[0]).max()
+        print(f"[batch] {case}: brick shape {shape} vs 16x16: max diff {d:.3e}", flush=True)
+        assert d == 0.0
