@@ -269,7 +269,8 @@ struct MhdChunk {  // interior points per accumulation chunk
 // [SUB][NQ^D] (compact_weno_weights).  Staged in LDS once per workgroup
 // where it fits in 64 KB (the scalar cache cannot hold it: 43 KB in 3D at
 // NQ = 2); the 3D NQ = 3 table (85 KB) stays with the stencil-major tables
-// read by scalar loads.
+// read by scalar loads.  Like the Euler kernel it runs on LDS latency: three
+// waves per SIMD (168 VGPRs) beat the two that 256 VGPRs allow by 10 %.
 template <int D, int NQ>
 struct MWenoLds {
   static constexpr int SUB = WenoLds<D, NQ>::SUB, FQ = MShape<D, NQ>::FQ, I = SInfo<D>::I;
@@ -279,7 +280,7 @@ struct MWenoLds {
 };
 
 template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_mhd_wenorec(const double *__restrict__ U,
+__global__ __launch_bounds__(BLK, 3) void k_mhd_wenorec(const double *__restrict__ U,
                                                      double *__restrict__ FS,
                                                      double *__restrict__ SRC, Grid g,
                                                      const double *__restrict__ FT,

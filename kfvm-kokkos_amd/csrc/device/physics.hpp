@@ -2,7 +2,10 @@
 // primitive map, Riemann solvers.  Every expression keeps the operand order of
 // the reference so that, with contraction disabled, results agree with a CPU
 // evaluation of the same formulas to the last bit (library pow/exp aside; the
-// HLLC shear factor uses powu, a few ulp from the library pow).
+// HLLC shear factor uses powu, a few ulp from the library pow).  The HLLC and
+// the GLM-MHD HLL / LLF solvers divide and take square roots with div_n /
+// sqrt_n, which gave the correctly rounded result for each of 2^26 random
+// operands spread over 2^-40 .. 2^40.
 //
 //   Hydro EOS        src/physics/hydro/Hydro_EOS.H:15-93
 //   VarConvert       src/physics/hydro/Hydro_K.H:25-65
@@ -20,6 +23,45 @@
 
 namespace kfvm {
 namespace dev {
+
+// Reciprocal, quotient and square root of the positive, normal operands of
+// the HLLC solver, from the hardware reciprocal and reciprocal-square-root
+// estimates: two Newton-Raphson steps on the reciprocal, then one residual
+// correction of the quotient (Markstein, IA-64 and Elementary Functions, 2000,
+// ch. 8), and for the square root one Goldschmidt step followed by two Newton
+// corrections (ibid., ch. 9).  They skip the operand scaling of the general
+// IEEE sequences (no subnormal or overflowing operands reach them).
+__device__ inline double rcp_n(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ inline double div_n(double a, double b) {
+  const double r = rcp_n(b);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+__device__ inline double sqrt_n(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  g = fma(fma(-g, g, x), h, g);
+  g = fma(fma(-g, g, x), h, g);
+  return x == 0.0 ? x : g;  // rsq(0) = inf
+}
+// a / b and sqrt(x), the IEEE operations or (N) the sequences above
+template <bool N>
+__device__ inline double dv(double a, double b) {
+  return N ? div_n(a, b) : a / b;
+}
+template <bool N>
+__device__ inline double sq(double x) {
+  return N ? sqrt_n(x) : sqrt(x);
+}
 
 struct Hydro {
   static constexpr int NV = 5;
@@ -93,43 +135,44 @@ struct Hydro {
   }
   // HLLC with the low-Mach / shear corrections (Hydro_Riemann.H:97-187).  The
   // formulas are the reference's; divisions by a common denominator are
-  // multiplications by its reciprocal (11 divisions instead of 29), so results
-  // differ from the reference's operation order by rounding only.
+  // multiplications by its reciprocal (10 divisions instead of 29), and the
+  // divisions and square roots are rcp_n / div_n / sqrt_n, so results differ
+  // from the reference's by rounding only.
   template <int DIR>
   __device__ static inline double solve_hllc(double g, const double *uL, const double *uR,
                                              double *F) {
     constexpr int v0 = 1 + DIR, v1 = 1 + (DIR + 1) % 3, v2 = 1 + (DIR + 2) % 3;
     const double gm1 = g - 1.0;
     const double rhoL = uL[0], rhoR = uR[0];
-    const double irL = 1.0 / rhoL, irR = 1.0 / rhoR;
+    const double irL = rcp_n(rhoL), irR = rcp_n(rhoR);
     const double unL = uL[v0] * irL, vtL = uL[v1] * irL, wtL = uL[v2] * irL;
     const double unR = uR[v0] * irR, vtR = uR[v1] * irR, wtR = uR[v2] * irR;
     const double vsqL = unL * unL + vtL * vtL + wtL * wtL, vsqR = unR * unR + vtR * vtR + wtR * wtR;
     const double pL = gm1 * (uL[4] - 0.5 * rhoL * vsqL), pR = gm1 * (uR[4] - 0.5 * rhoR * vsqR);
-    const double aL = sqrt(g * pL * irL), aR = sqrt(g * pR * irR);
+    const double aL = sqrt_n(g * pL * irL), aR = sqrt_n(g * pR * irR);
     const double hL = (uL[4] + pL) * irL, hR = (uR[4] + pR) * irR;
     // Batten speeds from Roe averages
-    const double srL = sqrt(rhoL), srR = sqrt(rhoR), iS = 1.0 / (srL + srR);
+    const double srL = sqrt_n(rhoL), srR = sqrt_n(rhoR), iS = rcp_n(srL + srR);
     const double uTil = (unL * srL + unR * srR) * iS, vTil = (vtL * srL + vtR * srR) * iS;
     const double wTil = (wtL * srL + wtR * srR) * iS, hTil = (hL * srL + hR * srR) * iS;
     const double vsqTil = uTil * uTil + vTil * vTil + wTil * wTil;
-    const double aTil = sqrt(gm1 * (hTil - 0.5 * vsqTil));
+    const double aTil = sqrt_n(gm1 * (hTil - 0.5 * vsqTil));
     const double sL = fmin(unL - aL, uTil - aTil), sR = fmax(unR + aR, uTil + aTil);
     const double sMax = fmax(fabs(sL), fabs(sR));
-    double M = fmax(sqrt(vsqL) / aL, sqrt(vsqR) / aR);
+    double M = fmax(div_n(sqrt_n(vsqL), aL), div_n(sqrt_n(vsqR), aR));
     const double psL0 = rhoL * (sL - unL), psR0 = rhoR * (sR - unR);
-    const double sS = ((pR - pL) + (psL0 * unL - psR0 * unR)) / (psL0 - psR0);
+    const double sS = div_n((pR - pL) + (psL0 * unL - psR0 * unR), psL0 - psR0);
     const double pS = 0.5 * ((pL + pR) + psL0 * (sS - unL) + psR0 * (sS - unR));
-    const double iL = 1.0 / (sL - sS), iR = 1.0 / (sR - sS);
+    const double iL = rcp_n(sL - sS), iR = rcp_n(sR - sS);
     const double sFacL = sS * iL, sFacR = sS * iR;
     const double uJump = unR - unL, vJump = vtR - vtL, wJump = wtR - wtL;
-    const double psiFac = psR0 * psL0 / (psR0 - psL0);
-    const double presRat = pL < pR ? pL / pR : pR / pL;
+    const double psiFac = div_n(psR0 * psL0, psR0 - psL0);
+    const double presRat = div_n(pL < pR ? pL : pR, pL < pR ? pR : pL);
     const double gFac = 1.0 - powu(presRat, M);  // presRat in (0, 1], M >= 0
     const double shFacL = psiFac * gFac * sL * iL, shFacR = psiFac * gFac * sR * iR;
     M = fmin(M, 1.0);
     const double oneMsq = 1.0 - M * M;
-    const double fM = M * sqrt(4.0 + oneMsq * oneMsq) / (1.0 + M * M);
+    const double fM = div_n(M * sqrt_n(4.0 + oneMsq * oneMsq), 1.0 + M * M);
     const double pFacL = psiFac * (fM - 1.0) * uJump + sL * pS * iL;
     const double pFacR = psiFac * (fM - 1.0) * uJump + sR * pS * iR;
     // regions of the reference: 0 <= sL: F(uL); else 0 <= sS: star L;
@@ -161,35 +204,43 @@ struct Hydro {
 // normal direction as ord = {DIR, DIR+1, DIR+2} (mod 3).
 struct Mhd {
   static constexpr int NV = 9;
+  // N: divisions and square roots by div_n / sqrt_n (the Riemann solvers)
+  template <bool N = false>
   __device__ static inline double ekin(const double *U) {
-    return 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0];
+    return dv<N>(0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]), U[0]);
   }
   __device__ static inline double prsb(const double *U) {
     return 0.5 * (U[5] * U[5] + U[6] * U[6] + U[7] * U[7]);
   }
+  template <bool N = false>
   __device__ static inline double eint(const double *U) {
-    return (U[4] - prsb(U) - ekin(U) - 0.5 * U[8] * U[8]);
+    return (U[4] - prsb(U) - ekin<N>(U) - 0.5 * U[8] * U[8]);
   }
-  __device__ static inline double pres(double g, const double *U) { return (g - 1.0) * eint(U); }
+  template <bool N = false>
+  __device__ static inline double pres(double g, const double *U) {
+    return (g - 1.0) * eint<N>(U);
+  }
   __device__ static inline double sound(double g, const double *U) {
     return sqrt(g * pres(g, U) / U[0]);
   }
   __device__ static inline double weno(double g, const double *U) {
     return pres(g, U) / pow(U[0], g);
   }
+  template <bool N = false>
   __device__ static inline double fast(double g, const double *U, double bl) {
-    const double asq = g * pres(g, U) / U[0];
-    const double bsq = 2.0 * prsb(U) / U[0];
-    const double blsq = bl * bl / U[0];
+    const double asq = dv<N>(g * pres<N>(g, U), U[0]);
+    const double bsq = dv<N>(2.0 * prsb(U), U[0]);
+    const double blsq = dv<N>(bl * bl, U[0]);
     const double absq = asq + bsq;
-    return sqrt(0.5 * (absq + sqrt(absq * absq - 4.0 * asq * blsq)));
+    return sq<N>(0.5 * (absq + sq<N>(absq * absq - 4.0 * asq * blsq)));
   }
+  template <bool N = false>
   __device__ static inline void cons2prim(double g, const double *U, double *V) {
     V[0] = U[0];
-    V[1] = U[1] / U[0];
-    V[2] = U[2] / U[0];
-    V[3] = U[3] / U[0];
-    V[4] = pres(g, U);
+    V[1] = dv<N>(U[1], U[0]);
+    V[2] = dv<N>(U[2], U[0]);
+    V[3] = dv<N>(U[3], U[0]);
+    V[4] = pres<N>(g, U);
     V[5] = U[5];
     V[6] = U[6];
     V[7] = U[7];
@@ -222,14 +273,14 @@ struct Mhd {
                                          const double *vL, const double *vR, double &sL,
                                          double &sR) {
     constexpr int v0 = 1 + DIR, b0 = 5 + DIR, b1 = 5 + (DIR + 1) % 3, b2 = 5 + (DIR + 2) % 3;
-    const double cfL = fast(g, uL, uL[b0]);
-    const double cfR = fast(g, uR, uR[b0]);
+    const double cfL = fast<true>(g, uL, uL[b0]);
+    const double cfR = fast<true>(g, uR, uR[b0]);
     const double sdL = fmin(vL[v0] - fmax(ch, cfL), vR[v0] - fmax(ch, cfR));
     const double sdR = fmax(vL[v0] + fmax(ch, cfL), vR[v0] + fmax(ch, cfR));
-    const double rtL = sqrt(uL[0]), rtR = sqrt(uR[0]);
-    const double rtSum = rtL + rtR, vSum = (rtL * vL[v0] + rtR * vR[v0]) / rtSum;
+    const double rtL = sqrt_n(uL[0]), rtR = sqrt_n(uR[0]);
+    const double rtSum = rtL + rtR, vSum = div_n(rtL * vL[v0] + rtR * vR[v0], rtSum);
     const double bd0 = uL[b0] - uR[b0], bd1 = uL[b1] - uR[b1], bd2 = uL[b2] - uR[b2];
-    const double bnDiff = sqrt(bd0 * bd0 + bd1 * bd1 + bd2 * bd2) / rtSum;
+    const double bnDiff = div_n(sqrt_n(bd0 * bd0 + bd1 * bd1 + bd2 * bd2), rtSum);
     const double aL = fmin(vL[v0], vSum) - cfL - bnDiff;
     const double aR = fmin(vR[v0], vSum) + cfR + bnDiff;
     sL = fmin(sdL, aL);
@@ -241,16 +292,16 @@ struct Mhd {
   __device__ static inline double solve_hll(double g, double ch, bool llf, const double *uL,
                                             const double *uR, double *F) {
     double vL[9], vR[9], FR[9];
-    cons2prim(g, uL, vL);
-    cons2prim(g, uR, vR);
+    cons2prim<true>(g, uL, vL);
+    cons2prim<true>(g, uR, vR);
     double sL, sR;
     const double sMax = speeds<DIR>(g, ch, uL, uR, vL, vR, sL, sR);
     if (llf) {
       sL = -sMax;
       sR = sMax;
     }
-    const double tR = (fmin(0.0, sR) - fmin(0.0, sL)) / (sR - sL), tL = 1.0 - tR;
-    const double tD = 0.5 * (sR * fabs(sL) - sL * fabs(sR)) / (sR - sL);
+    const double tR = div_n(fmin(0.0, sR) - fmin(0.0, sL), sR - sL), tL = 1.0 - tR;
+    const double tD = div_n(0.5 * (sR * fabs(sL) - sL * fabs(sR)), sR - sL);
     flux<DIR>(ch, uL, vL, F);
     flux<DIR>(ch, uR, vR, FR);
 #pragma unroll

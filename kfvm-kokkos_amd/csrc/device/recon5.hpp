@@ -525,7 +525,10 @@ struct WenoChunk {  // face points per accumulation chunk
 // from there: the scalar cache (16 KB) cannot hold them (36 KB in 3D at NQ=2),
 // so scalar loads of them kept missing to L2 with the next FMAs waiting on
 // them.  The grid is persistent (at most wenoBlocks workgroups striding over
-// the list), so the staging is paid once per workgroup.
+// the list), so the staging is paid once per workgroup.  Every lane streams
+// one weight per FMA out of LDS, so the kernel lives on LDS latency: three
+// waves per SIMD (at most 168 VGPRs, no spills) run it 11-14 % faster than the
+// two that 256 VGPRs allow; four (128 VGPRs) spill.
 template <int D, int NQ>
 struct WenoLds {
   static constexpr int SUB = SInfo<D>::C + SInfo<D>::Cc + (SInfo<D>::nSub - 2) * SInfo<D>::Cb;
@@ -534,7 +537,7 @@ struct WenoLds {
 };
 
 template <int D, int NQ>
-__global__ __launch_bounds__(BLK) void k_wenorec5(const double *__restrict__ U,
+__global__ __launch_bounds__(BLK, 3) void k_wenorec5(const double *__restrict__ U,
                                                   double *__restrict__ FS, Grid g,
                                                   const double *__restrict__ WC,
                                                   const int *__restrict__ list,
