@@ -54,10 +54,17 @@ hipError_t Launch<D, NQ>::weno(int nv, const double *U, double *FS, double *thet
                                double *WS, int cap, const double *WC, hipStream_t s) {
   if (nv == 5) {
     // persistent: every workgroup stages the weights once, then strides over the list
-    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 2048u);
     if (!WC) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list, counter,
-                       WS, WS ? cap : 0);
+    if constexpr (dev::WenoMfma<D, NQ>::USE) {
+      constexpr int cellsPerBlock = 16 * dev::WenoMfma<D, NQ>::WPB;
+      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + cellsPerBlock - 1) / cellsPerBlock), 2048u);
+      hipLaunchKernelGGL((dev::k_wenorec5_mfma<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list,
+                         counter, WS, WS ? cap : 0);
+    } else {
+      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 2048u);
+      hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list, counter,
+                         WS, WS ? cap : 0);
+    }
   } else {
     hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
                        dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);

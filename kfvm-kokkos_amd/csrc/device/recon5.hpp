@@ -622,6 +622,233 @@ __global__ __launch_bounds__(BLK, 3) void k_wenorec5(const double *__restrict__ 
   }
 }
 
+// KFVM-WENO on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  The weight
+// products of the lane-per-(cell, var) kernel above stream one LDS weight
+// per FMA and every lane of a wave reads the same weight, so that kernel is
+// bound by LDS bandwidth.  Here a wave takes 16 listed cells (the 16 columns
+// of the products) one variable at a time and each substencil S multiplies
+// its weight matrix A_S (rows: the I smoothness derivatives, then the F*Q face
+// points; columns: the substencil's cells) by the cells' linearised values:
+//   acc_S = A_S V_S,  beta_S = sum_i acc_S[i]^2 (i < I),
+//   out   = sum_S om_S acc_S[I + h]
+// with the WENO-AO weights om_S of the scalar kernel rewritten as
+//   om_S = c_S / tot, c_0 = w_0 / g_0, c_S = w_S - c_0 g_S, w_S = g_S / (beta_S^2 + 1e-40),
+//   tot = sum_S w_S,
+// so each substencil's part is added as soon as its beta is known.  The
+// weight fragments (A: lane l holds row l % 16, column l / 16 of a 16x4
+// block) are staged in LDS once per workgroup in lane order; the values (B:
+// lane l holds stencil row l / 16 of column l % 16) are written to a per-wave
+// LDS plane by the lanes that load them; element q of the result holds row
+// 4 q + l / 16 of column l % 16.  Sums run in a different order than
+// the reference's loops: results agree to rounding.
+template <int D, int NQ>
+struct WenoMfma {
+  static constexpr int C = SInfo<D>::C, I = SInfo<D>::I, nSub = SInfo<D>::nSub;
+  static constexpr int FQ = Shape<D, NQ>::F * Shape<D, NQ>::Q;
+  static constexpr int RT = (I + FQ + 15) / 16;  // 16-row tiles of [derivatives; faces]
+  static constexpr int CP = (C + 3) / 4 * 4;     // stencil rows of the value plane
+  static constexpr int kt(int S) { return ((S == 0 ? C : (S == 1 ? SInfo<D>::Cc : SInfo<D>::Cb)) + 3) / 4; }
+  static constexpr int fragOff(int S) { return S == 0 ? 0 : fragOff(S - 1) + RT * kt(S - 1); }
+  static constexpr int NFRAG = fragOff(nSub);
+  static constexpr int WPB = BLK / 64;  // waves per block
+  static constexpr int LDS = (NFRAG * 64 + WPB * CP * 16) * int(sizeof(double));
+  // 3D at NQ = 2 (two workgroups per CU): 17 % faster than the scalar kernel
+  // on the listed cells of the 256^3 planar Riemann run.  In 2D (13-cell
+  // stencils: 14 small products per variable) and at 3D NQ = 3 (74 KB of
+  // fragments, one workgroup per CU) the scalar kernel is faster.
+  static constexpr bool USE = D == 3 && NQ == 2 && LDS <= 80 * 1024;
+};
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// stencil index of row m of substencil S (m beyond the substencil: the
+// centre cell, whose weight there is zero)
+template <int D, int S>
+__device__ constexpr int msub(int m) {
+  return m < SubSten<D, S>::n ? SubSten<D, S>::idx(m) : 0;
+}
+
+template <int D, int NQ, int S>
+struct MfmaSub {
+  using M = WenoMfma<D, NQ>;
+  // acc = A_S V_S over the substencil's column tiles; vp: this wave's value
+  // plane, grp = lane / 16, col = lane % 16
+  __device__ __attribute__((always_inline)) static inline void run(const double *wf, const double *vp, int lane,
+                                                                   int grp, int col, v4d (&acc)[M::RT]) {
+#pragma unroll
+    for (int r = 0; r < M::RT; r++) acc[r] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < M::kt(S); k++) {
+      const int m0 = msub<D, S>(4 * k), m1 = msub<D, S>(4 * k + 1), m2 = msub<D, S>(4 * k + 2),
+                m3 = msub<D, S>(4 * k + 3);
+      const int m = grp == 0 ? m0 : (grp == 1 ? m1 : (grp == 2 ? m2 : m3));
+      const double b = vp[m * 16 + col];
+#pragma unroll
+      for (int r = 0; r < M::RT; r++) {
+        const double a = wf[(M::fragOff(S) + k * M::RT + r) * 64 + lane];
+        acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[r], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);  // bounded registers: one column tile in flight
+    }
+  }
+};
+
+template <int D, int NQ>
+__global__ __launch_bounds__(BLK, 2) void k_wenorec5_mfma(const double *__restrict__ U,
+                                                        double *__restrict__ FS, Grid g,
+                                                        const double *__restrict__ WC,
+                                                        const int *__restrict__ list,
+                                                        const int *__restrict__ counter,
+                                                        double *__restrict__ WS, int cap) {
+  using M = WenoMfma<D, NQ>;
+  constexpr auto ST = make_stencil_tables<D>();
+  constexpr int C = M::C, I = M::I, nSub = M::nSub, FQ = M::FQ, RT = M::RT, CP = M::CP;
+  constexpr int SUB = WenoLds<D, NQ>::SUB;
+  constexpr double gHi = 0.7, gLo = 0.7;
+  constexpr double g0 = gHi, g1 = (1.0 - gHi) * gLo, gb = (1.0 - gHi) * (1.0 - gLo) / double(2 * D);
+  __shared__ double wf[M::NFRAG * 64];
+  __shared__ double vplane[M::WPB][CP * 16];
+  // weight fragments in lane order: fragment (S, k, r) holds rows 16 r .. 16 r + 15
+  // and substencil columns 4 k .. 4 k + 3
+  for (int e = threadIdx.x; e < M::NFRAG * 64; e += BLK) {
+    const int f = e >> 6, l = e & 63;
+    int S = 0;
+#pragma unroll
+    for (int q = 1; q < nSub; q++) S += f >= M::fragOff(q) ? 1 : 0;
+    const int ns = S == 0 ? C : (S == 1 ? SInfo<D>::Cc : SInfo<D>::Cb);
+    const int off = S == 0 ? 0 : (S == 1 ? C : C + SInfo<D>::Cc + (S - 2) * SInfo<D>::Cb);
+    const int fl = f - M::fragOff(S), k = fl / RT, r = fl % RT;
+    const int row = 16 * r + (l & 15), m = 4 * k + (l >> 4);
+    double w = 0.0;
+    if (m < ns) {
+      if (row < I) w = WC[size_t(SUB) * FQ + size_t(off + m) * I + row];
+      else if (row < I + FQ) w = WC[size_t(off + m) * FQ + (row - I)];
+    }
+    wf[e] = w;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, grp = lane >> 4, col = lane & 15;
+  double *const vp = vplane[threadIdx.x >> 6];
+  const int n = list ? *counter : g.ncell;
+  const int waveId = blockIdx.x * M::WPB + (threadIdx.x >> 6);
+  const int nWaves = gridDim.x * M::WPB;
+  const size_t nc = g.ncell;
+  constexpr int NJ = (C + 3) / 4;  // stencil rows per lane group
+#pragma unroll 1
+  for (int t0 = waveId * 16; t0 < n; t0 += nWaves * 16) {
+    const int t = t0 + col;
+    const bool valid = t < n;
+    const int tt = valid ? t : n - 1;
+    const int c = list ? list[tt] : tt;
+    int i, j, k;
+    cell_ijk<D>(c, g, i, j, k);
+    double ref[5];
+#pragma unroll
+    for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
+    const LinPrim lp(ref, g.gamma);
+    // linearised values of all five variables at this lane's stencil rows
+    // (grp, grp + 4, ...), loaded together once per group of cells
+    double lv[5][NJ];
+#pragma unroll
+    for (int jj = 0; jj < NJ; jj++) {
+      const int m = grp + 4 * jj < C ? grp + 4 * jj : 0;
+      const CellRef r = map_cell<D, 5>(g, i + ST.off[m][0], j + ST.off[m][1], k + ST.off[m][2]);
+      const double s0 = ld(U, nc, r, 0), s1 = ld(U, nc, r, 1), s2 = ld(U, nc, r, 2), s3 = ld(U, nc, r, 3),
+                   s4 = ld(U, nc, r, 4);
+      lv[0][jj] = s0;
+      lv[1][jj] = (s1 - lp.u * s0) / lp.rho;
+      lv[2][jj] = (s2 - lp.v * s0) / lp.rho;
+      lv[3][jj] = (s3 - lp.w * s0) / lp.rho;
+      lv[4][jj] = lp.gamm * (lp.vsq * s0 - lp.u * s1 - lp.v * s2 - lp.w * s3 + s4);
+    }
+    const bool compact = list && t < cap;
+    double *const P = compact ? WS + size_t(tt) * (FQ * 5) : FS + fs_base(c, FQ * 5);
+    const size_t ps = compact ? 1 : FSB;
+#pragma unroll 1
+    for (int v = 0; v < 5; v++) {
+      int gq = grp;
+      asm volatile("" : "+v"(gq));
+      double vals[NJ];
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++)
+        vals[jj] = v == 0 ? lv[0][jj] : (v == 1 ? lv[1][jj] : (v == 2 ? lv[2][jj] : (v == 3 ? lv[3][jj] : lv[4][jj])));
+      // the previous variable's reads of the plane are done before it is rewritten
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++)
+        if (grp + 4 * jj < CP) vp[(grp + 4 * jj) * 16 + col] = vals[jj];
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      v4d out[RT], acc[RT];
+#pragma unroll
+      for (int r = 0; r < RT; r++) out[r] = v4d{0.0, 0.0, 0.0, 0.0};
+      double tot = 0.0, c0 = 0.0;
+      auto fold = [&](int S) {
+        // beta of this column: rows < I of acc, summed over the lane group
+        double b = 0.0;
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int row = 16 * r + 4 * q + grp;
+            b = row < I ? fma(acc[r][q], acc[r][q], b) : b;
+          }
+        b += __shfl_xor(b, 16);
+        b += __shfl_xor(b, 32);
+        const double gs = S == 0 ? g0 : (S == 1 ? g1 : gb);
+        const double w = gs / (b * b + 1.e-40);
+        tot += w;
+        double cs;
+        if (S == 0) {
+          c0 = w / g0;
+          cs = c0;
+        } else {
+          cs = w - c0 * gs;
+        }
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) out[r][q] = fma(cs, acc[r][q], out[r][q]);
+      };
+      __builtin_amdgcn_sched_barrier(0);
+      MfmaSub<D, NQ, 0>::run(wf, vp, lane, gq, col, acc);
+      fold(0);
+      __builtin_amdgcn_sched_barrier(0);
+      MfmaSub<D, NQ, 1>::run(wf, vp, lane, gq, col, acc);
+      fold(1);
+      __builtin_amdgcn_sched_barrier(0);
+      MfmaSub<D, NQ, 2>::run(wf, vp, lane, gq, col, acc);
+      fold(2);
+      __builtin_amdgcn_sched_barrier(0);
+      MfmaSub<D, NQ, 3>::run(wf, vp, lane, gq, col, acc);
+      fold(3);
+      __builtin_amdgcn_sched_barrier(0);
+      MfmaSub<D, NQ, 4>::run(wf, vp, lane, gq, col, acc);
+      fold(4);
+      __builtin_amdgcn_sched_barrier(0);
+      MfmaSub<D, NQ, 5>::run(wf, vp, lane, gq, col, acc);
+      fold(5);
+      if constexpr (nSub > 6) {
+        __builtin_amdgcn_sched_barrier(0);
+        MfmaSub<D, NQ, 6>::run(wf, vp, lane, gq, col, acc);
+        fold(6);
+        __builtin_amdgcn_sched_barrier(0);
+        MfmaSub<D, NQ, 7>::run(wf, vp, lane, gq, col, acc);
+        fold(7);
+      }
+      const double it = 1.0 / tot;
+      if (valid) {
+#pragma unroll
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int h = 16 * r + 4 * q + grp - I;
+            if (h >= 0 && h < FQ) P[(size_t(h) * 5 + v) * ps] = out[r][q] * it;
+          }
+      }
+    }
+  }
+}
+
 // Positivity limiter (Common_K.H:299-643), exact sequence, for the cells
 // whose linear states failed the fast test of k_linear_fused, skipping the
 // cells the selector handed to WENO (k_weno_post limits their new states).
