@@ -101,8 +101,8 @@ def flop_flux(nq):  # 2,620 at NQ = 2, 5,845 at NQ = 3
 # divisions and square roots count their Newton steps), and the fraction of
 # time the VALU is busy (SQ_ACTIVE_INST_VALU x 4 / SIMDs / cycles at the
 # kernel's traced duration), from the counter profile.
-COUNTERS = {"flux_rk": {"fp64_flop_per_cell": 8281.0, "valu_busy": 0.71},
-            "linear": {"fp64_flop_per_cell": 8836.0, "valu_busy": 0.58}}
+COUNTERS = {"flux_rk": {"fp64_flop_per_cell": 7754.0, "valu_busy": 0.64},
+            "linear": {"fp64_flop_per_cell": 8946.0, "valu_busy": 0.58}}
 COUNTERS_SOURCE = "profiles/fp64_counters_riemann3d_256.md"
 
 
