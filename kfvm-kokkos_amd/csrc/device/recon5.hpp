@@ -693,6 +693,28 @@ struct MfmaSub {
   }
 };
 
+// Offsets of stencil rows g, g + 4, g + 8, ... (rows past the stencil: the
+// centre), three rows per word at 9 bits each ((dx + 2) | (dy + 2) << 3 |
+// (dz + 2) << 6), so a lane keeps its rows' offsets in three registers.
+template <int D>
+struct MfmaRows {
+  static constexpr int C = SInfo<D>::C, NJ = (C + 3) / 4, NW = (NJ + 2) / 3;
+  unsigned w[4][NW];
+};
+template <int D>
+constexpr MfmaRows<D> make_mfma_rows() {
+  MfmaRows<D> t{};
+  constexpr auto ST = make_stencil_tables<D>();
+  for (int g = 0; g < 4; g++)
+    for (int jj = 0; jj < MfmaRows<D>::NJ; jj++) {
+      const int m = g + 4 * jj < MfmaRows<D>::C ? g + 4 * jj : 0;
+      const unsigned code = unsigned(ST.off[m][0] + 2) | unsigned(ST.off[m][1] + 2) << 3 |
+                            unsigned(ST.off[m][2] + 2) << 6;
+      t.w[g][jj / 3] |= code << (9 * (jj % 3));
+    }
+  return t;
+}
+
 template <int D, int NQ>
 __global__ __launch_bounds__(BLK, 2) void k_wenorec5_mfma(const double *__restrict__ U,
                                                         double *__restrict__ FS, Grid g,
@@ -734,6 +756,11 @@ __global__ __launch_bounds__(BLK, 2) void k_wenorec5_mfma(const double *__restri
   const int nWaves = gridDim.x * M::WPB;
   const size_t nc = g.ncell;
   constexpr int NJ = (C + 3) / 4;  // stencil rows per lane group
+  constexpr auto RW = make_mfma_rows<D>();
+  unsigned rowWord[MfmaRows<D>::NW];
+#pragma unroll
+  for (int q = 0; q < MfmaRows<D>::NW; q++)
+    rowWord[q] = grp == 0 ? RW.w[0][q] : (grp == 1 ? RW.w[1][q] : (grp == 2 ? RW.w[2][q] : RW.w[3][q]));
 #pragma unroll 1
   for (int t0 = waveId * 16; t0 < n; t0 += nWaves * 16) {
     const int t = t0 + col;
@@ -748,11 +775,20 @@ __global__ __launch_bounds__(BLK, 2) void k_wenorec5_mfma(const double *__restri
     const LinPrim lp(ref, g.gamma);
     // linearised values of all five variables at this lane's stencil rows
     // (grp, grp + 4, ...), loaded together once per group of cells
+    // (the offsets are unpacked again per group of cells: kept unpacked across
+    // the loop they would take 27 registers and spill)
+    unsigned rw[MfmaRows<D>::NW];
+#pragma unroll
+    for (int q = 0; q < MfmaRows<D>::NW; q++) {
+      rw[q] = rowWord[q];
+      asm volatile("" : "+v"(rw[q]));
+    }
     double lv[5][NJ];
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
-      const int m = grp + 4 * jj < C ? grp + 4 * jj : 0;
-      const CellRef r = map_cell<D, 5>(g, i + ST.off[m][0], j + ST.off[m][1], k + ST.off[m][2]);
+      const unsigned code = (rw[jj / 3] >> (9 * (jj % 3))) & 511u;
+      const int dx = int(code & 7u) - 2, dy = int((code >> 3) & 7u) - 2, dz = int(code >> 6) - 2;
+      const CellRef r = map_cell<D, 5>(g, i + dx, j + dy, k + dz);
       const double s0 = ld(U, nc, r, 0), s1 = ld(U, nc, r, 1), s2 = ld(U, nc, r, 2), s3 = ld(U, nc, r, 3),
                    s4 = ld(U, nc, r, 4);
       lv[0][jj] = s0;
