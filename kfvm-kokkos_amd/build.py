@@ -2,8 +2,9 @@
 
 Produces kfvm-kokkos_amd/build/libkfvm_amd.so (HIP kernels for gfx950 plus the
 C++17 host driver behind the C ABI of include/kfvm.h), the command-line
-driver kfvm-kokkos_amd/build/kfvm.ex and the FP64 throughput probe
-kfvm-kokkos_amd/build/fp64_peak.
+driver kfvm-kokkos_amd/build/kfvm.ex, the FP64 throughput probe
+kfvm-kokkos_amd/build/fp64_peak and the division / square-root check
+kfvm-kokkos_amd/build/div_sqrt_check.
 
     python kfvm-kokkos_amd/build.py [--force]
 """
@@ -19,6 +20,7 @@ BUILD = os.path.join(HERE, "build")
 LIB = os.path.join(BUILD, "libkfvm_amd.so")
 EXE = os.path.join(BUILD, "kfvm.ex")
 PROBE = os.path.join(BUILD, "fp64_peak")
+DIV_SQRT_CHECK = os.path.join(BUILD, "div_sqrt_check")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -99,6 +101,10 @@ def build(force=False, verbose=False):
     probe_src = os.path.join(CSRC, "probe", "fp64_peak.hip")
     if force or _stale(PROBE, probe_src, []):
         _run([HIPCC] + [f for f in HIP_FLAGS if f not in ("-x", "hip", "-fPIC")] + ["-o", PROBE, probe_src])
+    # the Riemann solvers' division and square root against IEEE (tests/test_gpu_div_sqrt.py)
+    check_src = os.path.join(CSRC, "probe", "div_sqrt_check.hip")
+    if force or _stale(DIV_SQRT_CHECK, check_src, [os.path.join(CSRC, "device", "physics.hpp")]):
+        _run([HIPCC] + [f for f in HIP_FLAGS if f not in ("-x", "hip", "-fPIC")] + ["-o", DIV_SQRT_CHECK, check_src])
     return LIB
 
 

@@ -4,8 +4,8 @@
 // evaluation of the same formulas to the last bit (library pow/exp aside; the
 // HLLC shear factor uses powu, a few ulp from the library pow).  The HLLC and
 // the GLM-MHD HLL / LLF solvers divide and take square roots with div_n /
-// sqrt_n, which gave the correctly rounded result for each of 2^26 random
-// operands spread over 2^-40 .. 2^40.
+// sqrt_n, which give the IEEE result for each of 2^26 random operands of both
+// signs spread over 2^-40 .. 2^40 (probe/div_sqrt_check.hip).
 //
 //   Hydro EOS        src/physics/hydro/Hydro_EOS.H:15-93
 //   VarConvert       src/physics/hydro/Hydro_K.H:25-65
