@@ -2,6 +2,7 @@
 // launch_decl.hpp; included by the instantiation units inst_*.hip only.
 #pragma once
 
+#include <cstdlib>
 #include <type_traits>
 
 #include "launch_decl.hpp"
@@ -56,15 +57,20 @@ hipError_t Launch<D, NQ>::weno(int nv, const double *U, double *FS, double *thet
     // persistent: every workgroup stages the weights once, then strides over the list
     if (!WC) return hipErrorInvalidValue;
     if constexpr (dev::WenoMfma<D, NQ>::USE) {
-      constexpr int cellsPerBlock = 16 * dev::WenoMfma<D, NQ>::WPB;
-      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + cellsPerBlock - 1) / cellsPerBlock), 2048u);
-      hipLaunchKernelGGL((dev::k_wenorec5_mfma<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list,
-                         counter, WS, WS ? cap : 0);
-    } else {
-      const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 2048u);
-      hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list, counter,
-                         WS, WS ? cap : 0);
+      // KFVM_WENO_SCALAR=1 (read at every launch) runs the scalar kernel
+      // instead: the two are compared in the tests
+      const char *ev = std::getenv("KFVM_WENO_SCALAR");
+      if (!(ev && ev[0] == '1')) {
+        constexpr int cellsPerBlock = 16 * dev::WenoMfma<D, NQ>::WPB;
+        const unsigned nb = std::min<unsigned>(unsigned((g.ncell + cellsPerBlock - 1) / cellsPerBlock), 2048u);
+        hipLaunchKernelGGL((dev::k_wenorec5_mfma<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list,
+                           counter, WS, WS ? cap : 0);
+        return hipGetLastError();
+      }
     }
+    const unsigned nb = std::min<unsigned>(unsigned((g.ncell + dev::CPB - 1) / dev::CPB), 2048u);
+    hipLaunchKernelGGL((dev::k_wenorec5<D, NQ>), dim3(nb), dim3(dev::BLK), 0, s, U, FS, g, WC, list, counter, WS,
+                       WS ? cap : 0);
   } else {
     hipLaunchKernelGGL((dev::k_weno<D, NQ, 1>), dim3(std::min<unsigned>(stage_blocks(g.ncell), 1u << 16)),
                        dim3(dev::BLK), 0, s, U, FS, thetaNew, g, FW, DW, list, counter);
