@@ -78,6 +78,10 @@ def flop_weno(nq):  # 47,100 at NQ = 2, 78,900 at NQ = 3 (Q = NQ^2 points per fa
 def flop_linear(nq):  # 7,920 at NQ = 2, 17,820 at NQ = 3
     return 2 * 6 * nq * nq * 5 * 33
 BYTES_STAGE = 40 * 5                           # read U_s, Uprev, Uhat; write U_s+1, Uhat
+
+
+def bytes_linear(nq):  # 1,000 at NQ = 2: read U, write the 6 nq^2 face states of 5 variables
+    return 40 + 8 * 5 * 6 * nq * nq
 # Whole-path FP64 work per zone-update of the same configuration, linear path
 # and full WENO (reconstruction, smoothness, WENO faces, Riemann solves at
 # ~200 flop-equivalents each, divergence and RK update): the path's FP64
@@ -306,9 +310,11 @@ def _roofline(kern, head, ncell, n, nq):
     roof["traffic"] = tr[0] if tr else None
     roof["traffic_source"] = tr[1] if tr else None
     roof["kernel"] = dom
-    # the algorithmic register traffic (200 B per cell) against HBM
-    roof["hbm"] = {"achieved": BYTES_STAGE * ncell / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                   "frac": BYTES_STAGE * ncell / t / 1e9 / PEAK_HBM_GBS,
+    # the kernel's algorithmic traffic against HBM: the RK stage's 200 B per
+    # cell for the flux kernel, U in and the face states out for the linear pass
+    nbytes = bytes_linear(nq) if dom == "linear" else BYTES_STAGE
+    roof["hbm"] = {"achieved": nbytes * ncell / t / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": nbytes * ncell / t / 1e9 / PEAK_HBM_GBS, "bytes_per_cell": nbytes,
                    "traffic_GBps": (tr[0] / t / 1e9) if tr else None}
     cn = COUNTERS.get(dom) if n == 256 and nq == 2 else None
     if cn:

@@ -48,3 +48,22 @@ def test_roofline_of_the_dominant_kernel():
     assert abs(r["path"]["flop_per_zone_update"] - (lin + 0.05 * (weno - lin))) < 1e-9
     # another size: no counter figure for the traffic
     assert bench._roofline(kern, head, 128 ** 3, 128, 2)["traffic"] is None
+
+
+def test_roofline_when_the_linear_pass_dominates():
+    """The linear pass and the flux kernel take about the same time; when the
+    linear pass takes more, its own algorithmic flops (the 24-point
+    contraction of five variables over 33 cells) and bytes (U in, the face
+    states out: 40 + 960 B per cell at NQ=2) are priced."""
+    import bench
+    head = {"weno_fraction": 0.04, "value": 1000.0}
+    kern = {"linear": {"ms_total": 21.0, "ms_avg": 2.1},
+            "flux_rk": {"ms_total": 20.0, "ms_avg": 2.0}}
+    r = bench._roofline(kern, head, 256 ** 3, 256, 2)
+    assert r["kernel"] == "linear"
+    assert bench.bytes_linear(2) == 40 + 24 * 5 * 8 == 1000
+    assert abs(r["achieved"] - 7920 * 256 ** 3 / 2.1e-3 / 1e12) < 1e-12
+    assert abs(r["hbm"]["achieved"] - 1000 * 256 ** 3 / 2.1e-3 / 1e9) < 1e-9
+    assert r["traffic"] == bench.TRAFFIC["linear"][0]
+    assert abs(r["fp64_counters"]["tflops"] - bench.COUNTERS["linear"]["fp64_flop_per_cell"] * 256 ** 3
+               / 2.1e-3 / 1e12) < 1e-9
