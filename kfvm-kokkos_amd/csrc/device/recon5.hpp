@@ -688,7 +688,6 @@ struct MfmaSub {
         const double a = wf[(M::fragOff(S) + k * M::RT + r) * 64 + lane];
         acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[r], 0, 0, 0);
       }
-      __builtin_amdgcn_sched_barrier(0);  // bounded registers: one column tile in flight
     }
   }
 };
