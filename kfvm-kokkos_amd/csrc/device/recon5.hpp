@@ -722,7 +722,6 @@ __global__ __launch_bounds__(BLK, 2) void k_wenorec5_mfma(const double *__restri
                                                         const int *__restrict__ counter,
                                                         double *__restrict__ WS, int cap) {
   using M = WenoMfma<D, NQ>;
-  constexpr auto ST = make_stencil_tables<D>();
   constexpr int C = M::C, I = M::I, nSub = M::nSub, FQ = M::FQ, RT = M::RT, CP = M::CP;
   constexpr int SUB = WenoLds<D, NQ>::SUB;
   constexpr double gHi = 0.7, gLo = 0.7;
@@ -772,16 +771,16 @@ __global__ __launch_bounds__(BLK, 2) void k_wenorec5_mfma(const double *__restri
 #pragma unroll
     for (int w = 0; w < 5; w++) ref[w] = U[size_t(w) * nc + c];
     const LinPrim lp(ref, g.gamma);
-    // linearised values of all five variables at this lane's stencil rows
-    // (grp, grp + 4, ...), loaded together once per group of cells
-    // (the offsets are unpacked again per group of cells: kept unpacked across
-    // the loop they would take 27 registers and spill)
+    // the stencil offsets are unpacked again per group of cells: kept
+    // unpacked across the loop they would take 27 registers and spill
     unsigned rw[MfmaRows<D>::NW];
 #pragma unroll
     for (int q = 0; q < MfmaRows<D>::NW; q++) {
       rw[q] = rowWord[q];
       asm volatile("" : "+v"(rw[q]));
     }
+    // linearised values of all five variables at this lane's stencil rows
+    // (grp, grp + 4, ...), loaded together once per group of cells
     double lv[5][NJ];
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
